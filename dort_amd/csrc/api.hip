@@ -1,0 +1,619 @@
+// api.hip -- the C-ABI of include/dortgpu.h: contexts, scene upload, intersection
+// queries and the wavefront render loop.
+//
+// The render loop replaces PathRenderer::render (src/dort/path_renderer.cpp:11-21)
+// and Renderer::iteration_tiled (src/dort/renderer.cpp:26-86): instead of tiles x
+// iterations on a thread pool, the paths of [sample_begin, sample_end) x pixels are
+// processed in chunks of `queue_tile` paths, each chunk running the bounce loop
+// raygen -> (extend -> shade -> connect) x (max_depth + 1) -> finalize on one stream.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "bvh_build.hpp"
+#include "dgpu_internal.hpp"
+
+using namespace dgpu;
+
+namespace {
+
+thread_local std::string g_err;
+
+dgpu_status fail(dgpu_status st, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return st;
+}
+
+#define HIP_TRY(x)                                                                   \
+  do {                                                                               \
+    hipError_t e_ = (x);                                                             \
+    if (e_ != hipSuccess) return fail(DGPU_EDEVICE, "%s failed: %s", #x, hipGetErrorString(e_)); \
+  } while (0)
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+}  // namespace
+
+struct dgpu_ctx {
+  int device = 0;
+  int num_cus = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::mutex mu;
+  std::map<uint64_t, uint32_t> grid_cache;
+  // render scratch, grown on demand
+  std::map<std::string, DevBuf> bufs;
+
+  ~dgpu_ctx() {
+    (void)hipSetDevice(device);
+    for (auto& kv : bufs) if (kv.second.p) (void)hipFree(kv.second.p);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  // Returns a device buffer of at least `bytes` under `name` (reused across calls).
+  void* buf(const std::string& name, size_t bytes, dgpu_status* st) {
+    DevBuf& b = bufs[name];
+    if (b.bytes >= bytes && b.p) return b.p;
+    if (b.p) { (void)hipFree(b.p); b.p = nullptr; b.bytes = 0; }
+    size_t want = std::max<size_t>(bytes, 256);
+    if (hipMalloc(&b.p, want) != hipSuccess) {
+      b.p = nullptr;
+      *st = fail(DGPU_ENOMEM, "hipMalloc(%zu) failed for %s", want, name.c_str());
+      return nullptr;
+    }
+    b.bytes = want;
+    return b.p;
+  }
+
+  uint32_t grid_for(int which, const KernelCfg& c) {
+    uint64_t key = ((uint64_t)which << 48) | ((uint64_t)c.width << 32) | ((uint64_t)c.lds << 16) | c.block;
+    auto it = grid_cache.find(key);
+    if (it != grid_cache.end()) return it->second;
+    int nb = 0;
+    const void* k = trace_kernel_ptr(which, c.width, c.lds, c.block);
+    if (!k || hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, (int)c.block, 0) != hipSuccess || nb <= 0)
+      nb = 4;
+    uint32_t g = (uint32_t)nb * (uint32_t)num_cus;
+    grid_cache[key] = g;
+    return g;
+  }
+};
+
+struct dgpu_scene {
+  dgpu_ctx* ctx = nullptr;
+  DevScene S;
+  std::vector<void*> allocs;
+  KernelCfg cfg;
+  uint32_t queue_tile = 0;
+  uint32_t stack_cap = MAX_STACK;
+  uint32_t num_area = 0;
+  dgpu_scene_info info;
+
+  ~dgpu_scene() {
+    if (ctx) (void)hipSetDevice(ctx->device);
+    for (void* p : allocs) (void)hipFree(p);
+  }
+
+  template <class T>
+  dgpu_status upload(const T* host, size_t count, const T** out) {
+    *out = nullptr;
+    if (count == 0) return DGPU_OK;
+    void* d = nullptr;
+    if (hipMalloc(&d, count * sizeof(T)) != hipSuccess) return fail(DGPU_ENOMEM, "hipMalloc(%zu) failed", count * sizeof(T));
+    allocs.push_back(d);
+    HIP_TRY(hipMemcpy(d, host, count * sizeof(T), hipMemcpyHostToDevice));
+    *out = (const T*)d;
+    return DGPU_OK;
+  }
+};
+
+namespace {
+
+dgpu_status validate_desc(const dgpu_scene_desc* d) {
+  if (!d) return fail(DGPU_EINVAL, "null scene descriptor");
+  if (d->num_triangles == 0) return fail(DGPU_EINVAL, "scene has no triangles");
+  if (!d->positions || !d->indices || !d->tri_material)
+    return fail(DGPU_EINVAL, "positions, indices and tri_material are required");
+  if (d->num_materials == 0 || !d->materials) return fail(DGPU_EINVAL, "scene has no materials");
+  for (size_t i = 0; i < 3 * (size_t)d->num_triangles; ++i)
+    if (d->indices[i] >= d->num_vertices) return fail(DGPU_EINVAL, "index %zu out of range", i);
+  for (uint32_t t = 0; t < d->num_triangles; ++t)
+    if (d->tri_material[t] >= d->num_materials) return fail(DGPU_EINVAL, "triangle %u: bad material", t);
+  for (size_t i = 0; i < 3 * (size_t)d->num_vertices; ++i)
+    if (!std::isfinite(d->positions[i])) return fail(DGPU_EINVAL, "non-finite vertex coordinate");
+  for (uint32_t m = 0; m < d->num_materials; ++m) {
+    const dgpu_material& mt = d->materials[m];
+    if (mt.type > DGPU_MAT_DIELECTRIC) return fail(DGPU_EUNSUPPORTED, "material %u: unsupported type %u", m, mt.type);
+    if (mt.type == DGPU_MAT_LAMBERT && mt.albedo_tex >= (int32_t)d->num_textures)
+      return fail(DGPU_EINVAL, "material %u: texture %d out of range", m, mt.albedo_tex);
+    if (mt.type == DGPU_MAT_GGX_CONDUCTOR && !(mt.roughness > 0.f))
+      return fail(DGPU_EINVAL, "material %u: GGX roughness must be > 0", m);
+  }
+  for (uint32_t l = 0; l < d->num_area_lights; ++l)
+    if (d->area_lights[l].triangle >= d->num_triangles) return fail(DGPU_EINVAL, "area light %u: bad triangle", l);
+  if (d->env && (!d->env->rgb || d->env->width == 0 || d->env->height == 0))
+    return fail(DGPU_EINVAL, "environment map without pixels");
+  for (uint32_t t = 0; t < d->num_textures; ++t)
+    if (!d->textures[t].rgb || d->textures[t].width == 0 || d->textures[t].height == 0)
+      return fail(DGPU_EINVAL, "texture %u without texels", t);
+  if (!(d->camera.fov > 0.f && d->camera.fov < 3.14159f)) return fail(DGPU_EINVAL, "bad camera fov");
+  return DGPU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+uint32_t dgpu_abi_version(void) { return DGPU_ABI_VERSION; }
+
+const char* dgpu_last_error(void) { return g_err.c_str(); }
+
+dgpu_status dgpu_create(int device, dgpu_ctx** out) {
+  if (!out) return fail(DGPU_EINVAL, "null output");
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail(DGPU_EDEVICE, "no HIP device");
+  if (device < 0 || device >= n) return fail(DGPU_EINVAL, "device %d out of range (%d devices)", device, n);
+  dgpu_ctx* c = new dgpu_ctx();
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess) { delete c; return fail(DGPU_EDEVICE, "hipSetDevice failed"); }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) { delete c; return fail(DGPU_EDEVICE, "hipGetDeviceProperties failed"); }
+  c->num_cus = prop.multiProcessorCount;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+    delete c;
+    return fail(DGPU_EDEVICE, "stream/event creation failed");
+  }
+  *out = c;
+  return DGPU_OK;
+}
+
+void dgpu_destroy(dgpu_ctx* ctx) { delete ctx; }
+
+dgpu_status dgpu_scene_create(dgpu_ctx* ctx, const dgpu_scene_desc* d, const dgpu_bvh_opts* opts,
+                              dgpu_scene** out) {
+  if (!ctx || !out) return fail(DGPU_EINVAL, "null context or output");
+  *out = nullptr;
+  dgpu_status st = validate_desc(d);
+  if (st != DGPU_OK) return st;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HIP_TRY(hipSetDevice(ctx->device));
+  dgpu_bvh_opts o{};
+  if (opts) o = *opts;
+  KernelCfg cfg;
+  cfg.width = o.width ? o.width : 2;
+  cfg.lds = o.stack_lds ? o.stack_lds : 16;
+  cfg.block = o.block_size ? o.block_size : 128;
+  cfg.grid = 0;
+  if (!variant_supported(cfg.width, cfg.lds, cfg.block))
+    return fail(DGPU_EUNSUPPORTED, "no kernel variant for width %u, stack %u, block %u", cfg.width, cfg.lds, cfg.block);
+
+  auto t0 = std::chrono::steady_clock::now();
+  dgpu_host::BvhParams bp;
+  bp.width = cfg.width;
+  bp.max_leaf_size = o.max_leaf_size ? o.max_leaf_size : 4;
+  bp.sah_bins = o.sah_bins ? o.sah_bins : 16;
+  if (bp.max_leaf_size > 16) return fail(DGPU_EINVAL, "max_leaf_size must be <= 16");
+  dgpu_host::BuiltBvh bvh;
+  try {
+    bvh = dgpu_host::build_bvh(d->positions, d->num_vertices, d->indices, d->num_triangles, bp);
+  } catch (const std::exception& e) {
+    return fail(DGPU_EINVAL, "BVH build failed: %s", e.what());
+  }
+  auto t1 = std::chrono::steady_clock::now();
+
+  dgpu_scene* s = new dgpu_scene();
+  s->ctx = ctx;
+  s->cfg = cfg;
+  s->queue_tile = o.queue_tile;
+  DevScene& S = s->S;
+  memset(&S, 0, sizeof S);
+  S.width = bvh.width;
+  S.num_tris = d->num_triangles;
+  s->stack_cap = (bvh.width - 1) * (bvh.depth + 1) + 2;
+  if (s->stack_cap < cfg.lds) s->stack_cap = cfg.lds;
+  S.stack_cap = s->stack_cap;
+#define UP(host, count, dst)                                                  \
+  do {                                                                        \
+    dgpu_status us_ = s->upload(host, count, &dst);                           \
+    if (us_ != DGPU_OK) { delete s; return us_; }                             \
+  } while (0)
+  UP((const float4*)bvh.nodes.data(), bvh.nodes.size() / 4, S.nodes);
+  UP((const float4*)bvh.tris.data(), bvh.tris.size() / 4, S.tris);
+  UP(d->tri_material, (size_t)d->num_triangles, S.tri_mat);
+
+  // scene bounds (Scene::bounds -> centroid / radius, lua_builder.cpp:124-128)
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (size_t t = 0; t < 3 * (size_t)d->num_triangles; ++t) {
+    const float* p = d->positions + 3 * (size_t)d->indices[t];
+    for (int k = 0; k < 3; ++k) { lo[k] = std::min(lo[k], p[k]); hi[k] = std::max(hi[k], p[k]); }
+  }
+  float radius = 0.5f * std::sqrt((hi[0] - lo[0]) * (hi[0] - lo[0]) + (hi[1] - lo[1]) * (hi[1] - lo[1]) +
+                                  (hi[2] - lo[2]) * (hi[2] - lo[2]));
+
+  // per-triangle uvs
+  if (d->uvs) {
+    std::vector<float2> tuv(3 * (size_t)d->num_triangles);
+    for (size_t t = 0; t < 3 * (size_t)d->num_triangles; ++t) {
+      uint32_t v = d->indices[t];
+      tuv[t] = make_float2(d->uvs[2 * (size_t)v], d->uvs[2 * (size_t)v + 1]);
+    }
+    UP(tuv.data(), tuv.size(), S.tri_uv);
+  }
+  // materials
+  std::vector<DevMaterial> mats(d->num_materials);
+  for (uint32_t m = 0; m < d->num_materials; ++m) {
+    const dgpu_material& a = d->materials[m];
+    DevMaterial& b = mats[m];
+    memset(&b, 0, sizeof b);
+    b.type = a.type;
+    b.tex = a.type == DGPU_MAT_LAMBERT ? a.albedo_tex : -1;
+    b.roughness = a.roughness;
+    b.ior_in = a.ior_inside;
+    b.albedo = make_float4(a.albedo[0], a.albedo[1], a.albedo[2], a.ior_outside);
+    b.eta = make_float4(a.eta[0], a.eta[1], a.eta[2], 0.f);
+    b.k = make_float4(a.k[0], a.k[1], a.k[2], 0.f);
+    b.refl = make_float4(a.reflect_tint[0], a.reflect_tint[1], a.reflect_tint[2], 0.f);
+    b.trans = make_float4(a.transmit_tint[0], a.transmit_tint[1], a.transmit_tint[2], 0.f);
+  }
+  UP(mats.data(), mats.size(), S.mats);
+  S.num_mats = d->num_materials;
+
+  // lights, in dort's add order: area lights, environment, infinite
+  std::vector<DevLight> lights;
+  std::vector<int32_t> tri_light(d->num_triangles, -1);
+  std::vector<float> powers;
+  for (uint32_t l = 0; l < d->num_area_lights; ++l) {
+    const dgpu_area_light& al = d->area_lights[l];
+    const float* p0 = d->positions + 3 * (size_t)d->indices[3 * (size_t)al.triangle];
+    const float* p1 = d->positions + 3 * (size_t)d->indices[3 * (size_t)al.triangle + 1];
+    const float* p2 = d->positions + 3 * (size_t)d->indices[3 * (size_t)al.triangle + 2];
+    float e1[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
+    float e2[3] = {p2[0] - p0[0], p2[1] - p0[1], p2[2] - p0[2]};
+    float cx = e1[1] * e2[2] - e1[2] * e2[1], cy = e1[2] * e2[0] - e1[0] * e2[2], cz = e1[0] * e2[1] - e1[1] * e2[0];
+    float area = 0.5f * std::sqrt(cx * cx + cy * cy + cz * cz);
+    DevLight L;
+    memset(&L, 0, sizeof L);
+    L.kind = LK_AREA;
+    L.tri = bvh.tri_leaf_of[al.triangle];
+    L.area = area;
+    L.radiance = make_float4(al.radiance[0], al.radiance[1], al.radiance[2], 0.f);
+    tri_light[al.triangle] = (int32_t)lights.size();
+    lights.push_back(L);
+    // DiffuseLight::approximate_power (diffuse_light.cpp:117-119)
+    powers.push_back((area * al.radiance[0] + area * al.radiance[1] + area * al.radiance[2]) / 3.f);
+  }
+  S.env_light = -1;
+  if (d->env) {
+    const dgpu_env_light& e = *d->env;
+    uint32_t w = e.width, h = e.height;
+    S.env_w = w;
+    S.env_h = h;
+    // frame (environment_light.cpp:6-13)
+    V3 up = v3(e.up[0], e.up[1], e.up[2]), fwd = v3(e.fwd[0], e.fwd[1], e.fwd[2]);
+    S.env_up = normalize(up);
+    S.env_t = normalize(cross(up, fwd));
+    S.env_s = cross(S.env_up, S.env_t);
+    std::vector<float4> rgb((size_t)w * h);
+    std::vector<float> values((size_t)w * h);
+    double vs[3] = {0, 0, 0};
+    float vsum[3] = {0.f, 0.f, 0.f}, wsum = 0.f;
+    (void)vs;
+    for (uint32_t y = 0; y < h; ++y) {
+      float theta = (float(y) + 0.5f) / float(h) * PI_F;
+      float ct = clampf(std::cos(theta), 0.f, 1.f);
+      for (uint32_t x = 0; x < w; ++x) {
+        const float* px = e.rgb + 3 * ((size_t)y * w + x);
+        rgb[(size_t)y * w + x] = make_float4(px[0] * e.scale[0], px[1] * e.scale[1], px[2] * e.scale[2], 0.f);
+        values[(size_t)y * w + x] = std::fabs((px[0] + px[1] + px[2]) / 3.f * ct);
+        for (int k = 0; k < 3; ++k) vsum[k] += px[k] * ct;
+        wsum += ct;
+      }
+    }
+    // PiecewiseDistrib2d (piecewise_distrib_2d.cpp:5-34)
+    std::vector<float> pdfs((size_t)w * h), marg(h + 1), cond((size_t)(w + 1) * h);
+    float sum = 0.f;
+    marg[0] = 0.f;
+    for (uint32_t y = 0; y < h; ++y) {
+      float ys = 0.f;
+      cond[(size_t)(w + 1) * y] = 0.f;
+      for (uint32_t x = 0; x < w; ++x) {
+        ys += values[(size_t)y * w + x];
+        cond[(size_t)(w + 1) * y + x + 1] = ys;
+      }
+      float inv = 1.f / ys;
+      for (uint32_t x = 0; x <= w; ++x) cond[(size_t)(w + 1) * y + x] *= inv;
+      sum += ys;
+      marg[y + 1] = sum;
+    }
+    float inv_sum = 1.f / sum;
+    for (uint32_t y = 0; y <= h; ++y) marg[y] *= inv_sum;
+    for (size_t i = 0; i < (size_t)w * h; ++i) pdfs[i] = values[i] * inv_sum;
+    if (!(sum > 0.f)) { delete s; return fail(DGPU_EINVAL, "environment map has no energy in the upper hemisphere"); }
+    UP(rgb.data(), rgb.size(), S.env_rgb);
+    UP(marg.data(), marg.size(), S.env_marg);
+    UP(cond.data(), cond.size(), S.env_cond);
+    UP(pdfs.data(), pdfs.size(), S.env_pdf);
+    DevLight L;
+    memset(&L, 0, sizeof L);
+    L.kind = LK_ENV;
+    S.env_light = (int32_t)lights.size();
+    lights.push_back(L);
+    // EnvironmentLight::approximate_power (environment_light.cpp:130-132)
+    float avg[3];
+    for (int k = 0; k < 3; ++k) avg[k] = e.scale[k] * vsum[k] / wsum;
+    float a = PI_F * radius * radius;
+    powers.push_back((a * avg[0] + a * avg[1] + a * avg[2]) / 3.f);
+  }
+  if (d->infinite_radiance) {
+    DevLight L;
+    memset(&L, 0, sizeof L);
+    L.kind = LK_INF;
+    L.radiance = make_float4(d->infinite_radiance[0], d->infinite_radiance[1], d->infinite_radiance[2], 0.f);
+    S.has_inf = 1;
+    S.inf_rad = v3(d->infinite_radiance[0], d->infinite_radiance[1], d->infinite_radiance[2]);
+    lights.push_back(L);
+    float a = PI_F * radius * radius;  // InfiniteLight::approximate_power
+    powers.push_back((a * L.radiance.x + a * L.radiance.y + a * L.radiance.z) / 3.f);
+  }
+  // DiscreteDistrib1d over light powers (scene.cpp:20-28, discrete_distrib_1d.cpp:4-20)
+  std::vector<float> cdf;
+  float acc = 0.f;
+  for (float p : powers) { cdf.push_back(acc); acc += p; }
+  cdf.push_back(acc);
+  float inv = acc == 0.f ? 0.f : 1.f / acc;
+  for (float& c : cdf) c *= inv;
+  S.num_lights = (uint32_t)lights.size();
+  s->num_area = d->num_area_lights;
+  if (!lights.empty()) {
+    UP(lights.data(), lights.size(), S.lights);
+    UP(cdf.data(), cdf.size(), S.light_cdf);
+  }
+  UP(tri_light.data(), tri_light.size(), S.tri_light);
+  // textures
+  if (d->num_textures) {
+    std::vector<float4> texels;
+    std::vector<uint4> info;
+    for (uint32_t t = 0; t < d->num_textures; ++t) {
+      const dgpu_texture& tx = d->textures[t];
+      info.push_back(make_uint4(tx.width, tx.height, (uint32_t)texels.size(), 0));
+      for (size_t i = 0; i < (size_t)tx.width * tx.height; ++i)
+        texels.push_back(make_float4(tx.rgb[3 * i], tx.rgb[3 * i + 1], tx.rgb[3 * i + 2], 0.f));
+    }
+    UP(texels.data(), texels.size(), S.tex_data);
+    UP(info.data(), info.size(), S.tex_info);
+    S.num_tex = d->num_textures;
+  }
+#undef UP
+  for (int i = 0; i < 12; ++i) S.c2w[i] = d->camera.camera_to_world[i];
+  S.project_dim = 2.f * std::tan(0.5f * d->camera.fov);
+  auto t2 = std::chrono::steady_clock::now();
+
+  s->info.nodes = bvh.num_nodes;
+  s->info.leaves = bvh.num_leaves;
+  s->info.depth = bvh.depth;
+  s->info.width = bvh.width;
+  s->info.node_bytes = (uint64_t)bvh.nodes.size() * 4;
+  s->info.triangle_bytes = (uint64_t)bvh.tris.size() * 4;
+  s->info.build_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  s->info.upload_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
+  *out = s;
+  return DGPU_OK;
+}
+
+void dgpu_scene_destroy(dgpu_scene* s) { delete s; }
+
+dgpu_status dgpu_scene_get_info(const dgpu_scene* s, dgpu_scene_info* out) {
+  if (!s || !out) return fail(DGPU_EINVAL, "null scene or output");
+  *out = s->info;
+  return DGPU_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+dgpu_status trace_common(dgpu_scene* s, const dgpu_rays* rays, uint32_t n, dgpu_hits* hits,
+                         uint32_t* occluded, uint32_t flags, double* kernel_ms) {
+  if (!s || !rays) return fail(DGPU_EINVAL, "null scene or rays");
+  if (!occluded && (!hits || !hits->t || !hits->prim || !hits->b1 || !hits->b2))
+    return fail(DGPU_EINVAL, "null hit buffers");
+  const void* rp[8] = {rays->ox, rays->oy, rays->oz, rays->dx, rays->dy, rays->dz, rays->tmin, rays->tmax};
+  for (const void* p : rp) if (!p) return fail(DGPU_EINVAL, "null ray component");
+  if (kernel_ms) *kernel_ms = 0.0;
+  if (n == 0) return DGPU_OK;
+  dgpu_ctx* ctx = s->ctx;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HIP_TRY(hipSetDevice(ctx->device));
+  dgpu_status st = DGPU_OK;
+  KernelCfg cfg = s->cfg;
+  cfg.grid = ctx->grid_for(occluded ? 1 : 0, cfg);
+  uint32_t* fetch = (uint32_t*)ctx->buf("fetch", 64, &st);
+  uint32_t* spill = (uint32_t*)ctx->buf("spill", (size_t)cfg.grid * cfg.block * s->stack_cap * 4, &st);
+  if (st != DGPU_OK) return st;
+  dgpu_rays dr = *rays;
+  dgpu_hits dh{nullptr, nullptr, nullptr, nullptr};
+  uint32_t* docc = occluded;
+  bool dev = flags & DGPU_BUF_DEVICE;
+  hipStream_t stream = ctx->stream;
+  if (!dev) {
+    float* in = (float*)ctx->buf("trace_in", (size_t)n * 8 * 4, &st);
+    float* outb = (float*)ctx->buf("trace_out", (size_t)n * 4 * 4, &st);
+    if (st != DGPU_OK) return st;
+    const float* comps[8] = {rays->ox, rays->oy, rays->oz, rays->dx, rays->dy, rays->dz, rays->tmin, rays->tmax};
+    const float** dst[8] = {&dr.ox, &dr.oy, &dr.oz, &dr.dx, &dr.dy, &dr.dz, &dr.tmin, &dr.tmax};
+    for (int k = 0; k < 8; ++k) {
+      HIP_TRY(hipMemcpyAsync(in + (size_t)k * n, comps[k], (size_t)n * 4, hipMemcpyHostToDevice, stream));
+      *dst[k] = in + (size_t)k * n;
+    }
+    if (occluded) docc = (uint32_t*)outb;
+    else dh = dgpu_hits{outb, (int32_t*)(outb + n), outb + 2 * (size_t)n, outb + 3 * (size_t)n};
+  } else if (hits) {
+    dh = *hits;
+  }
+  HIP_TRY(hipEventRecord(ctx->ev0, stream));
+  HIP_TRY(launch_trace_api(s->S, cfg, dr, occluded ? nullptr : &dh, docc, n, fetch, spill, stream));
+  HIP_TRY(hipEventRecord(ctx->ev1, stream));
+  if (!dev) {
+    if (occluded) {
+      HIP_TRY(hipMemcpyAsync(occluded, docc, (size_t)n * 4, hipMemcpyDeviceToHost, stream));
+    } else {
+      HIP_TRY(hipMemcpyAsync(hits->t, dh.t, (size_t)n * 4, hipMemcpyDeviceToHost, stream));
+      HIP_TRY(hipMemcpyAsync(hits->prim, dh.prim, (size_t)n * 4, hipMemcpyDeviceToHost, stream));
+      HIP_TRY(hipMemcpyAsync(hits->b1, dh.b1, (size_t)n * 4, hipMemcpyDeviceToHost, stream));
+      HIP_TRY(hipMemcpyAsync(hits->b2, dh.b2, (size_t)n * 4, hipMemcpyDeviceToHost, stream));
+    }
+  }
+  HIP_TRY(hipStreamSynchronize(stream));
+  if (kernel_ms) {
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    *kernel_ms = ms;
+  }
+  return DGPU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+dgpu_status dgpu_trace_closest(dgpu_scene* s, const dgpu_rays* rays, uint32_t n, dgpu_hits* out,
+                               uint32_t flags, double* kernel_ms) {
+  if (!out) return fail(DGPU_EINVAL, "null hits");
+  return trace_common(s, rays, n, out, nullptr, flags, kernel_ms);
+}
+
+dgpu_status dgpu_trace_any(dgpu_scene* s, const dgpu_rays* rays, uint32_t n, uint32_t* occluded,
+                           uint32_t flags, double* kernel_ms) {
+  if (!occluded) return fail(DGPU_EINVAL, "null occluded buffer");
+  return trace_common(s, rays, n, nullptr, occluded, flags, kernel_ms);
+}
+
+dgpu_status dgpu_render(dgpu_scene* s, const dgpu_render_opts* o, float* film_rgb_sum,
+                        float* film_weight, dgpu_stats* out_stats, const volatile int* cancel) {
+  if (!s || !o) return fail(DGPU_EINVAL, "null scene or options");
+  if (!film_rgb_sum || !film_weight) return fail(DGPU_EINVAL, "null film buffers");
+  if (o->x_res == 0 || o->y_res == 0) return fail(DGPU_EINVAL, "empty film");
+  if (o->sample_end < o->sample_begin) return fail(DGPU_EINVAL, "sample_end < sample_begin");
+  if (o->max_depth + 2 >= MAX_BOUNCES) return fail(DGPU_EINVAL, "max_depth must be < %u", MAX_BOUNCES - 2);
+  if (o->direct_strategy > DGPU_DIRECT_LIGHT) return fail(DGPU_EINVAL, "bad direct_strategy");
+  if ((uint64_t)o->x_res * o->y_res > 0xFFFFFFFFull) return fail(DGPU_EINVAL, "film too large");
+  if (o->sample_all_lights && s->S.num_lights > MAX_SLOT_LIGHTS)
+    return fail(DGPU_EUNSUPPORTED, "sample_all_lights supports at most %u lights", MAX_SLOT_LIGHTS);
+  if (out_stats) memset(out_stats, 0, sizeof *out_stats);
+  dgpu_ctx* ctx = s->ctx;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HIP_TRY(hipSetDevice(ctx->device));
+  hipStream_t stream = ctx->stream;
+  dgpu_status st = DGPU_OK;
+  const uint32_t npix = o->x_res * o->y_res;
+  const uint64_t total = (uint64_t)npix * (o->sample_end - o->sample_begin);
+  uint32_t tile = s->queue_tile ? s->queue_tile : (1u << 21);
+  uint32_t C = (uint32_t)std::min<uint64_t>(total ? total : 1, tile);
+  uint32_t slots = o->sample_all_lights ? 2 * std::max(1u, s->S.num_lights) : 2;
+
+  KernelCfg ce = s->cfg, cc = s->cfg;
+  ce.grid = ctx->grid_for(2, ce);
+  cc.grid = ctx->grid_for(3, cc);
+  uint32_t maxgrid = std::max(ce.grid, cc.grid);
+  WaveParams P;
+  memset(&P, 0, sizeof P);
+  P.S = s->S;
+  P.xr = o->x_res; P.yr = o->y_res; P.npix = npix;
+  P.min_depth = o->min_depth; P.max_depth = o->max_depth; P.rr_depth = o->rr_depth;
+  P.only_direct = o->only_direct; P.sample_all = o->sample_all_lights; P.strategy = o->direct_strategy;
+  P.slots = slots;
+  dgpu_rng_key(o->seed, o->scene_id, &P.k0, &P.k1);
+  P.sample_begin = o->sample_begin;
+  P.thr = (float4*)ctx->buf("thr", (size_t)C * 16, &st);
+  P.rad = (float4*)ctx->buf("rad", (size_t)C * 16, &st);
+  P.pend = (float4*)ctx->buf("pend", (size_t)C * slots * 16, &st);
+  P.vis = (uint8_t*)ctx->buf("vis", (size_t)C * slots, &st);
+  P.qo[0] = (float4*)ctx->buf("qo0", (size_t)C * 16, &st);
+  P.qo[1] = (float4*)ctx->buf("qo1", (size_t)C * 16, &st);
+  P.qd[0] = (float4*)ctx->buf("qd0", (size_t)C * 16, &st);
+  P.qd[1] = (float4*)ctx->buf("qd1", (size_t)C * 16, &st);
+  P.qpath[0] = (uint32_t*)ctx->buf("qp0", (size_t)C * 4, &st);
+  P.qpath[1] = (uint32_t*)ctx->buf("qp1", (size_t)C * 4, &st);
+  P.hit = (float4*)ctx->buf("hit", (size_t)C * 16, &st);
+  P.so = (float4*)ctx->buf("so", (size_t)C * slots * 16, &st);
+  P.sd = (float4*)ctx->buf("sd", (size_t)C * slots * 16, &st);
+  P.smeta = (uint2*)ctx->buf("smeta", (size_t)C * slots * 8, &st);
+  P.counters = (uint32_t*)ctx->buf("counters", NUM_COUNTERS * 4, &st);
+  P.stats = (unsigned long long*)ctx->buf("stats", 64, &st);
+  P.spill = (uint32_t*)ctx->buf("spill", (size_t)maxgrid * s->cfg.block * s->stack_cap * 4, &st);
+  bool dev = o->flags & DGPU_BUF_DEVICE;
+  if (dev) {
+    P.film_rgb = film_rgb_sum;
+    P.film_w = film_weight;
+  } else {
+    P.film_rgb = (float*)ctx->buf("film_rgb", (size_t)npix * 12, &st);
+    P.film_w = (float*)ctx->buf("film_w", (size_t)npix * 4, &st);
+  }
+  if (st != DGPU_OK) return st;
+  HIP_TRY(hipMemsetAsync(P.stats, 0, 64, stream));
+  if (!dev) {
+    HIP_TRY(hipMemsetAsync(P.film_rgb, 0, (size_t)npix * 12, stream));
+    HIP_TRY(hipMemsetAsync(P.film_w, 0, (size_t)npix * 4, stream));
+  }
+  HIP_TRY(hipEventRecord(ctx->ev0, stream));
+  bool cancelled = false;
+  for (uint64_t g0 = 0; g0 < total && !cancelled; g0 += C) {
+    P.g0 = g0;
+    P.n_chunk = (uint32_t)std::min<uint64_t>(C, total - g0);
+    HIP_TRY(launch_raygen(P, stream));
+    for (uint32_t b = 0; b <= o->max_depth; ++b) {
+      if (cancel && *cancel) { cancelled = true; break; }
+      HIP_TRY(launch_extend(P, ce, b, stream));
+      HIP_TRY(launch_shade(P, b, stream));
+      HIP_TRY(launch_connect(P, cc, b, stream));
+    }
+    if (!cancelled) HIP_TRY(launch_finalize(P, stream));
+  }
+  HIP_TRY(hipEventRecord(ctx->ev1, stream));
+  unsigned long long hstats[4] = {0, 0, 0, 0};
+  HIP_TRY(hipMemcpyAsync(hstats, P.stats, sizeof hstats, hipMemcpyDeviceToHost, stream));
+  std::vector<float> hrgb, hw;
+  if (!dev && !cancelled) {
+    hrgb.resize((size_t)npix * 3);
+    hw.resize(npix);
+    HIP_TRY(hipMemcpyAsync(hrgb.data(), P.film_rgb, (size_t)npix * 12, hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipMemcpyAsync(hw.data(), P.film_w, (size_t)npix * 4, hipMemcpyDeviceToHost, stream));
+  }
+  HIP_TRY(hipStreamSynchronize(stream));
+  if (cancelled) return fail(DGPU_ECANCELLED, "render cancelled");
+  if (!dev) {
+    for (size_t i = 0; i < (size_t)npix * 3; ++i) film_rgb_sum[i] += hrgb[i];
+    for (size_t i = 0; i < npix; ++i) film_weight[i] += hw[i];
+  }
+  if (out_stats) {
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    out_stats->paths = hstats[0];
+    out_stats->extension_rays = hstats[1];
+    out_stats->shadow_rays = hstats[2];
+    out_stats->dropped_samples = hstats[3];
+    out_stats->kernel_ms = ms;
+  }
+  return DGPU_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,32 @@
+// bvh_build.hpp -- host BVH builder interface (see bvh_build.cpp).
+#pragma once
+#include <cstdint>
+#include <vector>
+
+namespace dgpu_host {
+
+struct BvhParams {
+  uint32_t width = 2;          // 2 or 4
+  uint32_t max_leaf_size = 4;  // 1..16
+  uint32_t sah_bins = 16;
+};
+
+struct BuiltBvh {
+  std::vector<float> nodes;          // 16 floats per BVH2 node, 32 per BVH4 node
+  std::vector<float> tris;           // 12 floats per triangle, leaf order
+  std::vector<uint32_t> tri_leaf_of; // scene triangle id -> leaf-order slot
+  uint32_t width = 2;
+  uint32_t depth = 0;
+  uint32_t num_nodes = 0;
+  uint32_t num_leaves = 0;
+};
+
+// Leaf child code: bit 31 set, bits 27..30 = count-1, bits 0..26 = first triangle.
+inline uint32_t leaf_code(uint32_t first, uint32_t count) {
+  return 0x80000000u | ((count - 1u) << 27) | first;
+}
+
+BuiltBvh build_bvh(const float* positions, uint32_t num_vertices, const uint32_t* indices,
+                   uint32_t num_tris, const BvhParams& prm);
+
+}  // namespace dgpu_host

@@ -1,0 +1,64 @@
+// dgpu_internal.hpp -- launch interface between the C-ABI (api.hip) and the kernels.
+#pragma once
+#include "dgpu_device.hpp"
+
+namespace dgpu {
+
+// Device counters of one wavefront chunk (zeroed by raygen).
+constexpr uint32_t MAX_BOUNCES = 64;
+constexpr uint32_t CNT_QUEUE = 0;        // [b]: extension rays of bounce b
+constexpr uint32_t CNT_SHADOW = 64;      // [b]: shadow rays of bounce b
+constexpr uint32_t CNT_FETCH_EXT = 128;  // persistent-kernel work counters
+constexpr uint32_t CNT_FETCH_SHD = 192;
+constexpr uint32_t NUM_COUNTERS = 256;
+constexpr uint32_t MAX_SLOT_LIGHTS = 8;  // sample_all_lights supports up to this many lights
+
+struct KernelCfg {
+  uint32_t width;  // BVH width
+  uint32_t lds;    // traversal stack entries in LDS per lane
+  uint32_t block;  // traversal workgroup size
+  uint32_t grid;   // persistent grid
+};
+
+struct WaveParams {
+  DevScene S;
+  uint32_t xr, yr, npix;
+  uint32_t min_depth, max_depth, rr_depth;
+  uint32_t only_direct, sample_all, strategy;
+  uint32_t slots;          // pending NEE slots per path (2 per light considered)
+  uint32_t k0, k1;         // RNG key
+  uint32_t sample_begin;
+  uint32_t n_chunk;        // paths in this chunk
+  uint64_t g0;             // first global path index of the chunk
+  float4* thr;             // throughput rgb + flags
+  float4* rad;             // radiance accumulated so far
+  float4* pend;            // pending NEE contributions [path * slots + k]
+  uint8_t* vis;            // 1 = pending contribution visible
+  float4* qo[2];           // extension queues: origin + tmin
+  float4* qd[2];           //                   direction + tmax
+  uint32_t* qpath[2];      //                   path slot
+  float4* hit;             // t, b1, b2, leaf-order triangle (bits)
+  float4* so;              // shadow queue: origin + tmin
+  float4* sd;              //               direction + tmax
+  uint2* smeta;            //               pending slot, ignored triangle
+  uint32_t* counters;
+  unsigned long long* stats;  // paths, extension rays, shadow rays, dropped samples
+  uint32_t* spill;
+  float* film_rgb;
+  float* film_w;
+};
+
+bool variant_supported(uint32_t width, uint32_t lds, uint32_t block);
+// which: 0 closest-hit API, 1 any-hit API, 2 extend, 3 connect
+const void* trace_kernel_ptr(int which, uint32_t width, uint32_t lds, uint32_t block);
+
+hipError_t launch_trace_api(const DevScene& S, const KernelCfg& cfg, const dgpu_rays& rays,
+                            const dgpu_hits* hits, uint32_t* occluded, uint32_t n, uint32_t* fetch,
+                            uint32_t* spill, hipStream_t st);
+hipError_t launch_raygen(const WaveParams& P, hipStream_t st);
+hipError_t launch_extend(const WaveParams& P, const KernelCfg& cfg, uint32_t bounce, hipStream_t st);
+hipError_t launch_shade(const WaveParams& P, uint32_t bounce, hipStream_t st);
+hipError_t launch_connect(const WaveParams& P, const KernelCfg& cfg, uint32_t bounce, hipStream_t st);
+hipError_t launch_finalize(const WaveParams& P, hipStream_t st);
+
+}  // namespace dgpu

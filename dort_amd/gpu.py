@@ -1,0 +1,150 @@
+"""ctypes binding of libdortgpu.so (include/dortgpu.h).
+
+This is the Python side of the drop-in boundary: the same entry points a C++
+host (dort's `Renderer` seam, include/dort/renderer.hpp:12-34) calls.  It fails
+loudly when the HIP library is missing; there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdortgpu.so")
+
+_lib = None
+
+EXPORTS = ["dgpu_abi_version", "dgpu_last_error", "dgpu_create", "dgpu_destroy", "dgpu_scene_create",
+           "dgpu_scene_destroy", "dgpu_scene_get_info", "dgpu_trace_closest", "dgpu_trace_any",
+           "dgpu_render"]
+
+
+class DgpuError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{abi.STATUS_NAMES.get(status, status)}: {msg}")
+        self.status = status
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m dort_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        L.dgpu_abi_version.restype = C.c_uint32
+        L.dgpu_last_error.restype = C.c_char_p
+        L.dgpu_create.argtypes = [C.c_int, C.POINTER(C.c_void_p)]
+        L.dgpu_destroy.argtypes = [C.c_void_p]
+        L.dgpu_scene_create.argtypes = [C.c_void_p, C.POINTER(abi.SceneDesc), C.POINTER(abi.BvhOpts),
+                                        C.POINTER(C.c_void_p)]
+        L.dgpu_scene_destroy.argtypes = [C.c_void_p]
+        L.dgpu_scene_get_info.argtypes = [C.c_void_p, C.POINTER(abi.SceneInfo)]
+        L.dgpu_trace_closest.argtypes = [C.c_void_p, C.POINTER(abi.Rays), C.c_uint32,
+                                         C.POINTER(abi.Hits), C.c_uint32, C.POINTER(C.c_double)]
+        L.dgpu_trace_any.argtypes = [C.c_void_p, C.POINTER(abi.Rays), C.c_uint32, C.c_void_p,
+                                     C.c_uint32, C.POINTER(C.c_double)]
+        L.dgpu_render.argtypes = [C.c_void_p, C.POINTER(abi.RenderOpts), C.c_void_p, C.c_void_p,
+                                  C.POINTER(abi.RenderStats), C.c_void_p]
+        for n in ("dgpu_create", "dgpu_scene_create", "dgpu_scene_get_info", "dgpu_trace_closest",
+                  "dgpu_trace_any", "dgpu_render"):
+            getattr(L, n).restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def check(status):
+    if status != abi.DGPU_OK:
+        raise DgpuError(status, lib().dgpu_last_error().decode())
+
+
+class Context:
+    def __init__(self, device=0):
+        self.h = C.c_void_p()
+        check(lib().dgpu_create(device, C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().dgpu_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        self.close()
+
+    def scene(self, scene, **bvh):
+        return Scene(self, scene, **bvh)
+
+
+class Scene:
+    """A scene uploaded to the GPU (BVH built on the host, flattened to the GPU layout)."""
+
+    def __init__(self, ctx, scene, width=2, max_leaf_size=4, sah_bins=16, stack_lds=16,
+                 block_size=128, queue_tile=0):
+        self.ctx = ctx
+        self.packed = scene.pack()
+        self.scene = scene
+        self.opts = abi.BvhOpts(width, max_leaf_size, sah_bins, stack_lds, block_size, queue_tile)
+        self.h = C.c_void_p()
+        check(lib().dgpu_scene_create(ctx.h, C.byref(self.packed.desc), C.byref(self.opts),
+                                      C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().dgpu_scene_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        self.close()
+
+    def info(self):
+        i = abi.SceneInfo()
+        check(lib().dgpu_scene_get_info(self.h, C.byref(i)))
+        return {f: getattr(i, f) for f, _ in abi.SceneInfo._fields_}
+
+    def trace_closest(self, rays, out=None, device=False):
+        """rays: dict of 8 SoA arrays (numpy on host or torch tensors on the GPU)."""
+        n = len(rays["ox"])
+        r = abi.rays_struct(rays)
+        if out is None:
+            out = {"t": np.zeros(n, np.float32), "prim": np.zeros(n, np.int32),
+                   "b1": np.zeros(n, np.float32), "b2": np.zeros(n, np.float32)}
+        ptr = lambda a: a.data_ptr() if hasattr(a, "data_ptr") else a.ctypes.data
+        h = abi.Hits(ptr(out["t"]), ptr(out["prim"]), ptr(out["b1"]), ptr(out["b2"]))
+        ms = C.c_double()
+        check(lib().dgpu_trace_closest(self.h, C.byref(r), n, C.byref(h),
+                                       abi.BUF_DEVICE if device else 0, C.byref(ms)))
+        out["kernel_ms"] = ms.value
+        return out
+
+    def trace_any(self, rays, out=None, device=False):
+        n = len(rays["ox"])
+        r = abi.rays_struct(rays)
+        if out is None:
+            out = np.zeros(n, np.uint32)
+        ptr = out.data_ptr() if hasattr(out, "data_ptr") else out.ctypes.data
+        ms = C.c_double()
+        check(lib().dgpu_trace_any(self.h, C.byref(r), n, ptr, abi.BUF_DEVICE if device else 0,
+                                   C.byref(ms)))
+        return out, ms.value
+
+    def render(self, opts, film_rgb=None, film_w=None, device=False, cancel=None):
+        """Adds samples to (film_rgb (H,W,3), film_w (H,W)); returns (film_rgb, film_w, stats)."""
+        if film_rgb is None:
+            film_rgb = np.zeros((opts.y_res, opts.x_res, 3), np.float32)
+            film_w = np.zeros((opts.y_res, opts.x_res), np.float32)
+        ptr = lambda a: a.data_ptr() if hasattr(a, "data_ptr") else a.ctypes.data
+        o = abi.RenderOpts.from_buffer_copy(opts)
+        if device:
+            o.flags |= abi.BUF_DEVICE
+        st = abi.RenderStats()
+        cptr = None
+        if cancel is not None:
+            cptr = cancel.ctypes.data
+        check(lib().dgpu_render(self.h, C.byref(o), ptr(film_rgb), ptr(film_w), C.byref(st), cptr))
+        stats = {f: getattr(st, f) for f, _ in abi.RenderStats._fields_}
+        return film_rgb, film_w, stats
+
+    def image(self, opts):
+        rgb, w, _ = self.render(opts)
+        return rgb / np.maximum(w, 1e-30)[..., None]

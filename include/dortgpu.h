@@ -1,0 +1,220 @@
+/*
+ * dortgpu.h -- C-ABI of the MI355X path-tracing core for dort.
+ *
+ * The library replaces the body of dort's CPU path tracer: `PathRenderer::render`
+ * (src/dort/path_renderer.cpp:11-21) and everything it calls per sample
+ * (scene intersection, BSDFs, lights, film accumulation).  The seam in dort is
+ * the abstract `Renderer` (include/dort/renderer.hpp:12-34), selected by
+ * `lua_render_make` (src/dort/lua_render.cpp:102-221).  A host adapter
+ * (`dort_amd/host/gpu_path_renderer.hpp`) flattens a dort-style scene into a
+ * `dgpu_scene_desc` and calls the functions below.
+ *
+ * Conventions
+ *  - C99, no exceptions cross this boundary.  Every call returns a dgpu_status;
+ *    `dgpu_last_error()` gives a thread-local message for the last failure.
+ *  - Handles are opaque and library-owned.  Descriptor arrays are host memory
+ *    and are copied; the caller keeps ownership.
+ *  - Ray/hit/film buffers are host or device pointers, chosen per call by
+ *    DGPU_BUF_DEVICE in `flags`.  Device pointers must be allocations on the
+ *    context's device (e.g. torch tensors on that device).
+ *  - One HIP stream per context; calls on one context are serialised by an
+ *    internal mutex.  Separate contexts are independent.
+ *  - Rendering is deterministic for a given (scene, opts.seed, sample range):
+ *    random numbers are drawn from a counter-based generator keyed by
+ *    (seed, pixel, sample, bounce, dimension), never by launch geometry.
+ */
+#ifndef DORTGPU_H
+#define DORTGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DGPU_ABI_VERSION 1u
+
+typedef enum {
+  DGPU_OK = 0,
+  DGPU_EINVAL = 1,       /* bad argument (null handle, zero size, bad index ...) */
+  DGPU_ENOMEM = 2,       /* device or host allocation failed */
+  DGPU_EDEVICE = 3,      /* HIP runtime error (no device, launch failure ...) */
+  DGPU_EUNSUPPORTED = 4, /* scene feature not supported by the GPU path */
+  DGPU_ECANCELLED = 5    /* cancel flag observed between bounces */
+} dgpu_status;
+
+typedef struct dgpu_ctx dgpu_ctx;
+typedef struct dgpu_scene dgpu_scene;
+
+/* ---- scene description ------------------------------------------------- */
+
+enum {
+  DGPU_MAT_NONE = 0,          /* no BSDF (e.g. the black Lambert dort puts on area lights) */
+  DGPU_MAT_LAMBERT = 1,       /* lambert_material.cpp:5-35, two-sided */
+  DGPU_MAT_GGX_CONDUCTOR = 2, /* GGX microfacet conductor (no dort counterpart) */
+  DGPU_MAT_DIELECTRIC = 3     /* dielectric_material.cpp:32-104, delta reflect/refract */
+};
+
+typedef struct {
+  uint32_t type;           /* DGPU_MAT_* */
+  int32_t albedo_tex;      /* Lambert: texture index (uv lookup) or -1 for constant albedo */
+  float albedo[3];         /* Lambert albedo / conductor tint */
+  float roughness;         /* GGX alpha (no remapping, microfacet.cpp:10-12) */
+  float eta[3];            /* conductor complex IOR, real part (per RGB channel) */
+  float k[3];              /* conductor complex IOR, imaginary part */
+  float ior_inside;        /* dielectric */
+  float ior_outside;       /* dielectric (dort's Lua default is 1, lua_material.cpp:118) */
+  float reflect_tint[3];   /* dielectric */
+  float transmit_tint[3];  /* dielectric */
+} dgpu_material;
+
+typedef struct {
+  uint32_t triangle;       /* index into the scene triangles (the emitting shape) */
+  float radiance[3];       /* one-sided emitted radiance, diffuse_light.cpp:83-93 */
+} dgpu_area_light;
+
+typedef struct {
+  uint32_t width, height;  /* lat-long map, rows = theta from `up` (environment_light.cpp:163-186) */
+  const float* rgb;        /* width*height*3, row-major */
+  float up[3];
+  float fwd[3];
+  float scale[3];
+} dgpu_env_light;
+
+typedef struct {
+  uint32_t width, height;
+  const float* rgb;        /* width*height*3 fp32 texels, nearest + wrap (image_texture.cpp:5-11) */
+} dgpu_texture;
+
+typedef struct {
+  float camera_to_world[12]; /* row-major 3x4 affine (dort Transform, pinhole_camera.cpp:6-11) */
+  float fov;                 /* spans the longer film dimension (pinhole_camera.cpp:163-181) */
+} dgpu_camera;
+
+typedef struct {
+  uint32_t num_vertices;
+  uint32_t num_triangles;
+  const float* positions;      /* num_vertices*3 */
+  const uint32_t* indices;     /* num_triangles*3 */
+  const float* uvs;            /* num_vertices*2, or NULL: (0,0),(1,0),(0,1) per triangle (triangle.cpp:89-93) */
+  const uint32_t* tri_material;/* num_triangles material indices */
+  uint32_t num_materials;
+  const dgpu_material* materials;
+  uint32_t num_area_lights;
+  const dgpu_area_light* area_lights;
+  const dgpu_env_light* env;   /* NULL if none */
+  const float* infinite_radiance; /* NULL, or rgb of an InfiniteLight (infinite_light.cpp) */
+  uint32_t num_textures;
+  const dgpu_texture* textures;
+  dgpu_camera camera;
+} dgpu_scene_desc;
+
+/* BVH / kernel knobs (the autotuner's search space). Zero fields take defaults. */
+typedef struct {
+  uint32_t width;          /* 2 or 4 (default 2) */
+  uint32_t max_leaf_size;  /* triangles per leaf, 1..8 (default 4) */
+  uint32_t sah_bins;       /* binned-SAH bins (default 16) */
+  uint32_t stack_lds;      /* traversal stack entries kept in LDS per lane (default 16) */
+  uint32_t block_size;     /* threads per traversal workgroup: 64,128,256 (default 128) */
+  uint32_t queue_tile;     /* paths per wavefront chunk (default: sized from film) */
+} dgpu_bvh_opts;
+
+/* ---- rays and hits (SoA, trace_* calls) -------------------------------- */
+
+typedef struct {
+  const float* ox; const float* oy; const float* oz;
+  const float* dx; const float* dy; const float* dz;
+  const float* tmin; const float* tmax;
+} dgpu_rays;
+
+typedef struct {
+  float* t;          /* hit distance, INFINITY when missed */
+  int32_t* prim;     /* scene triangle index, -1 when missed */
+  float* b1;         /* barycentric weight of vertex 1 */
+  float* b2;         /* barycentric weight of vertex 2 */
+} dgpu_hits;
+
+enum { DGPU_BUF_DEVICE = 1u };
+
+/* ---- rendering ----------------------------------------------------------- */
+
+enum { DGPU_DIRECT_MIS = 0, DGPU_DIRECT_BSDF = 1, DGPU_DIRECT_LIGHT = 2 };
+
+typedef struct {
+  uint32_t x_res, y_res;
+  uint32_t sample_begin;   /* iterations [sample_begin, sample_end); iteration 0 samples pixel centres */
+  uint32_t sample_end;
+  uint64_t seed;
+  uint32_t min_depth;      /* path_renderer.hpp:16-21, lua_render.cpp:125-145 */
+  uint32_t max_depth;
+  uint32_t rr_depth;       /* RR when bounces >= rr_depth && bounces > min_depth; dort's rule is 2 */
+  uint32_t only_direct;
+  uint32_t sample_all_lights;
+  uint32_t direct_strategy;/* DGPU_DIRECT_* */
+  uint32_t flags;          /* DGPU_BUF_DEVICE: film pointers are device memory */
+  uint32_t scene_id;       /* mixed into the RNG key; batch renders use the scene's position */
+} dgpu_render_opts;
+
+typedef struct {
+  uint64_t paths;           /* paths traced */
+  uint64_t extension_rays;  /* closest-hit rays */
+  uint64_t shadow_rays;     /* any-hit rays (NEE light + NEE BSDF strategies) */
+  uint64_t dropped_samples; /* non-finite or negative samples dropped (renderer.cpp:36-41) */
+  double kernel_ms;         /* GPU time of the render, HIP events */
+} dgpu_stats;
+
+/* ---- context ------------------------------------------------------------- */
+
+uint32_t dgpu_abi_version(void);
+const char* dgpu_last_error(void);
+
+/* Creates a context on HIP device `device` with its own stream. */
+dgpu_status dgpu_create(int device, dgpu_ctx** out);
+void dgpu_destroy(dgpu_ctx* ctx);
+
+/* ---- scenes -------------------------------------------------------------- */
+
+/* Builds the BVH on the host (binned SAH, flattened to the GPU node layout) and
+ * uploads geometry, materials, lights, textures and the environment map. */
+dgpu_status dgpu_scene_create(dgpu_ctx* ctx, const dgpu_scene_desc* desc,
+                              const dgpu_bvh_opts* opts, dgpu_scene** out);
+void dgpu_scene_destroy(dgpu_scene* scene);
+
+typedef struct {
+  uint32_t nodes;          /* BVH nodes */
+  uint32_t leaves;
+  uint32_t depth;
+  uint32_t width;
+  uint64_t node_bytes;
+  uint64_t triangle_bytes;
+  double build_ms;         /* host BVH build + flatten */
+  double upload_ms;
+} dgpu_scene_info;
+
+dgpu_status dgpu_scene_get_info(const dgpu_scene* scene, dgpu_scene_info* out);
+
+/* ---- intersection (Scene::intersect / Scene::intersect_p, scene.cpp:8-18) - */
+
+/* Closest hit per ray; watertight FP32 triangle test, two-sided. */
+dgpu_status dgpu_trace_closest(dgpu_scene* scene, const dgpu_rays* rays, uint32_t n,
+                               dgpu_hits* out, uint32_t flags, double* kernel_ms);
+/* Any hit per ray in [tmin, tmax]; occluded[i] = 1 if some triangle is hit. */
+dgpu_status dgpu_trace_any(dgpu_scene* scene, const dgpu_rays* rays, uint32_t n,
+                           uint32_t* occluded, uint32_t flags, double* kernel_ms);
+
+/* ---- path tracing (PathRenderer::render, path_renderer.cpp:11-206) -------- */
+
+/* Adds the samples of iterations [sample_begin, sample_end) to the film:
+ * film_rgb_sum[3*(y*x_res+x)+c] += L, film_weight[y*x_res+x] += 1 per sample
+ * (box filter of radius 0.5, film.cpp:16-30).  `cancel` (host memory, may be
+ * NULL) is polled between bounces. */
+dgpu_status dgpu_render(dgpu_scene* scene, const dgpu_render_opts* opts,
+                        float* film_rgb_sum, float* film_weight,
+                        dgpu_stats* out_stats, const volatile int* cancel);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DORTGPU_H */

@@ -1,0 +1,116 @@
+"""GPU path-tracing parity against the oracle and dort's own reference images.
+
+* Counter-RNG parity: the oracle can draw the same counter-based random numbers as
+  the GPU, so both trace the same paths; images agree pixel by pixel except where a
+  path diverges on an FP tie (traversal edge, RR threshold).
+* dort's fixtures: test/ref/box_diff_area_pin_d{0..3}.hdr and box_diff_area_pin.hdr
+  (dort `pt`, 512x512, 500 iterations) judged by dort's own test_convergence.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from dort_amd import abi, scenes
+from oracle import convergence, oracle, rgbe
+from tests.test_oracle import golden_ok
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _pixel_parity(img_g, img_c, rel=1e-3, abs_=1e-4):
+    close = np.abs(img_g - img_c) <= rel * np.abs(img_c) + abs_
+    return float(close.all(axis=-1).mean())
+
+
+@pytest.mark.parametrize("strategy", [abi.DIRECT_MIS, abi.DIRECT_LIGHT, abi.DIRECT_BSDF])
+def test_cornell_counter_rng_parity(gpu_ctx, strategy):
+    sc = scenes.cornell()
+    opts = abi.render_opts(64, 64, spp=16, seed=7, max_depth=5, direct_strategy=strategy)
+    ref = oracle.OracleScene(sc).image(opts, rng="counter")
+    img = gpu_ctx.scene(sc).image(opts)
+    frac = _pixel_parity(img, ref)
+    rel_mean = abs(img.mean() - ref.mean()) / ref.mean()
+    print(f"strategy {strategy}: {frac:.4f} pixels match, mean rel diff {rel_mean:.2e}")
+    assert frac >= 0.97
+    assert rel_mean < 2e-3
+
+
+def test_terrain_counter_rng_parity(gpu_ctx):
+    sc = scenes.terrain_scene(quads=60, seed=4, sky_res=(128, 64))
+    opts = abi.render_opts(48, 48, spp=8, seed=3, max_depth=8, rr_depth=3)
+    ref = oracle.OracleScene(sc).image(opts, rng="counter")
+    img = gpu_ctx.scene(sc).image(opts)
+    frac = _pixel_parity(img, ref)
+    rel_mean = abs(img.mean() - ref.mean()) / ref.mean()
+    print(f"terrain: {frac:.4f} pixels match, mean rel diff {rel_mean:.2e}")
+    assert frac >= 0.95
+    assert rel_mean < 5e-3
+
+
+@pytest.mark.parametrize("name,min_depth,max_depth", [
+    ("box_diff_area_pin_d0", 0, 0), ("box_diff_area_pin_d1", 1, 1), ("box_diff_area_pin_d2", 2, 2),
+    ("box_diff_area_pin_d3", 3, 3), ("box_diff_area_pin", 0, 3)])
+def test_dort_reference_images(gpu_ctx, name, min_depth, max_depth):
+    ref = rgbe.read_hdr(os.path.join(GOLDEN, name + ".hdr"))
+    sc = scenes.dort_test_box("area")
+    opts = abi.render_opts(512, 512, spp=64, seed=11, min_depth=min_depth, max_depth=max_depth)
+    img = gpu_ctx.scene(sc).image(opts)
+    print(name, img.reshape(-1, 3).mean(0), ref.reshape(-1, 3).mean(0))
+    ok, msg = golden_ok(ref, img)
+    assert ok, msg
+
+
+def test_deterministic_and_progressive(gpu_ctx):
+    sc = scenes.cornell()
+    s = gpu_ctx.scene(sc)
+    opts = abi.render_opts(32, 32, spp=8, seed=5)
+    a, wa, _ = s.render(opts)
+    b, wb, _ = s.render(opts)
+    assert np.array_equal(a, b) and np.array_equal(wa, wb)
+    # [0,4) + [4,8) into the same film equals [0,8) up to summation order
+    c, wc, _ = s.render(abi.render_opts(32, 32, spp=4, sample_begin=0, seed=5))
+    c, wc, _ = s.render(abi.render_opts(32, 32, spp=4, sample_begin=4, seed=5), c, wc)
+    assert np.array_equal(wc, wa)
+    assert np.allclose(c, a, rtol=1e-5, atol=1e-6)
+    # a different chunk size (queue tile) gives the same image
+    s2 = gpu_ctx.scene(sc, queue_tile=1000)
+    d, wd, _ = s2.render(opts)
+    assert np.array_equal(wd, wa)
+    assert np.allclose(d, a, rtol=1e-5, atol=1e-6)
+
+
+def test_furnace_convex_sphere(gpu_ctx):
+    """A convex Lambert sphere of albedo a under a uniform infinite light L = 1 reflects
+    exactly a*L; the background is L."""
+    p, t = scenes.displaced_sphere(24, seed=1, amp=0.0)
+    b = scenes._Builder()
+    a = 0.6
+    m = b.material(scenes.lambert((a, a, a)))
+    b.mesh(p, t, m)
+    sc = b.build(scenes.look_at((0, 0, -4), (0, 0, 0), (0, 1, 0)), 0.6, infinite=(1.0, 1.0, 1.0))
+    img = gpu_ctx.scene(sc).image(abi.render_opts(64, 64, spp=256, seed=2, max_depth=1))
+    ys, xs = np.mgrid[0:64, 0:64]
+    r = np.hypot(xs + 0.5 - 32, ys + 0.5 - 32)
+    inside = r < 20
+    outside = r > 29
+    assert abs(img[inside].mean() - a) < 0.01 * a
+    assert np.allclose(img[outside], 1.0)
+
+
+def test_stats_and_errors(gpu_ctx):
+    sc = scenes.cornell()
+    s = gpu_ctx.scene(sc)
+    _, _, st = s.render(abi.render_opts(16, 16, spp=2))
+    assert st["paths"] == 16 * 16 * 2
+    assert st["extension_rays"] >= st["paths"]
+    assert st["dropped_samples"] == 0
+    from dort_amd import gpu
+    with pytest.raises(gpu.DgpuError):
+        s.render(abi.render_opts(16, 16, spp=1, max_depth=100))
+    cancel = np.ones(1, np.int32)
+    with pytest.raises(gpu.DgpuError) as e:
+        s.render(abi.render_opts(16, 16, spp=1), cancel=cancel)
+    assert e.value.status == abi.DGPU_ECANCELLED

@@ -1,0 +1,338 @@
+"""Pins the oracle (the CPU restatement of dort's `pt` path) before it is trusted.
+
+* Golden vectors: dort's own reference renders of its Cornell test box
+  (test/ref/box_diff_area_pin_d{0..3}.hdr, box_diff_area_pin.hdr: `pt`, 512x512,
+  500 iterations, dort's mt19937 sampler) judged with dort's test_convergence.
+* Brute force: BVH traversal against an O(N R) loop over all triangles.
+* Closed forms for the BSDFs, the environment light and whole-integrator furnaces.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from dort_amd import abi, scenes
+from oracle import convergence, oracle, rgbe
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+REFS = [("box_diff_area_pin_d0", 0, 0), ("box_diff_area_pin_d1", 1, 1), ("box_diff_area_pin_d2", 2, 2),
+        ("box_diff_area_pin_d3", 3, 3), ("box_diff_area_pin", 0, 3)]
+
+
+@pytest.fixture(scope="module")
+def test_box():
+    return oracle.OracleScene(scenes.dort_test_box("area"))
+
+
+# ----------------------------------------------------------------- golden images
+
+@pytest.mark.parametrize("name,min_depth,max_depth", REFS)
+def test_golden_dort_renders(test_box, name, min_depth, max_depth):
+    ref = rgbe.read_hdr(os.path.join(GOLDEN, name + ".hdr"))
+    assert ref.shape == (512, 512, 3)
+    opts = abi.render_opts(512, 512, spp=16, seed=42, min_depth=min_depth, max_depth=max_depth)
+    img = test_box.image(opts, rng="mt")
+    ok, msg = golden_ok(ref, img)
+    assert ok, msg
+
+
+def golden_ok(ref, img):
+    """dort's acceptance rule (test.lua:88-110, variation 8), the same tile test with a
+    4x tighter variation, and image means within 2%."""
+    status, msg = convergence.compare_with_ref(ref, img)
+    if status == "err":
+        return False, msg
+    msg = convergence.test_convergence(ref, img, variation=2.0)
+    if msg:
+        return False, msg
+    a, b = img.reshape(-1, 3).mean(0), ref.reshape(-1, 3).mean(0)
+    if not np.allclose(a, b, rtol=0.02, atol=2e-4):
+        return False, f"means {a} vs {b}"
+    return True, ""
+
+
+def test_golden_depth_decomposition():
+    """The full render of dort's reference (min 0, max 3) is the sum of the per-depth ones."""
+    full = rgbe.read_hdr(os.path.join(GOLDEN, "box_diff_area_pin.hdr")).reshape(-1, 3).mean(0)
+    parts = sum(rgbe.read_hdr(os.path.join(GOLDEN, f"box_diff_area_pin_d{d}.hdr")).reshape(-1, 3).mean(0)
+                for d in range(4))
+    np.testing.assert_allclose(full, parts, rtol=0.01)
+
+
+def test_counter_and_mt_rng_agree_in_expectation(test_box):
+    o = abi.render_opts(128, 128, spp=32, seed=3, min_depth=0, max_depth=3)
+    a = test_box.image(o, rng="mt").reshape(-1, 3).mean(0)
+    b = test_box.image(o, rng="counter").reshape(-1, 3).mean(0)
+    np.testing.assert_allclose(a, b, rtol=0.02)
+
+
+def test_convergence_rejects_a_wrong_image(test_box):
+    """A planted mistake in the oracle's scene (albedo x0.9) must fail the golden check."""
+    ref = rgbe.read_hdr(os.path.join(GOLDEN, "box_diff_area_pin_d1.hdr"))
+    sc = scenes.dort_test_box("area")
+    for m in sc.materials:
+        if m.type == abi.MAT_LAMBERT:
+            m.albedo = abi.f3(*[0.9 * a for a in m.albedo])
+    img = oracle.OracleScene(sc).image(abi.render_opts(512, 512, spp=16, min_depth=1, max_depth=1), rng="mt")
+    # dort's own rule (variation 8) is too loose to see a 10% albedo error here ...
+    assert convergence.compare_with_ref(ref, img)[0] != "err"
+    # ... the tightened tile test and the mean check both catch it
+    assert convergence.test_convergence(ref, img, variation=2.0)
+    assert not golden_ok(ref, img)[0]
+
+
+def test_normal_cdf_inverse_matches_scipy():
+    from scipy.stats import norm
+    for p in (1e-6, 1e-4, 0.01, 0.2, 0.5, 0.8, 0.999):
+        assert abs(convergence.normal_cdf_inverse(p) - norm.ppf(p)) < 5e-4
+
+
+# ----------------------------------------------------------------- intersection
+
+def test_bvh_traversal_equals_brute_force():
+    sc = scenes.icosphere_scene(freq=12, seed=2)
+    rays = scenes.intersection_workload(sc, 64, 32, 2048, 9)
+    o = oracle.OracleScene(sc)
+    a = o.trace(rays)
+    b = o.trace(rays, brute=True)
+    assert np.array_equal(a["prim"], b["prim"])
+    assert np.array_equal(a["t"], b["t"])
+    occ_a = o.trace(rays, any_hit=True)
+    occ_b = o.trace(rays, any_hit=True, brute=True)
+    assert np.array_equal(occ_a, occ_b)
+    assert np.array_equal(occ_a.astype(bool), a["prim"] >= 0)
+
+
+def test_bvh_sah_equals_middle():
+    sc = scenes.icosphere_scene(freq=12, seed=2)
+    rays = scenes.intersection_workload(sc, 64, 32, 2048, 9)
+    a = oracle.OracleScene(sc, split="middle").trace(rays)
+    b = oracle.OracleScene(sc, split="sah").trace(rays)
+    with np.errstate(invalid="ignore"):
+        same = (a["prim"] == b["prim"]) | (np.abs(a["t"] - b["t"]) <= 1e-6 * np.abs(a["t"]))
+    assert same.mean() > 0.9999
+
+
+def test_triangle_closed_forms():
+    """Axis-aligned hits, edge and vertex hits (inclusive), parallel rays, back faces."""
+    b = scenes._Builder()
+    m = b.material(scenes.lambert((1, 1, 1)))
+    b.mesh([(0, 0, 0), (1, 0, 0), (0, 1, 0)], [[0, 1, 2]], m)
+    sc = b.build(scenes.look_at((0, 0, -1), (0, 0, 0), (0, 1, 0)), 0.5)
+    o = oracle.OracleScene(sc)
+    origins = np.array([[0.25, 0.25, -2], [0.25, 0.25, 3], [0.5, 0.5, -1], [0, 0, -1], [1, 0, -1],
+                        [0.6, 0.6, -1], [0.25, 0.25, -1], [0.25, 0.25, -1]], np.float32)
+    dirs = np.array([[0, 0, 1], [0, 0, -1], [0, 0, 1], [0, 0, 1], [0, 0, 1], [0, 0, 1], [1, 0, 0],
+                     [0, 0, 1]], np.float32)
+    rays = scenes.ray_batch(origins, dirs)
+    rays["tmax"][7] = 0.5   # segment ends before the plane
+    r = o.trace(rays, brute=True)
+    assert list(r["prim"]) == [0, 0, 0, 0, 0, -1, -1, -1]
+    np.testing.assert_allclose(r["t"][:5], [2, 3, 1, 1, 1])
+    np.testing.assert_allclose(r["b1"][0], 0.25)
+    np.testing.assert_allclose(r["b2"][0], 0.25)
+    # through dort's BVH, rays starting exactly on a slab plane of a flat box with a zero
+    # direction component get 0 * inf = NaN in Box::fast_hit_p (box.cpp:57-90) and miss
+    rb = o.trace(rays)
+    assert list(rb["prim"]) == [0, 0, 0, -1, -1, -1, -1, -1]
+
+
+# ----------------------------------------------------------------- BSDFs
+
+def _sphere_dirs(n, seed):
+    rng = np.random.default_rng(seed)
+    v = rng.normal(size=(n, 3))
+    return (v / np.linalg.norm(v, axis=1, keepdims=True)).astype(np.float32)
+
+
+def _hemi_quadrature(f, n=400_000, seed=0):
+    """Monte Carlo over the upper hemisphere with uniform directions (pdf 1/2pi)."""
+    d = _sphere_dirs(n, seed)
+    d[:, 2] = np.abs(d[:, 2])
+    return f(d) * 2 * math.pi
+
+
+def test_lambert_closed_forms():
+    m = scenes.lambert((0.3, 0.5, 0.7))
+    wo = np.tile(np.array([[0.3, 0.1, 0.9486833]], np.float32), (50000, 1))
+    wi = _sphere_dirs(50000, 1)
+    wi[:, 2] = np.abs(wi[:, 2])
+    f, pdf = oracle.bsdf_eval(m, wi, wo)
+    np.testing.assert_allclose(f, np.tile([[0.3 / math.pi, 0.5 / math.pi, 0.7 / math.pi]], (50000, 1)), rtol=1e-6)
+    np.testing.assert_allclose(pdf, wi[:, 2] / math.pi, rtol=1e-5)
+    # two-sided: directions below the surface seen from below
+    f2, _ = oracle.bsdf_eval(m, -wi, -wo)
+    np.testing.assert_allclose(f2, f)
+    # transmission is zero
+    f3, _ = oracle.bsdf_eval(m, -wi, wo)
+    assert (f3 == 0).all()
+    # the pdf integrates to 1 over the hemisphere
+    val = _hemi_quadrature(lambda d: oracle.bsdf_eval(m, d, np.tile(wo[:1], (len(d), 1)))[1].mean())
+    assert abs(val - 1) < 0.01
+
+
+def test_lambert_sampling_matches_pdf():
+    m = scenes.lambert((0.5, 0.5, 0.5))
+    rng = np.random.default_rng(3)
+    n = 20000
+    wo = np.tile(np.array([[0.0, 0.6, 0.8]], np.float32), (n, 1))
+    wi, f, pdf, fl = oracle.bsdf_sample(m, wo, rng.random((n, 3)).astype(np.float32))
+    assert (wi[:, 2] >= 0).all()
+    np.testing.assert_allclose(pdf, wi[:, 2] / math.pi, rtol=1e-4, atol=1e-6)
+    # chi-square in cos(theta) bins: cos^2 theta is uniform under cosine sampling
+    h, _ = np.histogram(wi[:, 2] ** 2, bins=10, range=(0, 1))
+    chi2 = ((h - n / 10) ** 2 / (n / 10)).sum()
+    assert chi2 < 30
+
+
+def test_dielectric_closed_forms():
+    m = scenes.dielectric(1.5)
+    n = 4
+    wo = np.tile(np.array([[0.0, 0.0, 1.0]], np.float32), (n, 1))
+    uvc = np.array([[0.5, 0.5, 0.01], [0.5, 0.5, 0.99], [0.5, 0.5, 0.039], [0.5, 0.5, 0.041]], np.float32)
+    wi, f, pdf, fl = oracle.bsdf_sample(m, wo, uvc)
+    # normal incidence: Fresnel ((n-1)/(n+1))^2 = 0.04
+    assert fl[0] & 1 and fl[0] & 16 and abs(pdf[0] - 0.04) < 1e-6
+    np.testing.assert_allclose(f[0] * 1.0 / pdf[0], [1, 1, 1], rtol=1e-5)   # f cos / pdf = 1 for the reflection
+    assert fl[1] & 2 and abs(pdf[1] - 0.96) < 1e-6
+    np.testing.assert_allclose(wi[1], [0, 0, -1], atol=1e-7)
+    assert fl[2] & 1 and fl[3] & 2
+    # total internal reflection from inside beyond the critical angle
+    s = math.sin(math.radians(50))
+    wo2 = np.array([[s, 0, -math.cos(math.radians(50))]], np.float32)
+    wi2, f2, pdf2, fl2 = oracle.bsdf_sample(m, wo2, np.array([[0.5, 0.5, 0.99]], np.float32))
+    assert fl2[0] == 16 | 1 and pdf2[0] == 1.0
+    # delta lobes are never evaluated
+    f3, pdf3 = oracle.bsdf_eval(m, wi, wo)
+    assert (f3 == 0).all() and (pdf3 == 0).all()
+
+
+def test_ggx_pdf_normalised_against_sampler():
+    """The GGX pdf integrates over the upper hemisphere to 1 minus the probability that a
+    visible-normal sample reflects below the horizon (which the sampler rejects)."""
+    rng = np.random.default_rng(4)
+    for alpha, wz in ((0.05, 1.0), (0.2, 1.0), (0.5, 1.0), (0.4, 0.5)):
+        m = scenes.ggx_conductor(alpha)
+        w = np.array([[math.sqrt(1 - wz * wz), 0.0, wz]], np.float32)
+        n = 400000
+        wo = np.tile(w, (n, 1))
+        val = _hemi_quadrature(lambda d: oracle.bsdf_eval(m, d, wo[:len(d)])[1].mean(), n=n)
+        _, _, pdf, _ = oracle.bsdf_sample(m, wo, rng.random((n, 3)).astype(np.float32))
+        lost = float((pdf == 0).mean())
+        assert abs(val + lost - 1) < 0.02, (alpha, val, lost)
+
+
+def test_ggx_conductor_reciprocity_and_furnace():
+    m = scenes.ggx_conductor(0.3, "aluminium")
+    a = _sphere_dirs(20000, 5)
+    b = _sphere_dirs(20000, 6)
+    a[:, 2] = np.abs(a[:, 2])
+    b[:, 2] = np.abs(b[:, 2])
+    f_ab, _ = oracle.bsdf_eval(m, a, b)
+    f_ba, _ = oracle.bsdf_eval(m, b, a)
+    np.testing.assert_allclose(f_ab, f_ba, rtol=1e-4, atol=1e-6)
+    # white furnace with a perfect mirror-like Fresnel (huge k): albedo <= 1, decreasing with alpha
+    prev = 2.0
+    albedos = []
+    for alpha in (0.05, 0.3, 0.8):
+        mm = scenes.ggx_conductor(alpha)
+        mm.eta = abi.f3(1.0, 1.0, 1.0)
+        mm.k = abi.f3(1e4, 1e4, 1e4)
+        n = 100000
+        wo = np.tile(np.array([[0.6, 0.0, 0.8]], np.float32), (n, 1))
+        rng = np.random.default_rng(7)
+        wi, f, pdf, fl = oracle.bsdf_sample(mm, wo, rng.random((n, 3)).astype(np.float32))
+        ok = pdf > 0
+        est = np.where(ok, f[:, 0] * np.abs(wi[:, 2]) / np.where(ok, pdf, 1), 0).mean()
+        assert est <= 1.001
+        assert est <= prev + 0.01
+        prev = est
+        albedos.append(est)
+    # a nearly smooth mirror keeps all the energy; rougher ones lose it to masking
+    assert albedos[0] > 0.98 and albedos[2] < albedos[0] - 0.1
+
+
+def test_ggx_sampling_pdf_consistent():
+    m = scenes.ggx_conductor(0.25)
+    n = 5000
+    rng = np.random.default_rng(8)
+    wo = np.tile(np.array([[0.3, -0.2, 0.93273791]], np.float32), (n, 1))
+    wi, f, pdf, fl = oracle.bsdf_sample(m, wo, rng.random((n, 3)).astype(np.float32))
+    ok = pdf > 0
+    f2, pdf2 = oracle.bsdf_eval(m, wi[ok], wo[ok])
+    np.testing.assert_allclose(pdf2, pdf[ok], rtol=2e-3)
+    np.testing.assert_allclose(f2, f[ok], rtol=2e-3, atol=1e-6)
+
+
+# ----------------------------------------------------------------- lights
+
+def test_environment_pdf_normalised_and_round_trip():
+    sky = scenes.procedural_sky(128, 64)
+    b = scenes._Builder()
+    m = b.material(scenes.lambert((0.5, 0.5, 0.5)))
+    b.mesh([(0, 0, 0), (1, 0, 0), (0, 1, 0)], [[0, 1, 2]], m)
+    sc = b.build(scenes.look_at((0, 0, -1), (0, 0, 0), (0, 1, 0)), 0.5, env=sky)
+    o = oracle.OracleScene(sc)
+    d = _sphere_dirs(400000, 9)
+    pdf, rad = o.env_eval(d)
+    assert abs(pdf.mean() * 4 * math.pi - 1) < 0.02
+    assert (pdf[d[:, 1] < -1e-3] == 0).all()        # lower hemisphere is never sampled
+    rng = np.random.default_rng(1)
+    dirs, spdf, srad = o.env_sample(rng.random((20000, 2)).astype(np.float32))
+    p2, r2 = o.env_eval(dirs)
+    ok = spdf > 0
+    # a sampled direction maps back to its own pixel except within rounding of a pixel
+    # border (where the pdf jumps); dort's float arithmetic has the same property
+    same = np.abs(p2[ok] - spdf[ok]) <= 2e-3 * spdf[ok]
+    assert same.mean() > 0.995
+    np.testing.assert_allclose(r2[ok][same], srad[ok][same], rtol=1e-5)
+    # importance sampling estimates the upper-hemisphere irradiance-free integral of L
+    est = (srad[ok, 0] / spdf[ok]).mean()
+    mc = (rad[:, 0] * (d[:, 1] > 0)).mean() * 4 * math.pi
+    assert abs(est - mc) / mc < 0.03
+
+
+def test_light_pick_proportional_to_power():
+    sc = scenes.cornell()
+    kinds, cdf = oracle.OracleScene(sc).lights()
+    assert list(kinds) == [0, 0]
+    np.testing.assert_allclose(np.diff(cdf), [0.5, 0.5], rtol=1e-5)
+
+
+# ----------------------------------------------------------------- furnaces
+
+def _convex_sphere_scene(albedo, **kw):
+    p, t = scenes.displaced_sphere(24, seed=1, amp=0.0)
+    b = scenes._Builder()
+    m = b.material(scenes.lambert((albedo, albedo, albedo)))
+    b.mesh(p, t, m)
+    return b.build(scenes.look_at((0, 0, -4), (0, 0, 0), (0, 1, 0)), 0.6, **kw)
+
+
+@pytest.mark.parametrize("strategy", [abi.DIRECT_MIS, abi.DIRECT_LIGHT, abi.DIRECT_BSDF])
+def test_furnace_convex_sphere(strategy):
+    """A convex Lambert sphere of albedo a under a uniform infinite light reflects exactly a."""
+    a = 0.6
+    sc = _convex_sphere_scene(a, infinite=(1.0, 1.0, 1.0))
+    img = oracle.OracleScene(sc).image(abi.render_opts(64, 64, spp=64, seed=2, max_depth=1,
+                                                       direct_strategy=strategy))
+    ys, xs = np.mgrid[0:64, 0:64]
+    r = np.hypot(xs + 0.5 - 32, ys + 0.5 - 32)
+    assert abs(img[r < 20].mean() - a) < 0.015 * a
+    assert np.allclose(img[r > 29], 1.0)
+
+
+def test_all_lights_equals_single_pick():
+    sc = oracle.OracleScene(scenes.cornell())
+    a = sc.image(abi.render_opts(64, 64, spp=64, seed=4, sample_all_lights=True)).mean()
+    b = sc.image(abi.render_opts(64, 64, spp=64, seed=4, sample_all_lights=False)).mean()
+    assert abs(a - b) / a < 0.02
+
+
+def test_rr_depth_is_unbiased():
+    sc = oracle.OracleScene(scenes.cornell())
+    a = sc.image(abi.render_opts(64, 64, spp=64, seed=4, max_depth=8, rr_depth=2)).mean()
+    b = sc.image(abi.render_opts(64, 64, spp=64, seed=4, max_depth=8, rr_depth=100)).mean()
+    assert abs(a - b) / b < 0.02
