@@ -1,0 +1,210 @@
+"""Benchmark of the MI355X path-tracing core.
+
+One step = one dgpu_render of the displaced-noise terrain workload: a
+2,000,000-triangle heightfield plus a 2-triangle ground plane, lit by a
+1024x512 FP32 procedural sky (sun lobe of peak 50, gradient sky), per-object
+Lambert / GGX-conductor / dielectric materials drawn from a seed, rendered at
+1024x1024 with 64 samples per pixel, max depth 8, Russian roulette from depth 3,
+MIS between BSDF and environment sampling.  The scene (BVH, triangles, sky) and
+the film live in HBM before the timed region starts.
+
+Also measured in the same run (secondary fields): the ray-intersection workload,
+a 1,310,720-triangle displaced icosphere with 16,777,216 rays (half pinhole rays
+at 4096x2048, half random rays), closest hit and any hit, rays and hits resident
+in HBM.
+
+Printed: ONE JSON line with the metric, the roofline of the dominant kernel
+(achieved algorithmic GB/s from live HIP-event timings of every launch) and the
+CPU baseline (the oracle, a C++ restatement of dort's `pt`, timed on this host's
+cores over a bounded sample of the same workload).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--res", type=int, default=1024)
+    ap.add_argument("--spp", type=int, default=64)
+    ap.add_argument("--quads", type=int, default=1000)
+    ap.add_argument("--cpu-spp", type=int, default=8, help="samples per pixel of the CPU sample")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-intersect", action="store_true")
+    ap.add_argument("--width", type=int, default=2)
+    ap.add_argument("--stack", type=int, default=16)
+    ap.add_argument("--block", type=int, default=128)
+    ap.add_argument("--leaf", type=int, default=4)
+    ap.add_argument("--tile", type=int, default=0)
+    return ap.parse_args()
+
+
+def terrain_workload(args):
+    from dort_amd import abi, scenes
+    sc = scenes.terrain_scene(quads=args.quads, seed=2)
+    opts = abi.render_opts(args.res, args.res, spp=args.spp, seed=1234, max_depth=8, rr_depth=3)
+    return sc, opts
+
+
+def intersection_bench(ctx, args):
+    from dort_amd import abi, gpu, scenes
+    t0 = time.time()
+    sc = scenes.icosphere_scene(freq=256, seed=1)
+    rays = scenes.intersection_workload(sc, 4096, 2048, 4096 * 2048, seed=7)
+    n = len(rays["ox"])
+    log(f"intersection workload: {sc.num_triangles} triangles, {n} rays ({time.time() - t0:.1f}s to generate)")
+    gsc = ctx.scene(sc, width=args.width, max_leaf_size=args.leaf, stack_lds=args.stack,
+                    block_size=args.block)
+    info = gsc.info()
+    dev = {k: ctx.device_array(v) for k, v in rays.items()}
+    out = {"t": gpu.DeviceArray(ctx, n, np.float32), "prim": gpu.DeviceArray(ctx, n, np.int32),
+           "b1": gpu.DeviceArray(ctx, n, np.float32), "b2": gpu.DeviceArray(ctx, n, np.float32)}
+    occ = gpu.DeviceArray(ctx, n, np.uint32)
+    gsc.trace_closest(dev, out=out, device=True)
+    gsc.trace_any(dev, out=occ, device=True)
+    cms, ams = [], []
+    for _ in range(3):
+        cms.append(gsc.trace_closest(dev, out=out, device=True)["kernel_ms"])
+        ams.append(gsc.trace_any(dev, out=occ, device=True)[1])
+    c_ms, a_ms = float(np.median(cms)), float(np.median(ams))
+    hit_frac = float((out["prim"].numpy() >= 0).mean())
+    scene_bytes = 36 * sc.num_triangles + 32 * 2 * info["nodes"]
+    res = {"workload": f"displaced icosphere {sc.num_triangles} tris, {n} rays (4096x2048 pinhole + random)",
+           "Mrays_closest": n / c_ms / 1e3, "Mrays_any": n / a_ms / 1e3,
+           "closest_ms": c_ms, "any_ms": a_ms, "hit_fraction": hit_frac,
+           "closest_GBps_algorithmic": (48 * n + scene_bytes) / c_ms / 1e6,
+           "bvh": {"nodes": info["nodes"], "depth": info["depth"], "build_ms": info["build_ms"]}}
+    for v in list(dev.values()) + [out[k] for k in ("t", "prim", "b1", "b2")] + [occ]:
+        v.free()
+    return res
+
+
+def cpu_baseline(sc, args):
+    from dort_amd import abi
+    from oracle import oracle
+    threads = args.cpu_threads or max(1, min(os.cpu_count() or 1, 16))
+    o = oracle.OracleScene(sc)
+    opts = abi.render_opts(args.res, args.res, spp=args.cpu_spp, seed=1234, max_depth=8, rr_depth=3)
+    t0 = time.time()
+    _, _, cnt = o.render(opts, rng="mt", threads=threads)
+    dt = time.time() - t0
+    return {"value": cnt["paths"] / dt / 1e6, "unit": "Mpaths/s", "cores": threads, "kind": "port",
+            "sample": f"{args.res}x{args.res} x {args.cpu_spp} spp of the same terrain scene "
+                      f"({cnt['paths']} paths, {dt:.1f}s, dort's mt19937 sampler)",
+            "extension_rays_per_path": cnt["extension_rays"] / cnt["paths"],
+            "shadow_rays_per_path": cnt["shadow_rays"] / cnt["paths"]}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                                   "(sharding across GPUs is described in DESIGN.md, not run)"}))
+        sys.exit(2)
+    from dort_amd import abi, gpu
+    sc, opts = terrain_workload(args)
+    log(f"terrain workload: {sc.num_triangles} triangles, {args.res}x{args.res} x {args.spp} spp")
+    ctx = gpu.Context(0)
+    gsc = ctx.scene(sc, width=args.width, max_leaf_size=args.leaf, stack_lds=args.stack,
+                    block_size=args.block, queue_tile=args.tile)
+    info = gsc.info()
+    log(f"BVH: {info['nodes']} nodes, depth {info['depth']}, build {info['build_ms']:.0f} ms")
+    npix = args.res * args.res
+    film = gpu.DeviceArray(ctx, npix * 3, np.float32)
+    film.copy_from(np.zeros(npix * 3, np.float32))
+    wfilm = gpu.DeviceArray(ctx, npix, np.float32)
+    wfilm.copy_from(np.zeros(npix, np.float32))
+    paths = npix * args.spp
+    for i in range(args.warmup):
+        _, _, st = gsc.render(opts, film, wfilm, device=True)
+        log(f"warmup {i}: {st['kernel_ms']:.0f} ms")
+    times, stats = [], []
+    t_all = time.perf_counter()
+    for i in range(args.steps):
+        t0 = time.perf_counter()
+        _, _, st = gsc.render(opts, film, wfilm, device=True)
+        times.append(time.perf_counter() - t0)
+        stats.append(st)
+        log(f"step {i}: {times[-1] * 1e3:.0f} ms, {paths / times[-1] / 1e6:.1f} Mpaths/s")
+    total = time.perf_counter() - t_all
+    ms_per_step = total / args.steps * 1e3
+    value = paths * args.steps / total / 1e6
+
+    # live per-kernel timing (HIP events around every launch, on the library's stream)
+    popts = abi.RenderOpts.from_buffer_copy(opts)
+    popts.flags |= abi.RENDER_PROFILE
+    _, _, ps = gsc.render(popts, film, wfilm, device=True)
+    ext_rays, shd_rays = ps["extension_rays"], ps["shadow_rays"]
+    scene_bytes = 36 * sc.num_triangles + 32 * 2 * info["nodes"]
+    kernels = {
+        # closest-hit: 32 B ray in + 16 B hit out per ray, plus the scene once per launch
+        "extend": (ps["extend_ms"], ps["extend_launches"], 48 * ext_rays / max(1, ps["extend_launches"]) + scene_bytes),
+        # any-hit: 32 B ray + 8 B slot/ignore in, 1 B visibility out, plus the scene
+        "connect": (ps["connect_ms"], ps["connect_launches"], 41 * shd_rays / max(1, ps["connect_launches"]) + scene_bytes),
+        # shade: per extension ray 224 B (state, queue entry, hit, triangle, next entry) + 57 B per shadow ray
+        "shade": (ps["shade_ms"], ps["shade_launches"],
+                  (224 * ext_rays + 57 * shd_rays) / max(1, ps["shade_launches"])),
+    }
+    dom = max(kernels, key=lambda k: kernels[k][0])
+    kms, nl, bytes_per_launch = kernels[dom]
+    avg_ms = kms / max(1, nl)
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": dom,
+                "avg_launch_ms": round(avg_ms, 4), "launches": nl,
+                "algorithmic_bytes_per_launch": int(bytes_per_launch),
+                "kernel_ms": {k: round(v[0], 2) for k, v in kernels.items()},
+                "other_ms": round(ps["other_ms"], 2)}
+    log(f"profile: {roofline['kernel_ms']} dominant {dom} avg {avg_ms:.3f} ms, {achieved:.1f} GB/s")
+
+    secondary = {}
+    if not args.no_intersect:
+        film.free()
+        wfilm.free()
+        secondary["intersection"] = intersection_bench(ctx, args)
+        log(f"intersection: {secondary['intersection']['Mrays_closest']:.0f} Mrays/s closest, "
+            f"{secondary['intersection']['Mrays_any']:.0f} Mrays/s any")
+
+    cpu = None
+    if not args.no_cpu:
+        log("timing the CPU baseline (oracle) ...")
+        cpu = cpu_baseline(sc, args)
+        log(f"cpu: {cpu['value']:.2f} Mpaths/s on {cpu['cores']} threads")
+
+    line = {
+        "metric": "Mpaths/s at equal rMSE vs CPU", "value": round(value, 3), "unit": "Mpaths/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded procedural terrain, sky and materials)",
+        "config": {"workload": f"terrain {sc.num_triangles} tris + sky 1024x512, {args.res}x{args.res}, "
+                               f"{args.spp} spp, max depth 8, RR from depth 3, MIS",
+                   "paths_per_step": paths, "bvh_width": args.width, "stack_lds": args.stack,
+                   "block": args.block, "max_leaf": args.leaf,
+                   "extension_rays_per_path": round(ext_rays / paths, 3),
+                   "shadow_rays_per_path": round(shd_rays / paths, 3)},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "secondary": secondary,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -14,6 +14,7 @@ STATUS_NAMES = {0: "DGPU_OK", 1: "DGPU_EINVAL", 2: "DGPU_ENOMEM", 3: "DGPU_EDEVI
 MAT_NONE, MAT_LAMBERT, MAT_GGX_CONDUCTOR, MAT_DIELECTRIC = range(4)
 DIRECT_MIS, DIRECT_BSDF, DIRECT_LIGHT = range(3)
 BUF_DEVICE = 1
+RENDER_PROFILE = 2
 
 f3 = C.c_float * 3
 
@@ -78,7 +79,10 @@ class RenderOpts(C.Structure):
 class RenderStats(C.Structure):
     _fields_ = [("paths", C.c_uint64), ("extension_rays", C.c_uint64),
                 ("shadow_rays", C.c_uint64), ("dropped_samples", C.c_uint64),
-                ("kernel_ms", C.c_double)]
+                ("kernel_ms", C.c_double), ("extend_ms", C.c_double), ("shade_ms", C.c_double),
+                ("connect_ms", C.c_double), ("other_ms", C.c_double),
+                ("extend_launches", C.c_uint32), ("shade_launches", C.c_uint32),
+                ("connect_launches", C.c_uint32), ("other_launches", C.c_uint32)]
 
 
 class SceneInfo(C.Structure):

@@ -58,10 +58,22 @@ struct dgpu_ctx {
   std::map<uint64_t, uint32_t> grid_cache;
   // render scratch, grown on demand
   std::map<std::string, DevBuf> bufs;
+  // HIP events for per-launch profiling, reused across renders
+  std::vector<hipEvent_t> ev_pool;
+
+  hipEvent_t pool_event(size_t i) {
+    while (ev_pool.size() <= i) {
+      hipEvent_t e = nullptr;
+      if (hipEventCreate(&e) != hipSuccess) return nullptr;
+      ev_pool.push_back(e);
+    }
+    return ev_pool[i];
+  }
 
   ~dgpu_ctx() {
     (void)hipSetDevice(device);
     for (auto& kv : bufs) if (kv.second.p) (void)hipFree(kv.second.p);
+    for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     if (stream) (void)hipStreamDestroy(stream);
@@ -187,6 +199,36 @@ dgpu_status dgpu_create(int device, dgpu_ctx** out) {
 }
 
 void dgpu_destroy(dgpu_ctx* ctx) { delete ctx; }
+
+dgpu_status dgpu_device_alloc(dgpu_ctx* ctx, size_t bytes, void** out) {
+  if (!ctx || !out) return fail(DGPU_EINVAL, "null context or output");
+  *out = nullptr;
+  HIP_TRY(hipSetDevice(ctx->device));
+  if (hipMalloc(out, bytes ? bytes : 1) != hipSuccess) {
+    *out = nullptr;
+    return fail(DGPU_ENOMEM, "hipMalloc(%zu) failed", bytes);
+  }
+  return DGPU_OK;
+}
+
+dgpu_status dgpu_device_free(dgpu_ctx* ctx, void* ptr) {
+  if (!ctx) return fail(DGPU_EINVAL, "null context");
+  HIP_TRY(hipSetDevice(ctx->device));
+  if (ptr) HIP_TRY(hipFree(ptr));
+  return DGPU_OK;
+}
+
+dgpu_status dgpu_copy(dgpu_ctx* ctx, void* dst, const void* src, size_t bytes, int kind) {
+  if (!ctx || (!dst && bytes) || (!src && bytes)) return fail(DGPU_EINVAL, "null argument");
+  if (kind < 0 || kind > 2) return fail(DGPU_EINVAL, "bad copy kind %d", kind);
+  if (bytes == 0) return DGPU_OK;
+  HIP_TRY(hipSetDevice(ctx->device));
+  hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : (kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice);
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HIP_TRY(hipMemcpyAsync(dst, src, bytes, k, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return DGPU_OK;
+}
 
 dgpu_status dgpu_scene_create(dgpu_ctx* ctx, const dgpu_scene_desc* d, const dgpu_bvh_opts* opts,
                               dgpu_scene** out) {
@@ -574,19 +616,33 @@ dgpu_status dgpu_render(dgpu_scene* s, const dgpu_render_opts* o, float* film_rg
     HIP_TRY(hipMemsetAsync(P.film_rgb, 0, (size_t)npix * 12, stream));
     HIP_TRY(hipMemsetAsync(P.film_w, 0, (size_t)npix * 4, stream));
   }
+  const bool profile = (o->flags & DGPU_RENDER_PROFILE) != 0;
+  std::vector<std::pair<int, size_t>> marks;  // (kernel kind, first event of the pair)
+  size_t next_ev = 0;
+  auto timed = [&](int kind, auto&& launch) -> hipError_t {
+    if (!profile) return launch();
+    hipEvent_t a = ctx->pool_event(next_ev), b = ctx->pool_event(next_ev + 1);
+    if (!a || !b) return hipErrorOutOfMemory;
+    hipError_t e = hipEventRecord(a, stream);
+    if (e == hipSuccess) e = launch();
+    if (e == hipSuccess) e = hipEventRecord(b, stream);
+    marks.push_back({kind, next_ev});
+    next_ev += 2;
+    return e;
+  };
   HIP_TRY(hipEventRecord(ctx->ev0, stream));
   bool cancelled = false;
   for (uint64_t g0 = 0; g0 < total && !cancelled; g0 += C) {
     P.g0 = g0;
     P.n_chunk = (uint32_t)std::min<uint64_t>(C, total - g0);
-    HIP_TRY(launch_raygen(P, stream));
+    HIP_TRY(timed(3, [&] { return launch_raygen(P, stream); }));
     for (uint32_t b = 0; b <= o->max_depth; ++b) {
       if (cancel && *cancel) { cancelled = true; break; }
-      HIP_TRY(launch_extend(P, ce, b, stream));
-      HIP_TRY(launch_shade(P, b, stream));
-      HIP_TRY(launch_connect(P, cc, b, stream));
+      HIP_TRY(timed(0, [&] { return launch_extend(P, ce, b, stream); }));
+      HIP_TRY(timed(1, [&] { return launch_shade(P, b, stream); }));
+      HIP_TRY(timed(2, [&] { return launch_connect(P, cc, b, stream); }));
     }
-    if (!cancelled) HIP_TRY(launch_finalize(P, stream));
+    if (!cancelled) HIP_TRY(timed(3, [&] { return launch_finalize(P, stream); }));
   }
   HIP_TRY(hipEventRecord(ctx->ev1, stream));
   unsigned long long hstats[4] = {0, 0, 0, 0};
@@ -612,6 +668,15 @@ dgpu_status dgpu_render(dgpu_scene* s, const dgpu_render_opts* o, float* film_rg
     out_stats->shadow_rays = hstats[2];
     out_stats->dropped_samples = hstats[3];
     out_stats->kernel_ms = ms;
+    double* kms[4] = {&out_stats->extend_ms, &out_stats->shade_ms, &out_stats->connect_ms, &out_stats->other_ms};
+    uint32_t* kn[4] = {&out_stats->extend_launches, &out_stats->shade_launches, &out_stats->connect_launches,
+                       &out_stats->other_launches};
+    for (auto& m : marks) {
+      float e = 0.f;
+      HIP_TRY(hipEventElapsedTime(&e, ctx->ev_pool[m.second], ctx->ev_pool[m.second + 1]));
+      *kms[m.first] += e;
+      *kn[m.first] += 1;
+    }
   }
   return DGPU_OK;
 }

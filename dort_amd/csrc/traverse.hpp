@@ -80,8 +80,11 @@ DG_INLINE WtRay wt_setup(V3 o, V3 d) {
 }
 
 // Watertight test; on a hit inside [tmin, tmax] returns true with t, b1, b2.
+// Contraction into FMA is off here: an edge shared by two triangles must give edge
+// functions of exactly opposite sign, which needs both products rounded the same way.
 DG_INLINE bool wt_hit(const WtRay& w, float4 va, float4 vb, float4 vc, float tmin, float tmax,
                       float& t_out, float& b1_out, float& b2_out) {
+#pragma clang fp contract(off)
   V3 A = xyz(va) - w.o, B = xyz(vb) - w.o, C = xyz(vc) - w.o;
   float Az = comp(A, w.kz), Bz = comp(B, w.kz), Cz = comp(C, w.kz);
   float Ax = comp(A, w.kx) - w.sx * Az, Ay = comp(A, w.ky) - w.sy * Az;

@@ -16,9 +16,9 @@ LIB_PATH = os.path.join(HERE, "libdortgpu.so")
 
 _lib = None
 
-EXPORTS = ["dgpu_abi_version", "dgpu_last_error", "dgpu_create", "dgpu_destroy", "dgpu_scene_create",
-           "dgpu_scene_destroy", "dgpu_scene_get_info", "dgpu_trace_closest", "dgpu_trace_any",
-           "dgpu_render"]
+EXPORTS = ["dgpu_abi_version", "dgpu_last_error", "dgpu_create", "dgpu_destroy", "dgpu_device_alloc",
+           "dgpu_device_free", "dgpu_copy", "dgpu_scene_create", "dgpu_scene_destroy",
+           "dgpu_scene_get_info", "dgpu_trace_closest", "dgpu_trace_any", "dgpu_render"]
 
 
 class DgpuError(RuntimeError):
@@ -37,6 +37,9 @@ def lib():
         L.dgpu_last_error.restype = C.c_char_p
         L.dgpu_create.argtypes = [C.c_int, C.POINTER(C.c_void_p)]
         L.dgpu_destroy.argtypes = [C.c_void_p]
+        L.dgpu_device_alloc.argtypes = [C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p)]
+        L.dgpu_device_free.argtypes = [C.c_void_p, C.c_void_p]
+        L.dgpu_copy.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
         L.dgpu_scene_create.argtypes = [C.c_void_p, C.POINTER(abi.SceneDesc), C.POINTER(abi.BvhOpts),
                                         C.POINTER(C.c_void_p)]
         L.dgpu_scene_destroy.argtypes = [C.c_void_p]
@@ -47,7 +50,7 @@ def lib():
                                      C.c_uint32, C.POINTER(C.c_double)]
         L.dgpu_render.argtypes = [C.c_void_p, C.POINTER(abi.RenderOpts), C.c_void_p, C.c_void_p,
                                   C.POINTER(abi.RenderStats), C.c_void_p]
-        for n in ("dgpu_create", "dgpu_scene_create", "dgpu_scene_get_info", "dgpu_trace_closest",
+        for n in ("dgpu_create", "dgpu_device_alloc", "dgpu_device_free", "dgpu_copy", "dgpu_scene_create", "dgpu_scene_get_info", "dgpu_trace_closest",
                   "dgpu_trace_any", "dgpu_render"):
             getattr(L, n).restype = C.c_int
         _lib = L
@@ -74,6 +77,52 @@ class Context:
 
     def scene(self, scene, **bvh):
         return Scene(self, scene, **bvh)
+
+    def device_array(self, host):
+        """Copies a numpy array into a new DeviceArray on this context's GPU."""
+        d = DeviceArray(self, host.shape, host.dtype)
+        d.copy_from(host)
+        return d
+
+
+class DeviceArray:
+    """A device allocation of the library (HBM), with numpy round trips.  `data_ptr()`
+    gives the raw device address for DGPU_BUF_DEVICE calls."""
+
+    def __init__(self, ctx, shape, dtype):
+        self.ctx = ctx
+        self.shape = tuple(shape) if isinstance(shape, (tuple, list)) else (int(shape),)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        self.ptr = C.c_void_p()
+        check(lib().dgpu_device_alloc(ctx.h, self.nbytes, C.byref(self.ptr)))
+
+    def data_ptr(self):
+        return self.ptr.value
+
+    def __len__(self):
+        return self.shape[0]
+
+    def copy_from(self, host):
+        host = np.ascontiguousarray(host, dtype=self.dtype)
+        assert host.nbytes == self.nbytes
+        check(lib().dgpu_copy(self.ctx.h, self.ptr, host.ctypes.data, self.nbytes, 0))
+
+    def numpy(self):
+        out = np.empty(self.shape, self.dtype)
+        check(lib().dgpu_copy(self.ctx.h, out.ctypes.data, self.ptr, self.nbytes, 1))
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().dgpu_device_free(self.ctx.h, self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 class Scene:

@@ -135,7 +135,10 @@ typedef struct {
   float* b2;         /* barycentric weight of vertex 2 */
 } dgpu_hits;
 
-enum { DGPU_BUF_DEVICE = 1u };
+enum {
+  DGPU_BUF_DEVICE = 1u,    /* buffers are device pointers */
+  DGPU_RENDER_PROFILE = 2u /* dgpu_render: time every kernel launch with HIP events */
+};
 
 /* ---- rendering ----------------------------------------------------------- */
 
@@ -162,6 +165,9 @@ typedef struct {
   uint64_t shadow_rays;     /* any-hit rays (NEE light + NEE BSDF strategies) */
   uint64_t dropped_samples; /* non-finite or negative samples dropped (renderer.cpp:36-41) */
   double kernel_ms;         /* GPU time of the render, HIP events */
+  /* With DGPU_RENDER_PROFILE: GPU time and launch count per kernel kind. */
+  double extend_ms, shade_ms, connect_ms, other_ms;
+  uint32_t extend_launches, shade_launches, connect_launches, other_launches;
 } dgpu_stats;
 
 /* ---- context ------------------------------------------------------------- */
@@ -172,6 +178,13 @@ const char* dgpu_last_error(void);
 /* Creates a context on HIP device `device` with its own stream. */
 dgpu_status dgpu_create(int device, dgpu_ctx** out);
 void dgpu_destroy(dgpu_ctx* ctx);
+
+/* Device memory on the context's device, for callers that keep rays, hits or films
+ * resident in HBM (DGPU_BUF_DEVICE).  kind: 0 host->device, 1 device->host,
+ * 2 device->device.  Copies are synchronous. */
+dgpu_status dgpu_device_alloc(dgpu_ctx* ctx, size_t bytes, void** out);
+dgpu_status dgpu_device_free(dgpu_ctx* ctx, void* ptr);
+dgpu_status dgpu_copy(dgpu_ctx* ctx, void* dst, const void* src, size_t bytes, int kind);
 
 /* ---- scenes -------------------------------------------------------------- */
 

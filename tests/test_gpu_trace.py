@@ -62,8 +62,10 @@ def test_brute_force_small(gpu_ctx):
     ref = osc.trace(rays, brute=True)
     g = gpu_ctx.scene(sc).trace_closest(rays)
     c = classify_hits(ref, g)
+    # the Cornell box is all quads split on a diagonal: pixel-centre rays on its symmetry
+    # lines hit shared edges exactly, where either triangle is a valid answer
     assert c["fail"] == 0, c
-    assert c["agree"] >= 0.999 * c["n"]
+    assert c["agree"] >= 0.99 * c["n"]
 
 
 def test_watertight_closed_mesh(gpu_ctx):
@@ -90,14 +92,20 @@ def test_watertight_closed_mesh(gpu_ctx):
 
 
 def test_device_buffers(gpu_ctx, sphere):
-    torch = pytest.importorskip("torch")
+    """Rays and hits resident in HBM (DGPU_BUF_DEVICE) give the same hits as host buffers."""
     rays = _workload(sphere, res=(64, 32), n_random=2048)
-    host = gpu_ctx.scene(sphere).trace_closest(rays)
-    dev_rays = {k: torch.from_numpy(v).cuda() for k, v in rays.items()}
+    sc = gpu_ctx.scene(sphere)
+    host = sc.trace_closest(rays)
+    dev_rays = {k: gpu_ctx.device_array(v) for k, v in rays.items()}
     n = len(rays["ox"])
-    out = {"t": torch.zeros(n, device="cuda"), "prim": torch.zeros(n, dtype=torch.int32, device="cuda"),
-           "b1": torch.zeros(n, device="cuda"), "b2": torch.zeros(n, device="cuda")}
-    torch.cuda.synchronize()
-    gpu_ctx.scene(sphere).trace_closest(dev_rays, out=out, device=True)
-    assert np.array_equal(out["prim"].cpu().numpy(), host["prim"])
-    assert np.array_equal(out["t"].cpu().numpy(), host["t"])
+    out = {"t": gpu_ctx.device_array(np.zeros(n, np.float32)),
+           "prim": gpu_ctx.device_array(np.zeros(n, np.int32)),
+           "b1": gpu_ctx.device_array(np.zeros(n, np.float32)),
+           "b2": gpu_ctx.device_array(np.zeros(n, np.float32))}
+    sc.trace_closest(dev_rays, out=out, device=True)
+    assert np.array_equal(out["prim"].numpy(), host["prim"])
+    assert np.array_equal(out["t"].numpy(), host["t"])
+    occ_h, _ = sc.trace_any(rays)
+    occ_d = gpu_ctx.device_array(np.zeros(n, np.uint32))
+    sc.trace_any(dev_rays, out=occ_d, device=True)
+    assert np.array_equal(occ_d.numpy(), occ_h)
