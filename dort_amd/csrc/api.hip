@@ -117,6 +117,8 @@ struct dgpu_scene {
   uint32_t stack_cap = MAX_STACK;
   uint32_t num_area = 0;
   dgpu_scene_info info;
+  std::vector<uint4> tex_info_host;   // (w, h, texel offset, 0)
+  std::vector<DevMaterial> mats_host;
 
   ~dgpu_scene() {
     if (ctx) (void)hipSetDevice(ctx->device);
@@ -167,6 +169,21 @@ dgpu_status validate_desc(const dgpu_scene_desc* d) {
       return fail(DGPU_EINVAL, "texture %u without texels", t);
   if (!(d->camera.fov > 0.f && d->camera.fov < 3.14159f)) return fail(DGPU_EINVAL, "bad camera fov");
   return DGPU_OK;
+}
+
+DevMaterial to_dev_material(const dgpu_material& a) {
+  DevMaterial b;
+  memset(&b, 0, sizeof b);
+  b.type = a.type;
+  b.tex = a.type == DGPU_MAT_LAMBERT ? a.albedo_tex : -1;
+  b.roughness = a.roughness;
+  b.ior_in = a.ior_inside;
+  b.albedo = make_float4(a.albedo[0], a.albedo[1], a.albedo[2], a.ior_outside);
+  b.eta = make_float4(a.eta[0], a.eta[1], a.eta[2], 0.f);
+  b.k = make_float4(a.k[0], a.k[1], a.k[2], 0.f);
+  b.refl = make_float4(a.reflect_tint[0], a.reflect_tint[1], a.reflect_tint[2], 0.f);
+  b.trans = make_float4(a.transmit_tint[0], a.transmit_tint[1], a.transmit_tint[2], 0.f);
+  return b;
 }
 
 }  // namespace
@@ -302,22 +319,10 @@ dgpu_status dgpu_scene_create(dgpu_ctx* ctx, const dgpu_scene_desc* d, const dgp
   }
   // materials
   std::vector<DevMaterial> mats(d->num_materials);
-  for (uint32_t m = 0; m < d->num_materials; ++m) {
-    const dgpu_material& a = d->materials[m];
-    DevMaterial& b = mats[m];
-    memset(&b, 0, sizeof b);
-    b.type = a.type;
-    b.tex = a.type == DGPU_MAT_LAMBERT ? a.albedo_tex : -1;
-    b.roughness = a.roughness;
-    b.ior_in = a.ior_inside;
-    b.albedo = make_float4(a.albedo[0], a.albedo[1], a.albedo[2], a.ior_outside);
-    b.eta = make_float4(a.eta[0], a.eta[1], a.eta[2], 0.f);
-    b.k = make_float4(a.k[0], a.k[1], a.k[2], 0.f);
-    b.refl = make_float4(a.reflect_tint[0], a.reflect_tint[1], a.reflect_tint[2], 0.f);
-    b.trans = make_float4(a.transmit_tint[0], a.transmit_tint[1], a.transmit_tint[2], 0.f);
-  }
+  for (uint32_t m = 0; m < d->num_materials; ++m) mats[m] = to_dev_material(d->materials[m]);
   UP(mats.data(), mats.size(), S.mats);
   S.num_mats = d->num_materials;
+  s->mats_host = mats;
 
   // lights, in dort's add order: area lights, environment, infinite
   std::vector<DevLight> lights;
@@ -443,6 +448,7 @@ dgpu_status dgpu_scene_create(dgpu_ctx* ctx, const dgpu_scene_desc* d, const dgp
     UP(texels.data(), texels.size(), S.tex_data);
     UP(info.data(), info.size(), S.tex_info);
     S.num_tex = d->num_textures;
+    s->tex_info_host = info;
   }
 #undef UP
   for (int i = 0; i < 12; ++i) S.c2w[i] = d->camera.camera_to_world[i];
@@ -677,6 +683,117 @@ dgpu_status dgpu_render(dgpu_scene* s, const dgpu_render_opts* o, float* film_rg
       *kms[m.first] += e;
       *kn[m.first] += 1;
     }
+  }
+  return DGPU_OK;
+}
+
+dgpu_status dgpu_scene_update_texture(dgpu_scene* s, uint32_t tex, const float* rgb) {
+  if (!s || !rgb) return fail(DGPU_EINVAL, "null scene or texels");
+  if (tex >= s->tex_info_host.size()) return fail(DGPU_EINVAL, "texture %u out of range", tex);
+  uint4 info = s->tex_info_host[tex];
+  size_t n = (size_t)info.x * info.y;
+  std::vector<float4> t(n);
+  for (size_t i = 0; i < n; ++i) t[i] = make_float4(rgb[3 * i], rgb[3 * i + 1], rgb[3 * i + 2], 0.f);
+  dgpu_ctx* ctx = s->ctx;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HIP_TRY(hipSetDevice(ctx->device));
+  HIP_TRY(hipMemcpy((void*)(s->S.tex_data + info.z), t.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+  return DGPU_OK;
+}
+
+dgpu_status dgpu_scene_update_material(dgpu_scene* s, uint32_t material, const dgpu_material* m) {
+  if (!s || !m) return fail(DGPU_EINVAL, "null scene or material");
+  if (material >= s->mats_host.size()) return fail(DGPU_EINVAL, "material %u out of range", material);
+  if (m->type > DGPU_MAT_DIELECTRIC) return fail(DGPU_EUNSUPPORTED, "unsupported material type %u", m->type);
+  if (m->type == DGPU_MAT_LAMBERT && m->albedo_tex >= (int32_t)s->tex_info_host.size())
+    return fail(DGPU_EINVAL, "texture %d out of range", m->albedo_tex);
+  if (m->type == DGPU_MAT_GGX_CONDUCTOR && !(m->roughness > 0.f))
+    return fail(DGPU_EINVAL, "GGX roughness must be > 0");
+  DevMaterial dm = to_dev_material(*m);
+  dgpu_ctx* ctx = s->ctx;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HIP_TRY(hipSetDevice(ctx->device));
+  HIP_TRY(hipMemcpy((void*)(s->S.mats + material), &dm, sizeof dm, hipMemcpyHostToDevice));
+  s->mats_host[material] = dm;
+  return DGPU_OK;
+}
+
+dgpu_status dgpu_adjoint(dgpu_scene* s, const dgpu_render_opts* o, const float* dldi, float* d_tex,
+                         float* d_rough, float* image_out, dgpu_stats* out_stats) {
+  if (!s || !o || !dldi) return fail(DGPU_EINVAL, "null scene, options or dL/dI");
+  if (o->x_res == 0 || o->y_res == 0) return fail(DGPU_EINVAL, "empty film");
+  if (o->sample_end <= o->sample_begin) return fail(DGPU_EINVAL, "empty sample range");
+  if (o->max_depth + 2 >= MAX_BOUNCES) return fail(DGPU_EINVAL, "max_depth must be < %u", MAX_BOUNCES - 2);
+  if (o->direct_strategy > DGPU_DIRECT_LIGHT) return fail(DGPU_EINVAL, "bad direct_strategy");
+  if (o->sample_all_lights && s->S.num_lights > MAX_SLOT_LIGHTS)
+    return fail(DGPU_EUNSUPPORTED, "sample_all_lights supports at most %u lights", MAX_SLOT_LIGHTS);
+  if (d_tex && s->tex_info_host.empty()) return fail(DGPU_EINVAL, "scene has no texture 0");
+  if (out_stats) memset(out_stats, 0, sizeof *out_stats);
+  dgpu_ctx* ctx = s->ctx;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HIP_TRY(hipSetDevice(ctx->device));
+  hipStream_t stream = ctx->stream;
+  dgpu_status st = DGPU_OK;
+  const uint32_t npix = o->x_res * o->y_res;
+  const uint32_t spp = o->sample_end - o->sample_begin;
+  const uint64_t total = (uint64_t)npix * spp;
+  const size_t texels = s->tex_info_host.empty() ? 0 : (size_t)s->tex_info_host[0].x * s->tex_info_host[0].y;
+  const uint32_t nm = s->S.num_mats;
+  bool dev = o->flags & DGPU_BUF_DEVICE;
+  WaveParams P;
+  memset(&P, 0, sizeof P);
+  P.S = s->S;
+  P.xr = o->x_res; P.yr = o->y_res; P.npix = npix;
+  P.min_depth = o->min_depth; P.max_depth = o->max_depth; P.rr_depth = o->rr_depth;
+  P.only_direct = o->only_direct; P.sample_all = o->sample_all_lights; P.strategy = o->direct_strategy;
+  P.slots = 2;
+  dgpu_rng_key(o->seed, o->scene_id, &P.k0, &P.k1);
+  P.sample_begin = o->sample_begin;
+  uint32_t chunk = (uint32_t)std::min<uint64_t>(total, ADJ_CHUNK);
+  size_t threads = ((size_t)chunk + 63) / 64 * 64;
+  P.spill = (uint32_t*)ctx->buf("adj_spill", threads * s->stack_cap * 4, &st);
+  AdjArgs A;
+  memset(&A, 0, sizeof A);
+  A.inv_spp = 1.f / float(spp);
+  if (dev) {
+    A.dldi = dldi; A.g_tex = d_tex; A.g_rough = d_rough; A.image = image_out;
+  } else {
+    float* b_dldi = (float*)ctx->buf("adj_dldi", (size_t)npix * 12, &st);
+    float* b_tex = d_tex ? (float*)ctx->buf("adj_gtex", texels * 12, &st) : nullptr;
+    float* b_rough = d_rough ? (float*)ctx->buf("adj_grough", (size_t)nm * 4, &st) : nullptr;
+    float* b_img = image_out ? (float*)ctx->buf("adj_img", (size_t)npix * 12, &st) : nullptr;
+    if (st != DGPU_OK) return st;
+    HIP_TRY(hipMemcpyAsync(b_dldi, dldi, (size_t)npix * 12, hipMemcpyHostToDevice, stream));
+    if (b_tex) HIP_TRY(hipMemsetAsync(b_tex, 0, texels * 12, stream));
+    if (b_rough) HIP_TRY(hipMemsetAsync(b_rough, 0, (size_t)nm * 4, stream));
+    if (b_img) HIP_TRY(hipMemsetAsync(b_img, 0, (size_t)npix * 12, stream));
+    A.dldi = b_dldi; A.g_tex = b_tex; A.g_rough = b_rough; A.image = b_img;
+  }
+  if (st != DGPU_OK) return st;
+  HIP_TRY(hipEventRecord(ctx->ev0, stream));
+  for (uint64_t g0 = 0; g0 < total; g0 += chunk) {
+    P.g0 = g0;
+    P.n_chunk = (uint32_t)std::min<uint64_t>(chunk, total - g0);
+    HIP_TRY(launch_adjoint(P, A, s->S.width, stream));
+  }
+  HIP_TRY(hipEventRecord(ctx->ev1, stream));
+  std::vector<float> htex, hrough, himg;
+  if (!dev) {
+    if (d_tex) { htex.resize(texels * 3); HIP_TRY(hipMemcpyAsync(htex.data(), A.g_tex, texels * 12, hipMemcpyDeviceToHost, stream)); }
+    if (d_rough) { hrough.resize(nm); HIP_TRY(hipMemcpyAsync(hrough.data(), A.g_rough, (size_t)nm * 4, hipMemcpyDeviceToHost, stream)); }
+    if (image_out) { himg.resize((size_t)npix * 3); HIP_TRY(hipMemcpyAsync(himg.data(), A.image, (size_t)npix * 12, hipMemcpyDeviceToHost, stream)); }
+  }
+  HIP_TRY(hipStreamSynchronize(stream));
+  if (!dev) {
+    for (size_t i = 0; i < htex.size(); ++i) d_tex[i] += htex[i];
+    for (size_t i = 0; i < hrough.size(); ++i) d_rough[i] += hrough[i];
+    for (size_t i = 0; i < himg.size(); ++i) image_out[i] += himg[i];
+  }
+  if (out_stats) {
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    out_stats->paths = total;
+    out_stats->kernel_ms = ms;
   }
   return DGPU_OK;
 }

@@ -48,6 +48,17 @@ struct WaveParams {
   float* film_w;
 };
 
+struct AdjArgs {
+  const float* dldi;  // npix * 3, dL/dI
+  float* g_tex;       // texture-0 gradient, texels * 3 (or null)
+  float* g_rough;     // per-material roughness gradient (or null)
+  float* image;       // optional: += per-pixel mean of the pass-1 path radiance
+  float inv_spp;
+};
+
+// Paths per adjoint launch (bounds the traversal spill area: one slice per path).
+constexpr uint32_t ADJ_CHUNK = 1u << 20;
+
 bool variant_supported(uint32_t width, uint32_t lds, uint32_t block);
 // which: 0 closest-hit API, 1 any-hit API, 2 extend, 3 connect
 const void* trace_kernel_ptr(int which, uint32_t width, uint32_t lds, uint32_t block);
@@ -60,5 +71,6 @@ hipError_t launch_extend(const WaveParams& P, const KernelCfg& cfg, uint32_t bou
 hipError_t launch_shade(const WaveParams& P, uint32_t bounce, hipStream_t st);
 hipError_t launch_connect(const WaveParams& P, const KernelCfg& cfg, uint32_t bounce, hipStream_t st);
 hipError_t launch_finalize(const WaveParams& P, hipStream_t st);
+hipError_t launch_adjoint(const WaveParams& P, const AdjArgs& A, uint32_t width, hipStream_t st);
 
 }  // namespace dgpu

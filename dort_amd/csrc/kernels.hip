@@ -484,6 +484,188 @@ __global__ __launch_bounds__(256) void k_finalize(WaveParams P) {
   if (dropped) atomicAdd((unsigned long long*)&P.stats[3], (unsigned long long)dropped);
 }
 
+// ----------------------------------------------------------------- adjoint
+
+// Path-replay adjoint.  Each thread owns one path: pass 1 traces it to get its
+// total radiance, pass 2 re-traces it from the same counter-based random numbers
+// (memory stays O(pixels): nothing is taped) and deposits
+//   d loss / d theta = dldi . sum_j (d ln f_j / d theta) * (radiance carried by f_j)
+// for the albedo texels of texture 0 and every material's GGX roughness.  Sampling
+// pdfs, MIS weights and Russian-roulette decisions are held constant (detached),
+// which keeps the estimator unbiased for the gradient of the expected image.
+// The arithmetic per vertex is the same device code as the wavefront kernels
+// (surface_at, make_bsdf, estimate_direct, bsdf_sample).
+
+struct AdjSink {
+  float dl[3];          // dL/dI of the path's pixel divided by the sample count
+  float* g_tex;         // texture-0 gradient (RGB per texel) or null
+  float* g_rough;       // per-material roughness gradient or null
+  float* s_rough;       // workgroup partial sums for the first ADJ_LDS_MATS materials
+  uint32_t num_mats;
+};
+
+constexpr uint32_t ADJ_LDS_MATS = 64;
+constexpr uint32_t ADJ_BLOCK = 64;
+
+__device__ __forceinline__ void adj_deposit(const AdjSink& k, const Bsdf& b, int32_t texel, uint32_t mat,
+                                            V3 wi, V3 wo, V3 carried) {
+  if (b.kind == DGPU_MAT_LAMBERT && texel >= 0 && k.g_tex) {
+    // f = albedo / pi: d ln f_c / d albedo_c = 1 / albedo_c
+    float* g = k.g_tex + 3 * (size_t)texel;
+    if (b.albedo.x > 0.f) atomicAdd(g + 0, k.dl[0] * carried.x / b.albedo.x);
+    if (b.albedo.y > 0.f) atomicAdd(g + 1, k.dl[1] * carried.y / b.albedo.y);
+    if (b.albedo.z > 0.f) atomicAdd(g + 2, k.dl[2] * carried.z / b.albedo.z);
+  } else if (b.kind == DGPU_MAT_GGX_CONDUCTOR && k.g_rough) {
+    float r = conductor_dlnf_dalpha(b, b.to_local(wi), b.to_local(wo));
+    float v = r * (k.dl[0] * carried.x + k.dl[1] * carried.y + k.dl[2] * carried.z);
+    if (mat < ADJ_LDS_MATS) atomicAdd(k.s_rough + mat, v);
+    else atomicAdd(k.g_rough + mat, v);
+  }
+}
+
+template <bool ADJ, int WIDTH>
+__device__ V3 replay_path(const WaveParams& P, uint32_t pixel, uint32_t sample, LaneStack& st,
+                          V3 total, const AdjSink& sink, bool& film) {
+  const DevScene& S = P.S;
+  uint32_t x = pixel % P.xr, y = pixel / P.xr;
+  float jx = 0.5f, jy = 0.5f;
+  if (sample != 0) {
+    float r[4];
+    dgpu_rng_block(P.k0, P.k1, pixel, sample, 0, 2, r);
+    jx = r[2]; jy = r[3];
+  }
+  film = !(jx == 0.f || jy == 0.f);
+  TraceRay ray;
+  camera_ray(S, float(x) + jx, float(y) + jy, P.xr, P.yr, ray.o, ray.d);
+  ray.tmin = 0.f;
+  ray.tmax = INFINITY;
+  ray.ignore = 0xFFFFFFFFu;
+  V3 thr = v3(1.f, 1.f, 1.f), rad = v3(0.f, 0.f, 0.f);
+  uint32_t bounces = 0;
+  bool last_delta = false;
+  const uint32_t NON_DELTA = BSDF_ALL & ~BSDF_DELTA;
+  for (uint32_t bounce = 0; bounce < MAX_BOUNCES; ++bounce) {
+    TraceHit h;
+    bool is = trace_one<false, WIDTH>(S, ray, st, h);
+    Surface g;
+    if (is) g = surface_at(S, h.tri, h.t, h.b1, h.b2);
+    if ((bounces == 0u || last_delta) && bounces >= P.min_depth && bounces <= P.max_depth) {
+      if (is) {
+        int32_t li = S.tri_light[g.tri];
+        if (li >= 0 && dot(g.nn, ray.o - g.p) >= 0.f) rad = rad + thr * xyz(S.lights[li].radiance);
+      } else {
+        rad = rad + thr * background_radiance(S, ray.d);
+      }
+    }
+    bounces += 1u;
+    if (!is || bounces > P.max_depth) break;
+    V3 wo = normalize(-ray.d);
+    int32_t texel = -1;
+    Bsdf bsdf = make_bsdf(S, g, &texel);
+    uint32_t mat = S.tri_mat[g.tri];
+    float rb0[4], rb1[4], rb2[4];
+    dgpu_rng_block(P.k0, P.k1, pixel, sample, bounce, 0, rb0);
+    dgpu_rng_block(P.k0, P.k1, pixel, sample, bounce, 1, rb1);
+    dgpu_rng_block(P.k0, P.k1, pixel, sample, bounce, 2, rb2);
+    if (bounces >= P.min_depth && bsdf.matches(NON_DELTA) && S.num_lights > 0u) {
+      uint32_t nl_loop = P.sample_all ? S.num_lights : 1u;
+      for (uint32_t li = 0; li < nl_loop; ++li) {
+        NeeOut nb, nl;
+        nb.emit = false;
+        nl.emit = false;
+        if (P.sample_all) {
+          estimate_direct(P, g, wo, bsdf, li, 1.f, thr, rb0, rb1, nb, nl);
+        } else {
+          float u = rb0[0];
+          uint32_t b = 0, e = S.num_lights;
+          while (b + 1u < e) {
+            uint32_t m = b + (e - b) / 2u;
+            if (S.light_cdf[m] < u) b = m; else e = m;
+          }
+          float pp = S.light_cdf[b + 1] - S.light_cdf[b];
+          if (pp != 0.f) estimate_direct(P, g, wo, bsdf, b, 1.f / pp, thr, rb0, rb1, nb, nl);
+        }
+        NeeOut* outs[2] = {&nb, &nl};
+        for (int k = 0; k < 2; ++k) {
+          const NeeOut& n = *outs[k];
+          if (!n.emit) continue;
+          TraceRay sr;
+          sr.o = n.o; sr.d = n.d; sr.tmin = n.tmin; sr.tmax = n.tmax; sr.ignore = n.ignore;
+          TraceHit sh;
+          if (trace_one<true, WIDTH>(S, sr, st, sh)) continue;
+          rad = rad + n.contrib;
+          if (ADJ) adj_deposit(sink, bsdf, texel, mat, n.d, wo, n.contrib);
+        }
+      }
+    }
+    uint32_t req = (P.only_direct || bounces == P.max_depth) ? (BSDF_MODES | BSDF_DELTA) : BSDF_ALL;
+    V3 wi; float pdf; uint32_t fl;
+    V3 f = bsdf_sample(bsdf, wo, req, rb1[2], rb1[3], rb2[0], wi, pdf, fl);
+    if (is_black(f) || pdf == 0.f) break;
+    V3 contrib = f * (fabsf(dot(wi, g.nn)) / pdf);
+    bool delta = (fl & BSDF_DELTA) != 0u;
+    // radiance still to come through this continuation = total - accumulated so far
+    if (ADJ && !delta) adj_deposit(sink, bsdf, texel, mat, wi, wo, total - rad);
+    thr = thr * contrib;
+    last_delta = delta;
+    if (bounces == P.max_depth && !delta) break;
+    if (bounces >= P.rr_depth && bounces > P.min_depth) {
+      float q = clampf(average(contrib), 0.1f, 0.99f);
+      if (rb2[1] > q) break;
+      thr = thr / q;
+    }
+    ray.o = g.p; ray.d = wi; ray.tmin = g.eps; ray.tmax = INFINITY; ray.ignore = 0xFFFFFFFFu;
+  }
+  return rad;
+}
+
+template <int WIDTH>
+__global__ __launch_bounds__(ADJ_BLOCK) void k_adjoint(WaveParams P, AdjArgs A) {
+  __shared__ uint32_t stk[16 * ADJ_BLOCK];
+  __shared__ float s_rough[ADJ_LDS_MATS];
+  for (uint32_t k = threadIdx.x; k < ADJ_LDS_MATS; k += ADJ_BLOCK) s_rough[k] = 0.f;
+  __syncthreads();
+  LaneStack st;
+  st.lds = stk + threadIdx.x;
+  st.lds_stride = ADJ_BLOCK;
+  st.lds_cap = 16;
+  st.spill = P.spill + (size_t)blockIdx.x * ADJ_BLOCK + threadIdx.x;
+  st.spill_stride = (size_t)gridDim.x * ADJ_BLOCK;
+  st.cap = P.S.stack_cap;
+  uint32_t i = blockIdx.x * ADJ_BLOCK + threadIdx.x;
+  if (i < P.n_chunk) {
+    uint64_t g = P.g0 + i;
+    uint32_t sample = P.sample_begin + (uint32_t)(g / P.npix);
+    uint32_t pixel = (uint32_t)(g % P.npix);
+    AdjSink sink;
+    sink.dl[0] = A.dldi[3 * (size_t)pixel] * A.inv_spp;
+    sink.dl[1] = A.dldi[3 * (size_t)pixel + 1] * A.inv_spp;
+    sink.dl[2] = A.dldi[3 * (size_t)pixel + 2] * A.inv_spp;
+    sink.g_tex = A.g_tex;
+    sink.g_rough = A.g_rough;
+    sink.s_rough = s_rough;
+    sink.num_mats = P.S.num_mats;
+    bool film = true;
+    V3 total = replay_path<false, WIDTH>(P, pixel, sample, st, v3(0.f, 0.f, 0.f), sink, film);
+    bool ok = isfinite(total.x) && isfinite(total.y) && isfinite(total.z) && total.x >= 0.f &&
+              total.y >= 0.f && total.z >= 0.f;
+    if (ok && film) {
+      replay_path<true, WIDTH>(P, pixel, sample, st, total, sink, film);
+      if (A.image) {
+        atomicAdd(A.image + 3 * (size_t)pixel, total.x * A.inv_spp);
+        atomicAdd(A.image + 3 * (size_t)pixel + 1, total.y * A.inv_spp);
+        atomicAdd(A.image + 3 * (size_t)pixel + 2, total.z * A.inv_spp);
+      }
+    }
+  }
+  __syncthreads();
+  if (A.g_rough) {
+    uint32_t nm = P.S.num_mats < ADJ_LDS_MATS ? P.S.num_mats : ADJ_LDS_MATS;
+    for (uint32_t k = threadIdx.x; k < nm; k += ADJ_BLOCK)
+      if (s_rough[k] != 0.f) atomicAdd(A.g_rough + k, s_rough[k]);
+  }
+}
+
 // ----------------------------------------------------------------- launchers
 
 template <bool ANY, int WIDTH, int LDS, int BLOCK>
@@ -593,6 +775,13 @@ hipError_t launch_shade(const WaveParams& P, uint32_t bounce, hipStream_t st) {
 hipError_t launch_finalize(const WaveParams& P, hipStream_t st) {
   uint32_t blocks = (P.npix + 255u) / 256u;
   hipLaunchKernelGGL(k_finalize, dim3(blocks), dim3(256), 0, st, P);
+  return hipGetLastError();
+}
+
+hipError_t launch_adjoint(const WaveParams& P, const AdjArgs& A, uint32_t width, hipStream_t st) {
+  uint32_t blocks = (P.n_chunk + ADJ_BLOCK - 1u) / ADJ_BLOCK;
+  if (width == 4) hipLaunchKernelGGL(k_adjoint<4>, dim3(blocks), dim3(ADJ_BLOCK), 0, st, P, A);
+  else hipLaunchKernelGGL(k_adjoint<2>, dim3(blocks), dim3(ADJ_BLOCK), 0, st, P, A);
   return hipGetLastError();
 }
 

@@ -18,7 +18,8 @@ _lib = None
 
 EXPORTS = ["dgpu_abi_version", "dgpu_last_error", "dgpu_create", "dgpu_destroy", "dgpu_device_alloc",
            "dgpu_device_free", "dgpu_copy", "dgpu_scene_create", "dgpu_scene_destroy",
-           "dgpu_scene_get_info", "dgpu_trace_closest", "dgpu_trace_any", "dgpu_render"]
+           "dgpu_scene_get_info", "dgpu_scene_update_texture", "dgpu_scene_update_material",
+           "dgpu_trace_closest", "dgpu_trace_any", "dgpu_render", "dgpu_adjoint"]
 
 
 class DgpuError(RuntimeError):
@@ -50,8 +51,14 @@ def lib():
                                      C.c_uint32, C.POINTER(C.c_double)]
         L.dgpu_render.argtypes = [C.c_void_p, C.POINTER(abi.RenderOpts), C.c_void_p, C.c_void_p,
                                   C.POINTER(abi.RenderStats), C.c_void_p]
-        for n in ("dgpu_create", "dgpu_device_alloc", "dgpu_device_free", "dgpu_copy", "dgpu_scene_create", "dgpu_scene_get_info", "dgpu_trace_closest",
-                  "dgpu_trace_any", "dgpu_render"):
+        L.dgpu_scene_update_texture.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p]
+        L.dgpu_scene_update_material.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(abi.Material)]
+        L.dgpu_adjoint.argtypes = [C.c_void_p, C.POINTER(abi.RenderOpts), C.c_void_p, C.c_void_p,
+                                   C.c_void_p, C.c_void_p, C.POINTER(abi.RenderStats)]
+        for n in ("dgpu_create", "dgpu_device_alloc", "dgpu_device_free", "dgpu_copy",
+                  "dgpu_scene_create", "dgpu_scene_get_info", "dgpu_scene_update_texture",
+                  "dgpu_scene_update_material", "dgpu_trace_closest", "dgpu_trace_any", "dgpu_render",
+                  "dgpu_adjoint"):
             getattr(L, n).restype = C.c_int
         _lib = L
     return _lib
@@ -197,3 +204,27 @@ class Scene:
     def image(self, opts):
         rgb, w, _ = self.render(opts)
         return rgb / np.maximum(w, 1e-30)[..., None]
+
+    def update_texture(self, tex, rgb):
+        rgb = np.ascontiguousarray(rgb, dtype=np.float32)
+        check(lib().dgpu_scene_update_texture(self.h, tex, rgb.ctypes.data))
+
+    def update_material(self, index, material):
+        check(lib().dgpu_scene_update_material(self.h, index, C.byref(material)))
+
+    def adjoint(self, opts, dldi, want_texture=True, want_roughness=True, want_image=False):
+        """Gradient of sum(dldi * image) w.r.t. texture-0 texels and material roughness.
+        Returns (g_tex (h, w, 3) or None, g_rough (num_materials,) or None, image or None, stats)."""
+        dldi = np.ascontiguousarray(dldi, dtype=np.float32)
+        assert dldi.shape == (opts.y_res, opts.x_res, 3)
+        g_tex = None
+        if want_texture and self.scene.textures:
+            g_tex = np.zeros(self.scene.textures[0].shape, np.float32)
+        g_rough = np.zeros(len(self.scene.materials), np.float32) if want_roughness else None
+        img = np.zeros((opts.y_res, opts.x_res, 3), np.float32) if want_image else None
+        ptr = lambda a: None if a is None else a.ctypes.data
+        st = abi.RenderStats()
+        check(lib().dgpu_adjoint(self.h, C.byref(opts), dldi.ctypes.data, ptr(g_tex), ptr(g_rough),
+                                 ptr(img), C.byref(st)))
+        stats = {f: getattr(st, f) for f, _ in abi.RenderStats._fields_}
+        return g_tex, g_rough, img, stats

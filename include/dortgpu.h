@@ -216,6 +216,11 @@ dgpu_status dgpu_trace_closest(dgpu_scene* scene, const dgpu_rays* rays, uint32_
 dgpu_status dgpu_trace_any(dgpu_scene* scene, const dgpu_rays* rays, uint32_t n,
                            uint32_t* occluded, uint32_t flags, double* kernel_ms);
 
+/* Replace the texels of texture `tex` (width*height*3 fp32, host memory) or one
+ * material, without rebuilding the BVH (inverse rendering, finite differences). */
+dgpu_status dgpu_scene_update_texture(dgpu_scene* scene, uint32_t tex, const float* rgb);
+dgpu_status dgpu_scene_update_material(dgpu_scene* scene, uint32_t material, const dgpu_material* m);
+
 /* ---- path tracing (PathRenderer::render, path_renderer.cpp:11-206) -------- */
 
 /* Adds the samples of iterations [sample_begin, sample_end) to the film:
@@ -225,6 +230,20 @@ dgpu_status dgpu_trace_any(dgpu_scene* scene, const dgpu_rays* rays, uint32_t n,
 dgpu_status dgpu_render(dgpu_scene* scene, const dgpu_render_opts* opts,
                         float* film_rgb_sum, float* film_weight,
                         dgpu_stats* out_stats, const volatile int* cancel);
+
+/* ---- adjoint path replay (no dort counterpart) ----------------------------
+ * Gradient of L = sum_px dot(dL_dimage[px], I[px]), where I is the film mean of
+ * iterations [sample_begin, sample_end) rendered with `opts`, with respect to the
+ * texels of texture 0 (d_albedo_texels: width*height*3, may be NULL) and every
+ * material's GGX roughness (d_roughness: num_materials, may be NULL).  Each path is
+ * traced twice from its counter-based random numbers (forward, then replay with
+ * gradient deposits); memory stays O(pixels).  Outputs are accumulated (+=).
+ * With DGPU_BUF_DEVICE in opts->flags all three buffers are device pointers.
+ * image_out (optional, same memory kind) receives += the per-pixel mean radiance of
+ * the replayed paths. */
+dgpu_status dgpu_adjoint(dgpu_scene* scene, const dgpu_render_opts* opts, const float* dL_dimage,
+                         float* d_albedo_texels, float* d_roughness, float* image_out,
+                         dgpu_stats* out_stats);
 
 #ifdef __cplusplus
 }
