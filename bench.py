@@ -48,6 +48,9 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-intersect", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--calib", action="store_true",
+                    help="also run a 256 MiB device-to-device copy (counter calibration for rocprofv3 --pmc)")
     ap.add_argument("--width", type=int, default=2)
     ap.add_argument("--stack", type=int, default=16)
     ap.add_argument("--block", type=int, default=128)
@@ -94,6 +97,45 @@ def intersection_bench(ctx, args):
     for v in list(dev.values()) + [out[k] for k in ("t", "prim", "b1", "b2")] + [occ]:
         v.free()
     return res
+
+
+def adjoint_bench(ctx, args):
+    """Differentiable inverse rendering workload: 200k-triangle terrain, 512x512, 32 spp,
+    max depth 6; unknowns = 256x256 RGB albedo texture + 8 per-object GGX roughness;
+    loss = pixelwise L2 against a target rendered from hidden seeded parameters."""
+    from dort_amd import abi, scenes
+    hidden = scenes.terrain_scene(quads=316, seed=8, textured=True, tex_res=256, sky_res=(1024, 512),
+                                  ggx_roughness=np.linspace(0.08, 0.45, 8))
+    guess = scenes.terrain_scene(quads=316, seed=8, textured=True, tex_res=256, sky_res=(1024, 512),
+                                 ggx_roughness=np.full(8, 0.25))
+    guess.textures[0] = np.full_like(guess.textures[0], 0.5)
+    opts = abi.render_opts(512, 512, spp=32, seed=77, max_depth=6, rr_depth=3)
+    target = ctx.scene(hidden).image(opts)
+    gsc = ctx.scene(guess, width=args.width, max_leaf_size=args.leaf, stack_lds=args.stack,
+                    block_size=args.block)
+    n_params = guess.textures[0].size + 8
+    # warm up, then time forward + adjoint (one full gradient vector)
+    img = gsc.image(opts)
+    gsc.adjoint(opts, 2 * (img - target))
+    fwd, adj = [], []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        img = gsc.image(opts)
+        t1 = time.perf_counter()
+        g_tex, g_rough, _, st = gsc.adjoint(opts, 2 * (img - target))
+        t2 = time.perf_counter()
+        fwd.append(t1 - t0)
+        adj.append(t2 - t1)
+    t_fwd, t_adj = float(np.median(fwd)), float(np.median(adj))
+    grads_per_s = 1.0 / (t_fwd + t_adj)
+    fd_grads_per_s = 1.0 / (2 * n_params * t_fwd)
+    return {"workload": f"terrain {guess.num_triangles} tris, 512x512, 32 spp, depth 6, "
+                        f"{n_params} unknowns (256x256x3 albedo + 8 roughness)",
+            "grads_per_s": grads_per_s, "fd_grads_per_s": fd_grads_per_s,
+            "speedup_vs_fd": grads_per_s / fd_grads_per_s, "forward_ms": t_fwd * 1e3,
+            "adjoint_ms": t_adj * 1e3, "adjoint_kernel_ms": st["kernel_ms"],
+            "loss": float(((img - target) ** 2).sum()),
+            "grad_norm": float(np.sqrt((g_tex.astype(np.float64) ** 2).sum() + (g_rough ** 2).sum()))}
 
 
 def cpu_baseline(sc, args):
@@ -174,6 +216,16 @@ def main():
                 "other_ms": round(ps["other_ms"], 2)}
     log(f"profile: {roofline['kernel_ms']} dominant {dom} avg {avg_ms:.3f} ms, {achieved:.1f} GB/s")
 
+    if args.calib:
+        nb = 256 << 20
+        a = gpu.DeviceArray(ctx, nb // 4, np.float32)
+        b = gpu.DeviceArray(ctx, nb // 4, np.float32)
+        for _ in range(2):
+            gpu.check(gpu.lib().dgpu_copy(ctx.h, b.ptr, a.ptr, nb, 2))
+        a.free()
+        b.free()
+        log("calibration copy: 2 x 256 MiB device-to-device")
+
     secondary = {}
     if not args.no_intersect:
         film.free()
@@ -181,6 +233,12 @@ def main():
         secondary["intersection"] = intersection_bench(ctx, args)
         log(f"intersection: {secondary['intersection']['Mrays_closest']:.0f} Mrays/s closest, "
             f"{secondary['intersection']['Mrays_any']:.0f} Mrays/s any")
+
+    if not args.no_adjoint:
+        secondary["adjoint"] = adjoint_bench(ctx, args)
+        a = secondary["adjoint"]
+        log(f"adjoint: {a['grads_per_s']:.2f} grads/s, FD {a['fd_grads_per_s']:.2e} grads/s "
+            f"({a['speedup_vs_fd']:.0f}x), forward {a['forward_ms']:.0f} ms, adjoint {a['adjoint_ms']:.0f} ms")
 
     cpu = None
     if not args.no_cpu:

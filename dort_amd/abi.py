@@ -15,6 +15,7 @@ MAT_NONE, MAT_LAMBERT, MAT_GGX_CONDUCTOR, MAT_DIELECTRIC = range(4)
 DIRECT_MIS, DIRECT_BSDF, DIRECT_LIGHT = range(3)
 BUF_DEVICE = 1
 RENDER_PROFILE = 2
+BATCH_NO_GRAPH = 4
 
 f3 = C.c_float * 3
 
@@ -74,6 +75,11 @@ class RenderOpts(C.Structure):
                 ("rr_depth", C.c_uint32), ("only_direct", C.c_uint32),
                 ("sample_all_lights", C.c_uint32), ("direct_strategy", C.c_uint32),
                 ("flags", C.c_uint32), ("scene_id", C.c_uint32)]
+
+
+class BatchItem(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("opts", RenderOpts), ("film_rgb_sum", C.c_void_p),
+                ("film_weight", C.c_void_p)]
 
 
 class RenderStats(C.Structure):

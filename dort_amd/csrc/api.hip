@@ -608,6 +608,7 @@ dgpu_status dgpu_render(dgpu_scene* s, const dgpu_render_opts* o, float* film_rg
   P.counters = (uint32_t*)ctx->buf("counters", NUM_COUNTERS * 4, &st);
   P.stats = (unsigned long long*)ctx->buf("stats", 64, &st);
   P.spill = (uint32_t*)ctx->buf("spill", (size_t)maxgrid * s->cfg.block * s->stack_cap * 4, &st);
+  P.stack_cap = s->stack_cap;
   bool dev = o->flags & DGPU_BUF_DEVICE;
   if (dev) {
     P.film_rgb = film_rgb_sum;
@@ -641,14 +642,14 @@ dgpu_status dgpu_render(dgpu_scene* s, const dgpu_render_opts* o, float* film_rg
   for (uint64_t g0 = 0; g0 < total && !cancelled; g0 += C) {
     P.g0 = g0;
     P.n_chunk = (uint32_t)std::min<uint64_t>(C, total - g0);
-    HIP_TRY(timed(3, [&] { return launch_raygen(P, stream); }));
+    HIP_TRY(timed(3, [&] { return launch_raygen(P, P.n_chunk, false, stream); }));
     for (uint32_t b = 0; b <= o->max_depth; ++b) {
       if (cancel && *cancel) { cancelled = true; break; }
-      HIP_TRY(timed(0, [&] { return launch_extend(P, ce, b, stream); }));
-      HIP_TRY(timed(1, [&] { return launch_shade(P, b, stream); }));
-      HIP_TRY(timed(2, [&] { return launch_connect(P, cc, b, stream); }));
+      HIP_TRY(timed(0, [&] { return launch_extend(P, ce, b, false, stream); }));
+      HIP_TRY(timed(1, [&] { return launch_shade(P, P.n_chunk, b, false, stream); }));
+      HIP_TRY(timed(2, [&] { return launch_connect(P, cc, b, false, stream); }));
     }
-    if (!cancelled) HIP_TRY(timed(3, [&] { return launch_finalize(P, stream); }));
+    if (!cancelled) HIP_TRY(timed(3, [&] { return launch_finalize(P, P.n_chunk, false, stream); }));
   }
   HIP_TRY(hipEventRecord(ctx->ev1, stream));
   unsigned long long hstats[4] = {0, 0, 0, 0};
@@ -795,6 +796,224 @@ dgpu_status dgpu_adjoint(dgpu_scene* s, const dgpu_render_opts* o, const float* 
     out_stats->paths = total;
     out_stats->kernel_ms = ms;
   }
+  return DGPU_OK;
+}
+
+dgpu_status dgpu_render_batch(dgpu_ctx* ctx, const dgpu_batch_item* items, uint32_t n, uint32_t flags,
+                              dgpu_stats* out_stats) {
+  if (!ctx || !items) return fail(DGPU_EINVAL, "null context or items");
+  if (n == 0 || n > MAX_BATCH) return fail(DGPU_EINVAL, "batch size must be in [1, %u]", MAX_BATCH);
+  const dgpu_render_opts& o0 = items[0].opts;
+  const dgpu_scene* s0 = items[0].scene;
+  if (!s0) return fail(DGPU_EINVAL, "item 0: null scene");
+  for (uint32_t i = 0; i < n; ++i) {
+    const dgpu_batch_item& it = items[i];
+    const dgpu_render_opts& o = it.opts;
+    if (!it.scene || it.scene->ctx != ctx) return fail(DGPU_EINVAL, "item %u: scene of another context", i);
+    if (!it.film_rgb_sum || !it.film_weight) return fail(DGPU_EINVAL, "item %u: null film", i);
+    if (o.x_res == 0 || o.y_res == 0) return fail(DGPU_EINVAL, "item %u: empty film", i);
+    if ((uint64_t)o.x_res * o.y_res > (1u << 30)) return fail(DGPU_EINVAL, "item %u: film too large", i);
+    if (o.sample_end < o.sample_begin) return fail(DGPU_EINVAL, "item %u: sample_end < sample_begin", i);
+    if (o.min_depth != o0.min_depth || o.max_depth != o0.max_depth || o.rr_depth != o0.rr_depth ||
+        o.only_direct != o0.only_direct || o.direct_strategy != o0.direct_strategy)
+      return fail(DGPU_EINVAL, "item %u: depth / strategy options differ from item 0", i);
+    if (o.sample_all_lights) return fail(DGPU_EUNSUPPORTED, "batches do not support sample_all_lights");
+    const KernelCfg& c = it.scene->cfg;
+    if (c.width != s0->cfg.width || c.lds != s0->cfg.lds || c.block != s0->cfg.block)
+      return fail(DGPU_EINVAL, "item %u: BVH width / kernel knobs differ from item 0", i);
+  }
+  if (o0.max_depth + 2 >= MAX_BOUNCES) return fail(DGPU_EINVAL, "max_depth must be < %u", MAX_BOUNCES - 2);
+  if (o0.direct_strategy > DGPU_DIRECT_LIGHT) return fail(DGPU_EINVAL, "bad direct_strategy");
+  if (out_stats) memset(out_stats, 0, sizeof *out_stats);
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HIP_TRY(hipSetDevice(ctx->device));
+  hipStream_t stream = ctx->stream;
+  dgpu_status st = DGPU_OK;
+  const bool dev = flags & DGPU_BUF_DEVICE;
+  const bool use_graph = !(flags & DGPU_BATCH_NO_GRAPH);
+
+  // scene table and (for host films) one device film region
+  std::vector<BatchEntry> tab(n);
+  std::vector<uint64_t> paths(n);
+  size_t film_floats = 0;
+  uint32_t stack_cap = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    const dgpu_render_opts& o = items[i].opts;
+    BatchEntry& E = tab[i];
+    memset(&E, 0, sizeof E);
+    E.S = items[i].scene->S;
+    E.xr = o.x_res; E.yr = o.y_res; E.npix = o.x_res * o.y_res;
+    E.sample_begin = o.sample_begin;
+    dgpu_rng_key(o.seed, o.scene_id, &E.k0, &E.k1);
+    paths[i] = (uint64_t)E.npix * (o.sample_end - o.sample_begin);
+    film_floats += 4 * (size_t)E.npix;
+    stack_cap = std::max(stack_cap, items[i].scene->stack_cap);
+  }
+  float* dfilm = nullptr;
+  if (!dev) {
+    dfilm = (float*)ctx->buf("batch_film", film_floats * 4, &st);
+    if (st != DGPU_OK) return st;
+    HIP_TRY(hipMemsetAsync(dfilm, 0, film_floats * 4, stream));
+  }
+  size_t off = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    if (dev) {
+      tab[i].film_rgb = items[i].film_rgb_sum;
+      tab[i].film_w = items[i].film_weight;
+    } else {
+      tab[i].film_rgb = dfilm + off;
+      tab[i].film_w = dfilm + off + 3 * (size_t)tab[i].npix;
+      off += 4 * (size_t)tab[i].npix;
+    }
+  }
+
+  // chunks over the scene-major concatenation of every scene's paths
+  uint32_t C = s0->queue_tile ? s0->queue_tile : (1u << 21);
+  C = std::min(C, 1u << 23);  // slot indices (x2 NEE slots) must fit in 24 bits
+  std::vector<ChunkDesc> chunks;
+  {
+    uint32_t si = 0;
+    uint64_t local = 0;
+    while (si < n && paths[si] == 0) ++si;
+    while (si < n) {
+      ChunkDesc cd;
+      memset(&cd, 0, sizeof cd);
+      while (si < n && cd.n_chunk < C && cd.n_segs < MAX_CHUNK_SEGS) {
+        uint64_t take = std::min<uint64_t>(paths[si] - local, C - cd.n_chunk);
+        ChunkSeg& sg = cd.segs[cd.n_segs++];
+        sg.scene = si;
+        sg.slot_begin = cd.n_chunk;
+        sg.pair_begin = cd.n_pairs;
+        sg.local_begin = local;
+        sg.local_end = local + take;
+        sg.n_pairs = (uint32_t)std::min<uint64_t>(tab[si].npix, take);
+        cd.n_chunk += (uint32_t)take;
+        cd.n_pairs += sg.n_pairs;
+        local += take;
+        if (local == paths[si]) {
+          ++si;
+          local = 0;
+          while (si < n && paths[si] == 0) ++si;
+        }
+      }
+      chunks.push_back(cd);
+    }
+  }
+  uint64_t total = 0;
+  for (uint64_t p : paths) total += p;
+  if (chunks.empty()) return DGPU_OK;
+  uint32_t cap = 0;
+  for (const ChunkDesc& cd : chunks) cap = std::max(cap, cd.n_chunk);
+
+  KernelCfg ce = s0->cfg, cc = s0->cfg;
+  ce.grid = ctx->grid_for(4, ce);
+  cc.grid = ctx->grid_for(5, cc);
+  uint32_t maxgrid = std::max(ce.grid, cc.grid);
+  WaveParams P;
+  memset(&P, 0, sizeof P);
+  P.S = s0->S;
+  P.min_depth = o0.min_depth; P.max_depth = o0.max_depth; P.rr_depth = o0.rr_depth;
+  P.only_direct = o0.only_direct; P.sample_all = 0; P.strategy = o0.direct_strategy;
+  P.slots = 2;
+  P.thr = (float4*)ctx->buf("thr", (size_t)cap * 16, &st);
+  P.rad = (float4*)ctx->buf("rad", (size_t)cap * 16, &st);
+  P.pend = (float4*)ctx->buf("pend", (size_t)cap * 2 * 16, &st);
+  P.vis = (uint8_t*)ctx->buf("vis", (size_t)cap * 2, &st);
+  P.qo[0] = (float4*)ctx->buf("qo0", (size_t)cap * 16, &st);
+  P.qo[1] = (float4*)ctx->buf("qo1", (size_t)cap * 16, &st);
+  P.qd[0] = (float4*)ctx->buf("qd0", (size_t)cap * 16, &st);
+  P.qd[1] = (float4*)ctx->buf("qd1", (size_t)cap * 16, &st);
+  P.qpath[0] = (uint32_t*)ctx->buf("qp0", (size_t)cap * 4, &st);
+  P.qpath[1] = (uint32_t*)ctx->buf("qp1", (size_t)cap * 4, &st);
+  P.hit = (float4*)ctx->buf("hit", (size_t)cap * 16, &st);
+  P.so = (float4*)ctx->buf("so", (size_t)cap * 2 * 16, &st);
+  P.sd = (float4*)ctx->buf("sd", (size_t)cap * 2 * 16, &st);
+  P.smeta = (uint2*)ctx->buf("smeta", (size_t)cap * 2 * 8, &st);
+  P.pid = (uint2*)ctx->buf("pid", (size_t)cap * 8, &st);
+  P.counters = (uint32_t*)ctx->buf("counters", NUM_COUNTERS * 4, &st);
+  P.stats = (unsigned long long*)ctx->buf("stats", 64, &st);
+  P.spill = (uint32_t*)ctx->buf("spill", (size_t)maxgrid * s0->cfg.block * stack_cap * 4, &st);
+  P.stack_cap = stack_cap;
+  BatchEntry* dtab = (BatchEntry*)ctx->buf("batch_tab", n * sizeof(BatchEntry), &st);
+  ChunkDesc* dchunks = (ChunkDesc*)ctx->buf("batch_chunks", chunks.size() * sizeof(ChunkDesc), &st);
+  uint32_t* dcur = (uint32_t*)ctx->buf("batch_cur", 64, &st);
+  if (st != DGPU_OK) return st;
+  P.batch = dtab;
+  P.num_scenes = n;
+  P.chunks = dchunks;
+  P.chunk_cur = dcur;
+  HIP_TRY(hipMemcpyAsync(dtab, tab.data(), n * sizeof(BatchEntry), hipMemcpyHostToDevice, stream));
+  HIP_TRY(hipMemcpyAsync(dchunks, chunks.data(), chunks.size() * sizeof(ChunkDesc), hipMemcpyHostToDevice, stream));
+  HIP_TRY(hipMemsetAsync(dcur, 0, 4, stream));
+  HIP_TRY(hipMemsetAsync(P.stats, 0, 64, stream));
+  // tab and chunks must stay alive until the copies above have run
+  HIP_TRY(hipStreamSynchronize(stream));
+
+  // one chunk: raygen -> (extend -> shade -> connect) x (max_depth + 1) -> finalize -> advance
+  auto chunk_sequence = [&]() -> hipError_t {
+    hipError_t e = launch_raygen(P, cap, true, stream);
+    for (uint32_t b = 0; e == hipSuccess && b <= o0.max_depth; ++b) {
+      e = launch_extend(P, ce, b, true, stream);
+      if (e == hipSuccess) e = launch_shade(P, cap, b, true, stream);
+      if (e == hipSuccess) e = launch_connect(P, cc, b, true, stream);
+    }
+    if (e == hipSuccess) e = launch_finalize(P, cap, true, stream);
+    if (e == hipSuccess) e = launch_advance(dcur, stream);
+    return e;
+  };
+  HIP_TRY(hipEventRecord(ctx->ev0, stream));
+  if (use_graph) {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    hipError_t ce_ = chunk_sequence();
+    hipError_t ee = hipStreamEndCapture(stream, &graph);
+    if (ce_ != hipSuccess || ee != hipSuccess) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return fail(DGPU_EDEVICE, "graph capture failed: %s", hipGetErrorString(ce_ != hipSuccess ? ce_ : ee));
+    }
+    hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (ie != hipSuccess) {
+      (void)hipGraphDestroy(graph);
+      return fail(DGPU_EDEVICE, "hipGraphInstantiate failed: %s", hipGetErrorString(ie));
+    }
+    hipError_t le = hipSuccess;
+    for (size_t c = 0; c < chunks.size() && le == hipSuccess; ++c) le = hipGraphLaunch(exec, stream);
+    (void)hipStreamSynchronize(stream);
+    (void)hipGraphExecDestroy(exec);
+    (void)hipGraphDestroy(graph);
+    if (le != hipSuccess) return fail(DGPU_EDEVICE, "hipGraphLaunch failed: %s", hipGetErrorString(le));
+  } else {
+    for (size_t c = 0; c < chunks.size(); ++c) HIP_TRY(chunk_sequence());
+  }
+  HIP_TRY(hipEventRecord(ctx->ev1, stream));
+  unsigned long long hstats[4] = {0, 0, 0, 0};
+  HIP_TRY(hipMemcpyAsync(hstats, P.stats, sizeof hstats, hipMemcpyDeviceToHost, stream));
+  std::vector<float> hfilm;
+  if (!dev) {
+    hfilm.resize(film_floats);
+    HIP_TRY(hipMemcpyAsync(hfilm.data(), dfilm, film_floats * 4, hipMemcpyDeviceToHost, stream));
+  }
+  HIP_TRY(hipStreamSynchronize(stream));
+  if (!dev) {
+    size_t o = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+      size_t np = tab[i].npix;
+      for (size_t k = 0; k < 3 * np; ++k) items[i].film_rgb_sum[k] += hfilm[o + k];
+      for (size_t k = 0; k < np; ++k) items[i].film_weight[k] += hfilm[o + 3 * np + k];
+      o += 4 * np;
+    }
+  }
+  if (out_stats) {
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    out_stats->paths = hstats[0];
+    out_stats->extension_rays = hstats[1];
+    out_stats->shadow_rays = hstats[2];
+    out_stats->dropped_samples = hstats[3];
+    out_stats->kernel_ms = ms;
+  }
+  (void)total;
   return DGPU_OK;
 }
 

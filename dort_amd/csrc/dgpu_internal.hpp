@@ -13,11 +13,36 @@ constexpr uint32_t CNT_FETCH_SHD = 192;
 constexpr uint32_t NUM_COUNTERS = 256;
 constexpr uint32_t MAX_SLOT_LIGHTS = 8;  // sample_all_lights supports up to this many lights
 
+constexpr uint32_t MAX_BATCH = 255;      // scenes per batch (scene index in the top byte)
+constexpr uint32_t MAX_CHUNK_SEGS = 64;  // scenes overlapping one chunk of a batch
+
 struct KernelCfg {
   uint32_t width;  // BVH width
   uint32_t lds;    // traversal stack entries in LDS per lane
   uint32_t block;  // traversal workgroup size
   uint32_t grid;   // persistent grid
+};
+
+// One scene of a batch render.
+struct BatchEntry {
+  DevScene S;
+  uint32_t xr, yr, npix, sample_begin;
+  uint32_t k0, k1, pad0, pad1;
+  float* film_rgb;
+  float* film_w;
+};
+
+// The part of one scene's paths that a batch chunk holds: scene-local path indices
+// [local_begin, local_end) at chunk slots [slot_begin, ...); film pairs (one per
+// distinct pixel) start at pair_begin.
+struct ChunkSeg {
+  uint32_t scene, slot_begin, pair_begin, n_pairs;
+  uint64_t local_begin, local_end;
+};
+
+struct ChunkDesc {
+  uint32_t n_chunk, n_segs, n_pairs, pad;
+  ChunkSeg segs[MAX_CHUNK_SEGS];
 };
 
 struct WaveParams {
@@ -44,8 +69,15 @@ struct WaveParams {
   uint32_t* counters;
   unsigned long long* stats;  // paths, extension rays, shadow rays, dropped samples
   uint32_t* spill;
+  uint32_t stack_cap;
   float* film_rgb;
   float* film_w;
+  // batch rendering
+  const BatchEntry* batch;
+  uint32_t num_scenes;
+  const ChunkDesc* chunks;
+  uint32_t* chunk_cur;     // index of the chunk being rendered (advanced on the device)
+  uint2* pid;              // per path slot: (pixel, sample)
 };
 
 struct AdjArgs {
@@ -60,17 +92,20 @@ struct AdjArgs {
 constexpr uint32_t ADJ_CHUNK = 1u << 20;
 
 bool variant_supported(uint32_t width, uint32_t lds, uint32_t block);
-// which: 0 closest-hit API, 1 any-hit API, 2 extend, 3 connect
+// which: 0 closest-hit API, 1 any-hit API, 2 extend, 3 connect, 4/5 batched extend/connect
 const void* trace_kernel_ptr(int which, uint32_t width, uint32_t lds, uint32_t block);
 
 hipError_t launch_trace_api(const DevScene& S, const KernelCfg& cfg, const dgpu_rays& rays,
                             const dgpu_hits* hits, uint32_t* occluded, uint32_t n, uint32_t* fetch,
                             uint32_t* spill, hipStream_t st);
-hipError_t launch_raygen(const WaveParams& P, hipStream_t st);
-hipError_t launch_extend(const WaveParams& P, const KernelCfg& cfg, uint32_t bounce, hipStream_t st);
-hipError_t launch_shade(const WaveParams& P, uint32_t bounce, hipStream_t st);
-hipError_t launch_connect(const WaveParams& P, const KernelCfg& cfg, uint32_t bounce, hipStream_t st);
-hipError_t launch_finalize(const WaveParams& P, hipStream_t st);
+hipError_t launch_raygen(const WaveParams& P, uint32_t capacity, bool batch, hipStream_t st);
+hipError_t launch_extend(const WaveParams& P, const KernelCfg& cfg, uint32_t bounce, bool batch,
+                         hipStream_t st);
+hipError_t launch_shade(const WaveParams& P, uint32_t capacity, uint32_t bounce, bool batch, hipStream_t st);
+hipError_t launch_connect(const WaveParams& P, const KernelCfg& cfg, uint32_t bounce, bool batch,
+                          hipStream_t st);
+hipError_t launch_finalize(const WaveParams& P, uint32_t capacity, bool batch, hipStream_t st);
+hipError_t launch_advance(uint32_t* cur, hipStream_t st);
 hipError_t launch_adjoint(const WaveParams& P, const AdjArgs& A, uint32_t width, hipStream_t st);
 
 }  // namespace dgpu

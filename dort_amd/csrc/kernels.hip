@@ -41,17 +41,21 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred) {
 
 // ----------------------------------------------------------------- trace kernels
 
+struct BvhRef {
+  const float4* nodes;
+  const float4* tris;
+};
+
 template <bool ANY, int WIDTH>
-__device__ __forceinline__ bool trace_one(const DevScene& S, const TraceRay& r, LaneStack& st,
-                                          TraceHit& h) {
-  if (WIDTH == 4) return traverse_bvh4<ANY>(S.nodes, S.tris, r, st, h);
-  return traverse_bvh2<ANY>(S.nodes, S.tris, r, st, h);
+__device__ __forceinline__ bool trace_one(BvhRef b, const TraceRay& r, LaneStack& st, TraceHit& h) {
+  if (WIDTH == 4) return traverse_bvh4<ANY>(b.nodes, b.tris, r, st, h);
+  return traverse_bvh2<ANY>(b.nodes, b.tris, r, st, h);
 }
 
-// Shared persistent loop: `io.load(i, ray)` / `io.store(i, hit, found)`.
+// Shared persistent loop: `io.load(i, ray)` returns the ray's BVH, `io.store(i, hit, found)`.
 template <bool ANY, int WIDTH, int LDS, int BLOCK, class IO>
-__device__ __forceinline__ void trace_loop(const DevScene& S, IO& io, uint32_t n, uint32_t* fetch,
-                                           uint32_t* spill) {
+__device__ __forceinline__ void trace_loop(IO& io, uint32_t n, uint32_t* fetch, uint32_t* spill,
+                                           uint32_t stack_cap) {
   __shared__ uint32_t stk[LDS * BLOCK];
   LaneStack st;
   st.lds = stk + threadIdx.x;
@@ -60,7 +64,7 @@ __device__ __forceinline__ void trace_loop(const DevScene& S, IO& io, uint32_t n
   size_t gthreads = (size_t)gridDim.x * BLOCK;
   st.spill = spill + (size_t)blockIdx.x * BLOCK + threadIdx.x;
   st.spill_stride = gthreads;
-  st.cap = S.stack_cap;
+  st.cap = stack_cap;
   for (;;) {
     uint32_t base = 0;
     if (lane_id() == 0) base = atomicAdd(fetch, 64u);
@@ -69,9 +73,9 @@ __device__ __forceinline__ void trace_loop(const DevScene& S, IO& io, uint32_t n
     uint32_t i = base + lane_id();
     if (i < n) {
       TraceRay r;
-      io.load(i, r);
+      BvhRef bvh = io.load(i, r);
       TraceHit h;
-      bool found = trace_one<ANY, WIDTH>(S, r, st, h);
+      bool found = trace_one<ANY, WIDTH>(bvh, r, st, h);
       io.store(i, h, found);
     }
   }
@@ -81,70 +85,105 @@ struct ApiIO {
   dgpu_rays rays;
   dgpu_hits hits;
   uint32_t* occluded;
-  const float4* tris;
-  __device__ __forceinline__ void load(uint32_t i, TraceRay& r) const {
+  BvhRef bvh;
+  __device__ __forceinline__ BvhRef load(uint32_t i, TraceRay& r) const {
     r.o = v3(rays.ox[i], rays.oy[i], rays.oz[i]);
     r.d = v3(rays.dx[i], rays.dy[i], rays.dz[i]);
     r.tmin = rays.tmin[i];
     r.tmax = rays.tmax[i];
     r.ignore = 0xFFFFFFFFu;
+    return bvh;
   }
   __device__ __forceinline__ void store(uint32_t i, const TraceHit& h, bool found) const {
     if (occluded) { occluded[i] = found ? 1u : 0u; return; }
     hits.t[i] = found ? h.t : INFINITY;
-    hits.prim[i] = found ? (int32_t)__float_as_uint(tris[3 * h.tri].w) : -1;
+    hits.prim[i] = found ? (int32_t)__float_as_uint(bvh.tris[3 * h.tri].w) : -1;
     hits.b1[i] = found ? h.b1 : 0.f;
     hits.b2[i] = found ? h.b2 : 0.f;
   }
 };
 
 template <bool ANY, int WIDTH, int LDS, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_trace_api(DevScene S, ApiIO io, uint32_t n,
-                                                     uint32_t* fetch, uint32_t* spill) {
-  trace_loop<ANY, WIDTH, LDS, BLOCK>(S, io, n, fetch, spill);
+__global__ __launch_bounds__(BLOCK) void k_trace_api(ApiIO io, uint32_t n, uint32_t* fetch,
+                                                     uint32_t* spill, uint32_t stack_cap) {
+  trace_loop<ANY, WIDTH, LDS, BLOCK>(io, n, fetch, spill, stack_cap);
 }
 
+// Batched queues carry the scene index in the top byte of the path / slot word.
+constexpr uint32_t SCENE_SHIFT = 24;
+constexpr uint32_t LOW24 = 0x00FFFFFFu;
+
+template <bool BATCH>
 struct ExtendIO {
   const float4* qo;
   const float4* qd;
+  const uint32_t* qpath;
   float4* hit;
-  __device__ __forceinline__ void load(uint32_t i, TraceRay& r) const {
+  BvhRef bvh;            // single scene
+  const BvhRef* table;   // batch: per-scene BVHs (in LDS)
+  __device__ __forceinline__ BvhRef load(uint32_t i, TraceRay& r) const {
     float4 o = qo[i], d = qd[i];
     r.o = xyz(o); r.tmin = o.w;
     r.d = xyz(d); r.tmax = d.w;
     r.ignore = 0xFFFFFFFFu;
+    if (BATCH) return table[qpath[i] >> SCENE_SHIFT];
+    return bvh;
   }
   __device__ __forceinline__ void store(uint32_t i, const TraceHit& h, bool found) const {
     hit[i] = make_float4(found ? h.t : INFINITY, h.b1, h.b2, __uint_as_float(found ? h.tri : 0xFFFFFFFFu));
   }
 };
 
+template <bool BATCH>
 struct ConnectIO {
   const float4* so;
   const float4* sd;
-  const uint2* meta;   // (pending slot, ignored triangle)
+  const uint2* meta;   // (pending slot [| scene << 24], ignored triangle)
   uint8_t* vis;
-  __device__ __forceinline__ void load(uint32_t i, TraceRay& r) const {
+  BvhRef bvh;
+  const BvhRef* table;
+  __device__ __forceinline__ BvhRef load(uint32_t i, TraceRay& r) const {
     float4 o = so[i], d = sd[i];
     r.o = xyz(o); r.tmin = o.w;
     r.d = xyz(d); r.tmax = d.w;
-    r.ignore = meta[i].y;
+    uint2 m = meta[i];
+    r.ignore = m.y;
+    if (BATCH) return table[m.x >> SCENE_SHIFT];
+    return bvh;
   }
   __device__ __forceinline__ void store(uint32_t i, const TraceHit&, bool found) const {
-    if (found) vis[meta[i].x] = 0;
+    if (found) vis[BATCH ? (meta[i].x & LOW24) : meta[i].x] = 0;
   }
 };
 
-template <int WIDTH, int LDS, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_extend(DevScene S, ExtendIO io, const uint32_t* count,
-                                                  uint32_t* fetch, uint32_t* spill) {
-  trace_loop<false, WIDTH, LDS, BLOCK>(S, io, *count, fetch, spill);
+template <bool BATCH, int BLOCK>
+__device__ __forceinline__ const BvhRef* load_bvh_table(const BatchEntry* batch, uint32_t nscenes,
+                                                        BvhRef* lds) {
+  if (!BATCH) return nullptr;
+  for (uint32_t k = threadIdx.x; k < nscenes; k += BLOCK) {
+    lds[k].nodes = batch[k].S.nodes;
+    lds[k].tris = batch[k].S.tris;
+  }
+  __syncthreads();
+  return lds;
 }
 
-template <int WIDTH, int LDS, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_connect(DevScene S, ConnectIO io, const uint32_t* count,
-                                                   uint32_t* fetch, uint32_t* spill) {
-  trace_loop<true, WIDTH, LDS, BLOCK>(S, io, *count, fetch, spill);
+template <int WIDTH, int LDS, int BLOCK, bool BATCH>
+__global__ __launch_bounds__(BLOCK) void k_extend(ExtendIO<BATCH> io, const uint32_t* count,
+                                                  uint32_t* fetch, uint32_t* spill, uint32_t stack_cap,
+                                                  const BatchEntry* batch, uint32_t nscenes) {
+  __shared__ BvhRef tab[BATCH ? MAX_BATCH : 1];
+  io.table = load_bvh_table<BATCH, BLOCK>(batch, nscenes, tab);
+  trace_loop<false, WIDTH, LDS, BLOCK>(io, *count, fetch, spill, stack_cap);
+}
+
+template <int WIDTH, int LDS, int BLOCK, bool BATCH>
+__global__ __launch_bounds__(BLOCK) void k_connect(ConnectIO<BATCH> io, const uint32_t* count,
+                                                   uint32_t* fetch, uint32_t* spill, uint32_t stack_cap,
+                                                   const BatchEntry* batch, uint32_t nscenes) {
+  __shared__ BvhRef tab[BATCH ? MAX_BATCH : 1];
+  io.table = load_bvh_table<BATCH, BLOCK>(batch, nscenes, tab);
+  trace_loop<true, WIDTH, LDS, BLOCK>(io, *count, fetch, spill, stack_cap);
 }
 
 __global__ void k_zero_u32(uint32_t* p, uint32_t n) {
@@ -152,32 +191,68 @@ __global__ void k_zero_u32(uint32_t* p, uint32_t n) {
   if (i < n) p[i] = 0u;
 }
 
+// Advances the batch's current chunk; the last node of the captured bounce-loop graph.
+__global__ void k_advance(uint32_t* cur) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *cur += 1u;
+}
+
 // ----------------------------------------------------------------- path state
 
-// thr.w holds flags: bits 0..15 bounces, bit 16 last bounce delta, bit 17 no film weight.
-constexpr uint32_t F_DELTA = 1u << 16;
-constexpr uint32_t F_NOFILM = 1u << 17;
+// thr.w holds flags: bits 0..7 bounces, bit 8 last bounce delta, bit 9 no film weight,
+// bits 24..31 the scene of a batched path.
+constexpr uint32_t F_BOUNCES = 0xFFu;
+constexpr uint32_t F_DELTA = 1u << 8;
+constexpr uint32_t F_NOFILM = 1u << 9;
 
+// Chunk segment holding batched slot i (binary search over the segment starts).
+__device__ __forceinline__ const ChunkSeg& find_seg_slot(const ChunkDesc& cd, uint32_t i) {
+  uint32_t b = 0, e = cd.n_segs;
+  while (b + 1u < e) {
+    uint32_t m = (b + e) / 2u;
+    if (cd.segs[m].slot_begin <= i) b = m; else e = m;
+  }
+  return cd.segs[b];
+}
+
+template <bool BATCH>
 __global__ void k_raygen(WaveParams P) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < NUM_COUNTERS) P.counters[i] = (i == CNT_QUEUE) ? P.n_chunk : 0u;
-  if (i >= P.n_chunk) return;
-  uint64_t g = P.g0 + i;
-  uint32_t sample = P.sample_begin + (uint32_t)(g / P.npix);
-  uint32_t pixel = (uint32_t)(g % P.npix);
-  uint32_t x = pixel % P.xr, y = pixel / P.xr;
+  uint32_t n_chunk = BATCH ? P.chunks[*P.chunk_cur].n_chunk : P.n_chunk;
+  if (i < NUM_COUNTERS) P.counters[i] = (i == CNT_QUEUE) ? n_chunk : 0u;
+  if (i >= n_chunk) return;
+  uint32_t sample, pixel, scene = 0, xr, yr, k0, k1;
+  const DevScene* S;
+  if (BATCH) {
+    const ChunkDesc& cd = P.chunks[*P.chunk_cur];
+    const ChunkSeg& sg = find_seg_slot(cd, i);
+    scene = sg.scene;
+    const BatchEntry& E = P.batch[scene];
+    uint64_t local = sg.local_begin + (i - sg.slot_begin);
+    sample = E.sample_begin + (uint32_t)(local / E.npix);
+    pixel = (uint32_t)(local % E.npix);
+    xr = E.xr; yr = E.yr; k0 = E.k0; k1 = E.k1;
+    S = &E.S;
+    P.pid[i] = make_uint2(pixel, sample);
+  } else {
+    uint64_t g = P.g0 + i;
+    sample = P.sample_begin + (uint32_t)(g / P.npix);
+    pixel = (uint32_t)(g % P.npix);
+    xr = P.xr; yr = P.yr; k0 = P.k0; k1 = P.k1;
+    S = &P.S;
+  }
+  uint32_t x = pixel % xr, y = pixel / xr;
   float jx = 0.5f, jy = 0.5f;
   if (sample != 0) {  // iteration 0 samples pixel centres (path_renderer.cpp:17)
     float r[4];
-    dgpu_rng_block(P.k0, P.k1, pixel, sample, 0, 2, r);
+    dgpu_rng_block(k0, k1, pixel, sample, 0, 2, r);
     jx = r[2]; jy = r[3];
   }
-  uint32_t flags = (jx == 0.f || jy == 0.f) ? F_NOFILM : 0u;
+  uint32_t flags = ((jx == 0.f || jy == 0.f) ? F_NOFILM : 0u) | (scene << SCENE_SHIFT);
   V3 o, d;
-  camera_ray(P.S, float(x) + jx, float(y) + jy, P.xr, P.yr, o, d);
+  camera_ray(*S, float(x) + jx, float(y) + jy, xr, yr, o, d);
   P.qo[0][i] = make_float4(o.x, o.y, o.z, 0.f);
   P.qd[0][i] = make_float4(d.x, d.y, d.z, INFINITY);
-  P.qpath[0][i] = i;
+  P.qpath[0][i] = i | (scene << SCENE_SHIFT);
   P.thr[i] = make_float4(1.f, 1.f, 1.f, __uint_as_float(flags));
   P.rad[i] = make_float4(0.f, 0.f, 0.f, 0.f);
   for (uint32_t k = 0; k < P.slots; ++k) P.vis[(size_t)i * P.slots + k] = 0;
@@ -238,11 +313,10 @@ __device__ __forceinline__ V3 light_tri_normal(const DevScene& S, uint32_t tri) 
 }
 
 // estimate_direct (path_renderer.cpp:135-206) for one light; writes up to two shadow rays.
-__device__ __forceinline__ void estimate_direct(const WaveParams& P, const Surface& g, V3 wo,
-                                                const Bsdf& bsdf, uint32_t li, float scale, V3 thr,
+__device__ __forceinline__ void estimate_direct(const DevScene& S, const WaveParams& P, const Surface& g,
+                                                V3 wo, const Bsdf& bsdf, uint32_t li, float scale, V3 thr,
                                                 const float rb0[4], const float rb1[4],
                                                 NeeOut& nb, NeeOut& nl) {
-  const DevScene& S = P.S;
   const DevLight L = S.lights[li];
   bool use_bsdf = P.strategy != DGPU_DIRECT_LIGHT;
   bool use_light = P.strategy != DGPU_DIRECT_BSDF;
@@ -298,24 +372,23 @@ __device__ __forceinline__ void estimate_direct(const WaveParams& P, const Surfa
   }
 }
 
-__device__ __forceinline__ void emit_shadow(const WaveParams& P, const NeeOut& s, uint32_t slot,
+__device__ __forceinline__ void emit_shadow(const WaveParams& P, const NeeOut& s, uint32_t slot_word,
                                             uint32_t qi) {
   P.so[qi] = make_float4(s.o.x, s.o.y, s.o.z, s.tmin);
   P.sd[qi] = make_float4(s.d.x, s.d.y, s.d.z, s.tmax);
-  P.smeta[qi] = make_uint2(slot, s.ignore);
+  P.smeta[qi] = make_uint2(slot_word, s.ignore);
 }
 
 // One bounce of PathRenderer::sample for every path in the current queue.
 // Every lane of a wave runs the same number of queue appends (inactive lanes
 // append nothing), so the ballot-based compaction sees the whole wave.
-__global__ __launch_bounds__(256) void k_shade(WaveParams P, uint32_t bounce) {
+template <bool BATCH>
+__device__ __forceinline__ void shade_body(const WaveParams& P, const DevScene& S, uint32_t bounce,
+                                           bool active, uint32_t i, uint32_t p, uint32_t scene,
+                                           uint32_t pixel, uint32_t sample, uint32_t k0, uint32_t k1) {
   const uint32_t cur = bounce & 1u, nxt = cur ^ 1u;
-  const DevScene& S = P.S;
   const uint32_t SL = P.slots;
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  uint32_t n = P.counters[CNT_QUEUE + bounce];
-  bool active = i < n;
-  uint32_t p = 0, flags = 0, bounces = 0;
+  uint32_t flags = 0, bounces = 0;
   V3 thr = v3(0.f, 0.f, 0.f), rad = v3(0.f, 0.f, 0.f), wo = v3(0.f, 0.f, 1.f);
   Surface g;
   Bsdf bsdf;
@@ -323,7 +396,6 @@ __global__ __launch_bounds__(256) void k_shade(WaveParams P, uint32_t bounce) {
   bool shading = false;
   float rb0[4] = {0.f, 0.f, 0.f, 0.f}, rb1[4] = {0.f, 0.f, 0.f, 0.f}, rb2[4] = {0.f, 0.f, 0.f, 0.f};
   if (active) {
-    p = P.qpath[cur][i];
     float4 t4 = P.thr[p], r4 = P.rad[p];
     thr = xyz(t4);
     rad = xyz(r4);
@@ -333,7 +405,7 @@ __global__ __launch_bounds__(256) void k_shade(WaveParams P, uint32_t bounce) {
       size_t s = (size_t)p * SL + k;
       if (P.vis[s]) { rad = rad + xyz(P.pend[s]); P.vis[s] = 0; }
     }
-    bounces = flags & 0xFFFFu;
+    bounces = flags & F_BOUNCES;
     bool last_delta = (flags & F_DELTA) != 0u;
     float4 o4 = P.qo[cur][i], d4 = P.qd[cur][i], h4 = P.hit[i];
     V3 ro = xyz(o4), rd = xyz(d4);
@@ -354,25 +426,23 @@ __global__ __launch_bounds__(256) void k_shade(WaveParams P, uint32_t bounce) {
       shading = true;
       wo = normalize(-rd);
       bsdf = make_bsdf(S, g);
-      uint64_t gidx = P.g0 + p;
-      uint32_t sample = P.sample_begin + (uint32_t)(gidx / P.npix);
-      uint32_t pixel = (uint32_t)(gidx % P.npix);
-      dgpu_rng_block(P.k0, P.k1, pixel, sample, bounce, 0, rb0);
-      dgpu_rng_block(P.k0, P.k1, pixel, sample, bounce, 1, rb1);
-      dgpu_rng_block(P.k0, P.k1, pixel, sample, bounce, 2, rb2);
+      dgpu_rng_block(k0, k1, pixel, sample, bounce, 0, rb0);
+      dgpu_rng_block(k0, k1, pixel, sample, bounce, 1, rb1);
+      dgpu_rng_block(k0, k1, pixel, sample, bounce, 2, rb2);
     }
   }
   // direct lighting (path_renderer.cpp:68-76, 113-132): up to 2 shadow rays per light
   bool do_nee = shading && bounces >= P.min_depth && bsdf.matches(BSDF_ALL & ~BSDF_DELTA) &&
                 S.num_lights > 0u;
-  uint32_t lights_loop = P.sample_all ? S.num_lights : 1u;
+  uint32_t lights_loop = (!BATCH && P.sample_all) ? P.S.num_lights : 1u;
+  const uint32_t scene_bits = scene << SCENE_SHIFT;
   for (uint32_t li = 0; li < lights_loop; ++li) {
     NeeOut nb, nl;
     nb.emit = false;
     nl.emit = false;
     if (do_nee) {
-      if (P.sample_all) {
-        estimate_direct(P, g, wo, bsdf, li, 1.f, thr, rb0, rb1, nb, nl);
+      if (!BATCH && P.sample_all) {
+        estimate_direct(S, P, g, wo, bsdf, li, 1.f, thr, rb0, rb1, nb, nl);
       } else {
         // DiscreteDistrib1d::sample / pdf (discrete_distrib_1d.cpp:22-42)
         float u = rb0[0];
@@ -382,7 +452,7 @@ __global__ __launch_bounds__(256) void k_shade(WaveParams P, uint32_t bounce) {
           if (S.light_cdf[m] < u) b = m; else e = m;
         }
         float pp = S.light_cdf[b + 1] - S.light_cdf[b];
-        if (pp != 0.f) estimate_direct(P, g, wo, bsdf, b, 1.f / pp, thr, rb0, rb1, nb, nl);
+        if (pp != 0.f) estimate_direct(S, P, g, wo, bsdf, b, 1.f / pp, thr, rb0, rb1, nb, nl);
       }
     }
     uint32_t slot_b = (uint32_t)((size_t)p * SL + 2 * li), slot_l = slot_b + 1;
@@ -390,13 +460,13 @@ __global__ __launch_bounds__(256) void k_shade(WaveParams P, uint32_t bounce) {
     if (nb.emit) {
       P.pend[slot_b] = make_float4(nb.contrib.x, nb.contrib.y, nb.contrib.z, 0.f);
       P.vis[slot_b] = 1;
-      emit_shadow(P, nb, slot_b, qb);
+      emit_shadow(P, nb, slot_b | scene_bits, qb);
     }
     uint32_t ql = wave_append(&P.counters[CNT_SHADOW + bounce], nl.emit);
     if (nl.emit) {
       P.pend[slot_l] = make_float4(nl.contrib.x, nl.contrib.y, nl.contrib.z, 0.f);
       P.vis[slot_l] = 1;
-      emit_shadow(P, nl, slot_l, ql);
+      emit_shadow(P, nl, slot_l | scene_bits, ql);
     }
   }
   // continuation: Bsdf::sample_light_f, throughput, Russian roulette (path_renderer.cpp:78-107)
@@ -422,7 +492,7 @@ __global__ __launch_bounds__(256) void k_shade(WaveParams P, uint32_t bounce) {
       if (go) {
         cont = true;
         next_o = g.p; next_d = wi; next_eps = g.eps;
-        flags = (flags & ~(0xFFFFu | F_DELTA)) | bounces | (delta ? F_DELTA : 0u);
+        flags = (flags & ~(F_BOUNCES | F_DELTA)) | bounces | (delta ? F_DELTA : 0u);
       }
     }
   }
@@ -434,54 +504,112 @@ __global__ __launch_bounds__(256) void k_shade(WaveParams P, uint32_t bounce) {
   if (cont) {
     P.qo[nxt][qn] = make_float4(next_o.x, next_o.y, next_o.z, next_eps);
     P.qd[nxt][qn] = make_float4(next_d.x, next_d.y, next_d.z, INFINITY);
-    P.qpath[nxt][qn] = p;
+    P.qpath[nxt][qn] = p | scene_bits;
   }
 }
 
-// Resolves the last pending NEE of every path and adds the chunk's samples to the
-// film pixel by pixel in sample order (Film::add_sample with the r = 0.5 box filter;
-// non-finite or negative samples are dropped, renderer.cpp:36-41).
-__global__ __launch_bounds__(256) void k_finalize(WaveParams P) {
-  uint32_t px = blockIdx.x * blockDim.x + threadIdx.x;
+template <bool BATCH>
+__global__ __launch_bounds__(256) void k_shade(WaveParams P, uint32_t bounce) {
+  const uint32_t cur = bounce & 1u;
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t n = P.counters[CNT_QUEUE + bounce];
+  bool active = i < n;
+  if (BATCH) {
+    uint32_t p = 0, scene = 0;
+    uint2 ps = make_uint2(0u, 0u);
+    if (active) {
+      p = P.qpath[cur][i] & LOW24;
+      scene = __float_as_uint(P.thr[p].w) >> SCENE_SHIFT;
+      ps = P.pid[p];
+    }
+    const BatchEntry& E = P.batch[scene];
+    shade_body<true>(P, E.S, bounce, active, i, p, scene, ps.x, ps.y, E.k0, E.k1);
+  } else {
+    uint32_t p = active ? P.qpath[cur][i] : 0u;
+    uint64_t gidx = P.g0 + p;
+    uint32_t sample = P.sample_begin + (uint32_t)(gidx / P.npix);
+    uint32_t pixel = (uint32_t)(gidx % P.npix);
+    shade_body<false>(P, P.S, bounce, active, i, p, 0u, pixel, sample, P.k0, P.k1);
+  }
+}
+
+// Film::add_sample with the r = 0.5 box filter: each sample lands on one pixel with
+// weight 1; non-finite or negative samples are dropped (renderer.cpp:36-41).  Samples
+// are added to the film value one at a time in sample order, so the film does not
+// depend on how paths were split into chunks or batched with other scenes.
+__device__ __forceinline__ void film_accumulate(const WaveParams& P, float* film_rgb, float* film_w,
+                                                uint32_t px, uint32_t slot, uint32_t npix,
+                                                uint64_t local, uint64_t local_end) {
+  float fr = film_rgb[3 * (size_t)px], fg = film_rgb[3 * (size_t)px + 1], fb = film_rgb[3 * (size_t)px + 2];
+  float fw = film_w[px];
+  uint32_t dropped = 0;
+  bool touched = false;
+  for (; local < local_end; local += npix, slot += npix) {
+    float4 r4 = P.rad[slot];
+    V3 rad = xyz(r4);
+    for (uint32_t k = 0; k < P.slots; ++k) {
+      size_t s = (size_t)slot * P.slots + k;
+      if (P.vis[s]) rad = rad + xyz(P.pend[s]);
+    }
+    bool ok = isfinite(rad.x) && isfinite(rad.y) && isfinite(rad.z) && rad.x >= 0.f &&
+              rad.y >= 0.f && rad.z >= 0.f;
+    if (!ok) { ++dropped; continue; }
+    uint32_t flags = __float_as_uint(P.thr[slot].w);
+    if (flags & F_NOFILM) continue;
+    fr += rad.x; fg += rad.y; fb += rad.z;
+    fw += 1.f;
+    touched = true;
+  }
+  if (touched) {
+    film_rgb[3 * (size_t)px] = fr;
+    film_rgb[3 * (size_t)px + 1] = fg;
+    film_rgb[3 * (size_t)px + 2] = fb;
+    film_w[px] = fw;
+  }
+  if (dropped) atomicAdd((unsigned long long*)&P.stats[3], (unsigned long long)dropped);
+}
+
+__device__ __forceinline__ void add_chunk_stats(const WaveParams& P, uint32_t n_chunk) {
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     uint64_t ext = 0, shd = 0;
     for (uint32_t b = 0; b < MAX_BOUNCES; ++b) {
       ext += P.counters[CNT_QUEUE + b];
       shd += P.counters[CNT_SHADOW + b];
     }
-    P.stats[0] += P.n_chunk;
+    P.stats[0] += n_chunk;
     P.stats[1] += ext;
     P.stats[2] += shd;
   }
-  if (px >= P.npix) return;
-  uint64_t g_end = P.g0 + P.n_chunk;
-  uint64_t g = P.g0 + ((px + P.npix - (uint32_t)(P.g0 % P.npix)) % P.npix);
-  V3 sum = v3(0.f, 0.f, 0.f);
-  float wsum = 0.f;
-  uint32_t dropped = 0;
-  for (; g < g_end; g += P.npix) {
-    uint32_t i = (uint32_t)(g - P.g0);
-    float4 r4 = P.rad[i];
-    V3 rad = xyz(r4);
-    for (uint32_t k = 0; k < P.slots; ++k) {
-      size_t s = (size_t)i * P.slots + k;
-      if (P.vis[s]) rad = rad + xyz(P.pend[s]);
+}
+
+// Resolves the last pending NEE of every path and adds the chunk's samples to the film.
+template <bool BATCH>
+__global__ __launch_bounds__(256) void k_finalize(WaveParams P) {
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (BATCH) {
+    const ChunkDesc& cd = P.chunks[*P.chunk_cur];
+    add_chunk_stats(P, cd.n_chunk);
+    if (t >= cd.n_pairs) return;
+    uint32_t b = 0, e = cd.n_segs;
+    while (b + 1u < e) {
+      uint32_t m = (b + e) / 2u;
+      if (cd.segs[m].pair_begin <= t) b = m; else e = m;
     }
-    bool ok = isfinite(rad.x) && isfinite(rad.y) && isfinite(rad.z) && rad.x >= 0.f &&
-              rad.y >= 0.f && rad.z >= 0.f;
-    if (!ok) { ++dropped; continue; }
-    uint32_t flags = __float_as_uint(P.thr[i].w);
-    if (flags & F_NOFILM) continue;
-    sum = sum + rad;
-    wsum += 1.f;
+    const ChunkSeg& sg = cd.segs[b];
+    const BatchEntry& E = P.batch[sg.scene];
+    uint32_t q = t - sg.pair_begin;
+    uint64_t local = sg.local_begin + q;
+    uint32_t px = (uint32_t)(local % E.npix);
+    film_accumulate(P, E.film_rgb, E.film_w, px, sg.slot_begin + q, E.npix, local, sg.local_end);
+  } else {
+    add_chunk_stats(P, P.n_chunk);
+    uint32_t px = t;
+    if (px >= P.npix) return;
+    uint64_t g_end = P.g0 + P.n_chunk;
+    uint64_t g = P.g0 + ((px + P.npix - (uint32_t)(P.g0 % P.npix)) % P.npix);
+    if (g >= g_end) return;
+    film_accumulate(P, P.film_rgb, P.film_w, px, (uint32_t)(g - P.g0), P.npix, g, g_end);
   }
-  if (wsum > 0.f) {
-    P.film_rgb[3 * (size_t)px] += sum.x;
-    P.film_rgb[3 * (size_t)px + 1] += sum.y;
-    P.film_rgb[3 * (size_t)px + 2] += sum.z;
-    P.film_w[px] += wsum;
-  }
-  if (dropped) atomicAdd((unsigned long long*)&P.stats[3], (unsigned long long)dropped);
 }
 
 // ----------------------------------------------------------------- adjoint
@@ -546,7 +674,7 @@ __device__ V3 replay_path(const WaveParams& P, uint32_t pixel, uint32_t sample, 
   const uint32_t NON_DELTA = BSDF_ALL & ~BSDF_DELTA;
   for (uint32_t bounce = 0; bounce < MAX_BOUNCES; ++bounce) {
     TraceHit h;
-    bool is = trace_one<false, WIDTH>(S, ray, st, h);
+    bool is = trace_one<false, WIDTH>(BvhRef{S.nodes, S.tris}, ray, st, h);
     Surface g;
     if (is) g = surface_at(S, h.tri, h.t, h.b1, h.b2);
     if ((bounces == 0u || last_delta) && bounces >= P.min_depth && bounces <= P.max_depth) {
@@ -574,7 +702,7 @@ __device__ V3 replay_path(const WaveParams& P, uint32_t pixel, uint32_t sample, 
         nb.emit = false;
         nl.emit = false;
         if (P.sample_all) {
-          estimate_direct(P, g, wo, bsdf, li, 1.f, thr, rb0, rb1, nb, nl);
+          estimate_direct(S, P, g, wo, bsdf, li, 1.f, thr, rb0, rb1, nb, nl);
         } else {
           float u = rb0[0];
           uint32_t b = 0, e = S.num_lights;
@@ -583,7 +711,7 @@ __device__ V3 replay_path(const WaveParams& P, uint32_t pixel, uint32_t sample, 
             if (S.light_cdf[m] < u) b = m; else e = m;
           }
           float pp = S.light_cdf[b + 1] - S.light_cdf[b];
-          if (pp != 0.f) estimate_direct(P, g, wo, bsdf, b, 1.f / pp, thr, rb0, rb1, nb, nl);
+          if (pp != 0.f) estimate_direct(S, P, g, wo, bsdf, b, 1.f / pp, thr, rb0, rb1, nb, nl);
         }
         NeeOut* outs[2] = {&nb, &nl};
         for (int k = 0; k < 2; ++k) {
@@ -592,7 +720,7 @@ __device__ V3 replay_path(const WaveParams& P, uint32_t pixel, uint32_t sample, 
           TraceRay sr;
           sr.o = n.o; sr.d = n.d; sr.tmin = n.tmin; sr.tmax = n.tmax; sr.ignore = n.ignore;
           TraceHit sh;
-          if (trace_one<true, WIDTH>(S, sr, st, sh)) continue;
+          if (trace_one<true, WIDTH>(BvhRef{S.nodes, S.tris}, sr, st, sh)) continue;
           rad = rad + n.contrib;
           if (ADJ) adj_deposit(sink, bsdf, texel, mat, n.d, wo, n.contrib);
         }
@@ -668,34 +796,11 @@ __global__ __launch_bounds__(ADJ_BLOCK) void k_adjoint(WaveParams P, AdjArgs A) 
 
 // ----------------------------------------------------------------- launchers
 
-template <bool ANY, int WIDTH, int LDS, int BLOCK>
-static hipError_t launch_api_t(const DevScene& S, const ApiIO& io, uint32_t n, uint32_t* fetch,
-                               uint32_t* spill, uint32_t grid, hipStream_t st) {
-  hipLaunchKernelGGL((k_trace_api<ANY, WIDTH, LDS, BLOCK>), dim3(grid), dim3(BLOCK), 0, st, S, io, n,
-                     fetch, spill);
-  return hipGetLastError();
-}
-
-template <int WIDTH, int LDS, int BLOCK>
-static hipError_t launch_ext_t(const DevScene& S, const ExtendIO& io, const uint32_t* count,
-                               uint32_t* fetch, uint32_t* spill, uint32_t grid, hipStream_t st) {
-  hipLaunchKernelGGL((k_extend<WIDTH, LDS, BLOCK>), dim3(grid), dim3(BLOCK), 0, st, S, io, count, fetch,
-                     spill);
-  return hipGetLastError();
-}
-
-template <int WIDTH, int LDS, int BLOCK>
-static hipError_t launch_con_t(const DevScene& S, const ConnectIO& io, const uint32_t* count,
-                               uint32_t* fetch, uint32_t* spill, uint32_t grid, hipStream_t st) {
-  hipLaunchKernelGGL((k_connect<WIDTH, LDS, BLOCK>), dim3(grid), dim3(BLOCK), 0, st, S, io, count, fetch,
-                     spill);
-  return hipGetLastError();
-}
-
-// Dispatch over the compiled (width, LDS stack, block) variants.
-#define DG_VARIANTS(X)          \
-  X(2, 8, 128) X(2, 16, 128) X(2, 32, 128) X(2, 16, 64) X(2, 16, 256) \
-  X(4, 8, 128) X(4, 16, 128) X(4, 32, 128) X(4, 16, 64) X(4, 16, 256)
+// Compiled (BVH width, LDS stack entries, workgroup size) variants of the trace kernels.
+#define DG_VARIANTS(X)                                                              \
+  X(2, 8, 64) X(2, 16, 64) X(2, 32, 64) X(2, 8, 128) X(2, 16, 128) X(2, 32, 128)      \
+  X(2, 8, 256) X(2, 16, 256) X(2, 32, 256) X(4, 8, 64) X(4, 16, 64) X(4, 32, 64)     \
+  X(4, 8, 128) X(4, 16, 128) X(4, 32, 128) X(4, 8, 256) X(4, 16, 256) X(4, 32, 256)
 
 bool variant_supported(uint32_t width, uint32_t lds, uint32_t block) {
 #define X(W, L, B) if (width == W && lds == L && block == B) return true;
@@ -709,8 +814,10 @@ const void* trace_kernel_ptr(int which, uint32_t width, uint32_t lds, uint32_t b
   if (width == W && lds == L && block == B) {                                              \
     if (which == 0) return (const void*)&k_trace_api<false, W, L, B>;                      \
     if (which == 1) return (const void*)&k_trace_api<true, W, L, B>;                       \
-    if (which == 2) return (const void*)&k_extend<W, L, B>;                                \
-    return (const void*)&k_connect<W, L, B>;                                               \
+    if (which == 2) return (const void*)&k_extend<W, L, B, false>;                         \
+    if (which == 3) return (const void*)&k_connect<W, L, B, false>;                        \
+    if (which == 4) return (const void*)&k_extend<W, L, B, true>;                          \
+    return (const void*)&k_connect<W, L, B, true>;                                         \
   }
   DG_VARIANTS(X)
 #undef X
@@ -724,57 +831,89 @@ hipError_t launch_trace_api(const DevScene& S, const KernelCfg& cfg, const dgpu_
   io.rays = rays;
   if (hits) io.hits = *hits; else io.hits = dgpu_hits{nullptr, nullptr, nullptr, nullptr};
   io.occluded = occluded;
-  io.tris = S.tris;
+  io.bvh = BvhRef{S.nodes, S.tris};
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, fetch, 1u);
   bool any = occluded != nullptr;
-#define X(W, L, B)                                                                          \
-  if (cfg.width == W && cfg.lds == L && cfg.block == B)                                    \
-    return any ? launch_api_t<true, W, L, B>(S, io, n, fetch, spill, cfg.grid, st)           \
-               : launch_api_t<false, W, L, B>(S, io, n, fetch, spill, cfg.grid, st);
+#define X(W, L, B)                                                                               \
+  if (cfg.width == W && cfg.lds == L && cfg.block == B) {                                        \
+    if (any) hipLaunchKernelGGL((k_trace_api<true, W, L, B>), dim3(cfg.grid), dim3(B), 0, st, io, n, \
+                                fetch, spill, S.stack_cap);                                      \
+    else hipLaunchKernelGGL((k_trace_api<false, W, L, B>), dim3(cfg.grid), dim3(B), 0, st, io, n, \
+                            fetch, spill, S.stack_cap);                                          \
+    return hipGetLastError();                                                                    \
+  }
   DG_VARIANTS(X)
 #undef X
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_extend(const WaveParams& P, const KernelCfg& cfg, uint32_t bounce, hipStream_t st) {
-  ExtendIO io{P.qo[bounce & 1u], P.qd[bounce & 1u], P.hit};
+hipError_t launch_extend(const WaveParams& P, const KernelCfg& cfg, uint32_t bounce, bool batch,
+                         hipStream_t st) {
   const uint32_t* count = P.counters + CNT_QUEUE + bounce;
   uint32_t* fetch = P.counters + CNT_FETCH_EXT + bounce;
-#define X(W, L, B)                                           \
-  if (cfg.width == W && cfg.lds == L && cfg.block == B)     \
-    return launch_ext_t<W, L, B>(P.S, io, count, fetch, P.spill, cfg.grid, st);
+  ExtendIO<false> io{P.qo[bounce & 1u], P.qd[bounce & 1u], P.qpath[bounce & 1u], P.hit,
+                     BvhRef{P.S.nodes, P.S.tris}, nullptr};
+  ExtendIO<true> iob{P.qo[bounce & 1u], P.qd[bounce & 1u], P.qpath[bounce & 1u], P.hit,
+                     BvhRef{nullptr, nullptr}, nullptr};
+#define X(W, L, B)                                                                               \
+  if (cfg.width == W && cfg.lds == L && cfg.block == B) {                                        \
+    if (batch) hipLaunchKernelGGL((k_extend<W, L, B, true>), dim3(cfg.grid), dim3(B), 0, st, iob, \
+                                  count, fetch, P.spill, P.stack_cap, P.batch, P.num_scenes);     \
+    else hipLaunchKernelGGL((k_extend<W, L, B, false>), dim3(cfg.grid), dim3(B), 0, st, io, count, \
+                            fetch, P.spill, P.stack_cap, P.batch, P.num_scenes);                  \
+    return hipGetLastError();                                                                    \
+  }
   DG_VARIANTS(X)
 #undef X
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_connect(const WaveParams& P, const KernelCfg& cfg, uint32_t bounce, hipStream_t st) {
-  ConnectIO io{P.so, P.sd, P.smeta, P.vis};
+hipError_t launch_connect(const WaveParams& P, const KernelCfg& cfg, uint32_t bounce, bool batch,
+                          hipStream_t st) {
   const uint32_t* count = P.counters + CNT_SHADOW + bounce;
   uint32_t* fetch = P.counters + CNT_FETCH_SHD + bounce;
-#define X(W, L, B)                                           \
-  if (cfg.width == W && cfg.lds == L && cfg.block == B)     \
-    return launch_con_t<W, L, B>(P.S, io, count, fetch, P.spill, cfg.grid, st);
+  ConnectIO<false> io{P.so, P.sd, P.smeta, P.vis, BvhRef{P.S.nodes, P.S.tris}, nullptr};
+  ConnectIO<true> iob{P.so, P.sd, P.smeta, P.vis, BvhRef{nullptr, nullptr}, nullptr};
+#define X(W, L, B)                                                                                \
+  if (cfg.width == W && cfg.lds == L && cfg.block == B) {                                         \
+    if (batch) hipLaunchKernelGGL((k_connect<W, L, B, true>), dim3(cfg.grid), dim3(B), 0, st, iob, \
+                                  count, fetch, P.spill, P.stack_cap, P.batch, P.num_scenes);      \
+    else hipLaunchKernelGGL((k_connect<W, L, B, false>), dim3(cfg.grid), dim3(B), 0, st, io, count, \
+                            fetch, P.spill, P.stack_cap, P.batch, P.num_scenes);                   \
+    return hipGetLastError();                                                                     \
+  }
   DG_VARIANTS(X)
 #undef X
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_raygen(const WaveParams& P, hipStream_t st) {
-  uint32_t blocks = (std::max(P.n_chunk, (uint32_t)NUM_COUNTERS) + 255u) / 256u;
-  hipLaunchKernelGGL(k_raygen, dim3(blocks), dim3(256), 0, st, P);
+hipError_t launch_raygen(const WaveParams& P, uint32_t capacity, bool batch, hipStream_t st) {
+  uint32_t blocks = (std::max(capacity, (uint32_t)NUM_COUNTERS) + 255u) / 256u;
+  if (batch) hipLaunchKernelGGL(k_raygen<true>, dim3(blocks), dim3(256), 0, st, P);
+  else hipLaunchKernelGGL(k_raygen<false>, dim3(blocks), dim3(256), 0, st, P);
   return hipGetLastError();
 }
 
-hipError_t launch_shade(const WaveParams& P, uint32_t bounce, hipStream_t st) {
-  uint32_t blocks = (P.n_chunk + 255u) / 256u;
-  hipLaunchKernelGGL(k_shade, dim3(blocks), dim3(256), 0, st, P, bounce);
+hipError_t launch_shade(const WaveParams& P, uint32_t capacity, uint32_t bounce, bool batch, hipStream_t st) {
+  uint32_t blocks = (capacity + 255u) / 256u;
+  if (batch) hipLaunchKernelGGL(k_shade<true>, dim3(blocks), dim3(256), 0, st, P, bounce);
+  else hipLaunchKernelGGL(k_shade<false>, dim3(blocks), dim3(256), 0, st, P, bounce);
   return hipGetLastError();
 }
 
-hipError_t launch_finalize(const WaveParams& P, hipStream_t st) {
-  uint32_t blocks = (P.npix + 255u) / 256u;
-  hipLaunchKernelGGL(k_finalize, dim3(blocks), dim3(256), 0, st, P);
+hipError_t launch_finalize(const WaveParams& P, uint32_t capacity, bool batch, hipStream_t st) {
+  if (batch) {
+    uint32_t blocks = (capacity + 255u) / 256u;
+    hipLaunchKernelGGL(k_finalize<true>, dim3(blocks), dim3(256), 0, st, P);
+  } else {
+    uint32_t blocks = (P.npix + 255u) / 256u;
+    hipLaunchKernelGGL(k_finalize<false>, dim3(blocks), dim3(256), 0, st, P);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_advance(uint32_t* cur, hipStream_t st) {
+  hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, st, cur);
   return hipGetLastError();
 }
 

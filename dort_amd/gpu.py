@@ -19,7 +19,7 @@ _lib = None
 EXPORTS = ["dgpu_abi_version", "dgpu_last_error", "dgpu_create", "dgpu_destroy", "dgpu_device_alloc",
            "dgpu_device_free", "dgpu_copy", "dgpu_scene_create", "dgpu_scene_destroy",
            "dgpu_scene_get_info", "dgpu_scene_update_texture", "dgpu_scene_update_material",
-           "dgpu_trace_closest", "dgpu_trace_any", "dgpu_render", "dgpu_adjoint"]
+           "dgpu_trace_closest", "dgpu_trace_any", "dgpu_render", "dgpu_render_batch", "dgpu_adjoint"]
 
 
 class DgpuError(RuntimeError):
@@ -51,6 +51,8 @@ def lib():
                                      C.c_uint32, C.POINTER(C.c_double)]
         L.dgpu_render.argtypes = [C.c_void_p, C.POINTER(abi.RenderOpts), C.c_void_p, C.c_void_p,
                                   C.POINTER(abi.RenderStats), C.c_void_p]
+        L.dgpu_render_batch.argtypes = [C.c_void_p, C.POINTER(abi.BatchItem), C.c_uint32, C.c_uint32,
+                                        C.POINTER(abi.RenderStats)]
         L.dgpu_scene_update_texture.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p]
         L.dgpu_scene_update_material.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(abi.Material)]
         L.dgpu_adjoint.argtypes = [C.c_void_p, C.POINTER(abi.RenderOpts), C.c_void_p, C.c_void_p,
@@ -58,7 +60,7 @@ def lib():
         for n in ("dgpu_create", "dgpu_device_alloc", "dgpu_device_free", "dgpu_copy",
                   "dgpu_scene_create", "dgpu_scene_get_info", "dgpu_scene_update_texture",
                   "dgpu_scene_update_material", "dgpu_trace_closest", "dgpu_trace_any", "dgpu_render",
-                  "dgpu_adjoint"):
+                  "dgpu_render_batch", "dgpu_adjoint"):
             getattr(L, n).restype = C.c_int
         _lib = L
     return _lib
@@ -84,6 +86,25 @@ class Context:
 
     def scene(self, scene, **bvh):
         return Scene(self, scene, **bvh)
+
+    def render_batch(self, items, films=None, device=False, graph=True):
+        """Renders [(Scene, RenderOpts), ...] in one batched wavefront.  Returns
+        ([(film_rgb, film_w), ...], stats); films (optional) are accumulated into."""
+        n = len(items)
+        arr = (abi.BatchItem * n)()
+        if films is None:
+            films = [(np.zeros((o.y_res, o.x_res, 3), np.float32), np.zeros((o.y_res, o.x_res), np.float32))
+                     for _, o in items]
+        ptr = lambda a: a.data_ptr() if hasattr(a, "data_ptr") else a.ctypes.data
+        for i, ((sc, o), (rgb, w)) in enumerate(zip(items, films)):
+            arr[i].scene = sc.h.value
+            arr[i].opts = o
+            arr[i].film_rgb_sum = ptr(rgb)
+            arr[i].film_weight = ptr(w)
+        flags = (abi.BUF_DEVICE if device else 0) | (0 if graph else abi.BATCH_NO_GRAPH)
+        st = abi.RenderStats()
+        check(lib().dgpu_render_batch(self.h, arr, n, flags, C.byref(st)))
+        return films, {f: getattr(st, f) for f, _ in abi.RenderStats._fields_}
 
     def device_array(self, host):
         """Copies a numpy array into a new DeviceArray on this context's GPU."""

@@ -136,8 +136,9 @@ typedef struct {
 } dgpu_hits;
 
 enum {
-  DGPU_BUF_DEVICE = 1u,    /* buffers are device pointers */
-  DGPU_RENDER_PROFILE = 2u /* dgpu_render: time every kernel launch with HIP events */
+  DGPU_BUF_DEVICE = 1u,     /* buffers are device pointers */
+  DGPU_RENDER_PROFILE = 2u, /* dgpu_render: time every kernel launch with HIP events */
+  DGPU_BATCH_NO_GRAPH = 4u  /* dgpu_render_batch: launch kernels directly, no hipGraph */
 };
 
 /* ---- rendering ----------------------------------------------------------- */
@@ -230,6 +231,25 @@ dgpu_status dgpu_scene_update_material(dgpu_scene* scene, uint32_t material, con
 dgpu_status dgpu_render(dgpu_scene* scene, const dgpu_render_opts* opts,
                         float* film_rgb_sum, float* film_weight,
                         dgpu_stats* out_stats, const volatile int* cancel);
+
+/* ---- batched rendering (no dort counterpart) -------------------------------
+ * Renders many scenes in one wavefront: every scene's paths go into shared queues,
+ * the per-chunk bounce loop (raygen, extend/shade/connect per bounce, film) is
+ * captured once in a hipGraph and replayed per chunk; the chunk index advances on
+ * the device.  Scenes may differ in triangles, resolution, sample range and seed;
+ * they must share the depth / strategy options, BVH width and kernel knobs, and a
+ * context.  Each item's film is accumulated like dgpu_render's, so batch results
+ * equal per-scene renders.  flags: DGPU_BUF_DEVICE (films are device pointers),
+ * DGPU_BATCH_NO_GRAPH. */
+typedef struct {
+  dgpu_scene* scene;
+  dgpu_render_opts opts;
+  float* film_rgb_sum;
+  float* film_weight;
+} dgpu_batch_item;
+
+dgpu_status dgpu_render_batch(dgpu_ctx* ctx, const dgpu_batch_item* items, uint32_t n,
+                              uint32_t flags, dgpu_stats* out_stats);
 
 /* ---- adjoint path replay (no dort counterpart) ----------------------------
  * Gradient of L = sum_px dot(dL_dimage[px], I[px]), where I is the film mean of
