@@ -49,6 +49,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-intersect", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--calib", action="store_true",
                     help="also run a 256 MiB device-to-device copy (counter calibration for rocprofv3 --pmc)")
     ap.add_argument("--width", type=int, default=2)
@@ -57,6 +58,22 @@ def parse():
     ap.add_argument("--leaf", type=int, default=4)
     ap.add_argument("--tile", type=int, default=0)
     return ap.parse_args()
+
+
+def committed_traffic(args, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc summary of this
+    workload (FETCH_SIZE x2.0 + WRITE_SIZE x1.0, copy-kernel calibration), when the run
+    uses the profiled configuration; otherwise None."""
+    path = os.path.join(ROOT, "profiles", "terrain_2M_1024x1024_64spp_pmc.md")
+    default = (args.quads, args.res, args.spp, args.width, args.stack, args.block, args.leaf) == \
+        (1000, 1024, 64, 2, 16, 128, 4)
+    if not default or not os.path.exists(path):
+        return None, None
+    for line in open(path):
+        if line.startswith(f"| `k_{kernel}"):
+            cols = [c.strip() for c in line.strip().strip("|").split("|")]
+            return int((float(cols[3]) + float(cols[4])) * 1e6), os.path.relpath(path, ROOT)
+    return None, None
 
 
 def terrain_workload(args):
@@ -138,6 +155,51 @@ def adjoint_bench(ctx, args):
             "grad_norm": float(np.sqrt((g_tex.astype(np.float64) ** 2).sum() + (g_rough ** 2).sum()))}
 
 
+def batch_bench(ctx, args):
+    """64 independent scenes, triangle counts log-uniform in [1k, 500k] (displaced
+    geodesic spheres and boxes on a ground plane), resolutions uniform in {256^2, 512^2,
+    768^2}, 16 spp, max depth 6, sky-lit.  Batched wavefront (one hipGraph replayed per
+    chunk) against one dgpu_render launch sequence per scene; films in HBM for both."""
+    from dort_amd import abi, gpu, scenes
+    t0 = time.time()
+    batch = scenes.batch_scenes(64, seed=3)
+    items, films = [], []
+    for k, (sc, res) in enumerate(batch):
+        gsc = ctx.scene(sc, width=args.width, max_leaf_size=args.leaf, stack_lds=args.stack,
+                        block_size=args.block)
+        o = abi.render_opts(res, res, spp=16, seed=500 + k, max_depth=6, rr_depth=3, scene_id=k)
+        items.append((gsc, o))
+        f = gpu.DeviceArray(ctx, res * res * 3, np.float32)
+        f.copy_from(np.zeros(res * res * 3, np.float32))
+        w = gpu.DeviceArray(ctx, res * res, np.float32)
+        w.copy_from(np.zeros(res * res, np.float32))
+        films.append((f, w))
+    tris = sum(sc.num_triangles for sc, _ in batch)
+    paths = sum(o.x_res * o.y_res * 16 for _, o in items)
+    log(f"batch workload: 64 scenes, {tris} triangles, {paths / 1e6:.1f} Mpaths ({time.time() - t0:.1f}s to build)")
+
+    def per_scene():
+        for (gsc, o), (f, w) in zip(items, films):
+            gsc.render(o, f, w, device=True)
+
+    def batched():
+        ctx.render_batch(items, films=films, device=True)
+
+    per_scene()
+    batched()
+    tp, tb = [], []
+    for _ in range(2):
+        t0 = time.perf_counter(); per_scene(); tp.append(time.perf_counter() - t0)
+        t0 = time.perf_counter(); batched(); tb.append(time.perf_counter() - t0)
+    t_p, t_b = min(tp), min(tb)
+    for f, w in films:
+        f.free()
+        w.free()
+    return {"workload": f"64 scenes, {tris} tris (log-uniform 1k-500k), 256^2/512^2/768^2, 16 spp, depth 6",
+            "paths": paths, "per_scene_Mpaths_s": paths / t_p / 1e6, "batched_Mpaths_s": paths / t_b / 1e6,
+            "speedup": t_p / t_b, "per_scene_ms": t_p * 1e3, "batched_ms": t_b * 1e3}
+
+
 def cpu_baseline(sc, args):
     from dort_amd import abi
     from oracle import oracle
@@ -205,11 +267,13 @@ def main():
                   (224 * ext_rays + 57 * shd_rays) / max(1, ps["shade_launches"])),
     }
     dom = max(kernels, key=lambda k: kernels[k][0])
+    traffic, traffic_src = committed_traffic(args, dom)
     kms, nl, bytes_per_launch = kernels[dom]
     avg_ms = kms / max(1, nl)
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": dom,
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "traffic_source": traffic_src,
+                "kernel": dom,
                 "avg_launch_ms": round(avg_ms, 4), "launches": nl,
                 "algorithmic_bytes_per_launch": int(bytes_per_launch),
                 "kernel_ms": {k: round(v[0], 2) for k, v in kernels.items()},
@@ -239,6 +303,12 @@ def main():
         a = secondary["adjoint"]
         log(f"adjoint: {a['grads_per_s']:.2f} grads/s, FD {a['fd_grads_per_s']:.2e} grads/s "
             f"({a['speedup_vs_fd']:.0f}x), forward {a['forward_ms']:.0f} ms, adjoint {a['adjoint_ms']:.0f} ms")
+
+    if not args.no_batch:
+        secondary["batch"] = batch_bench(ctx, args)
+        b = secondary["batch"]
+        log(f"batch: {b['batched_Mpaths_s']:.1f} Mpaths/s batched vs {b['per_scene_Mpaths_s']:.1f} per scene "
+            f"({b['speedup']:.2f}x)")
 
     cpu = None
     if not args.no_cpu:
