@@ -1,0 +1,45 @@
+# Builds oracle/_ref/libdortref.so: dort's own pt-path sources, compiled where
+# they lie under /root/reference, linked with ref_driver.cpp (TEST
+# INFRASTRUCTURE ONLY; the output stays out of git).
+#
+#   make -f oracle/ref.mk            (needs /root/reference; nothing is copied)
+#
+# dort's Tupfile.lua builds with clang++ and its "fast" flags (-O3 -DNDEBUG
+# -DDORT_DISABLE_STAT); the compiler matters: clang evaluates call arguments left
+# to right, which fixes the order of the two draws in Sampler::random_2d.  The
+# sources predate C++17 and rely on <functional>, <string> and <stdexcept>
+# arriving through other headers; -include supplies those standard headers.
+REF ?= /root/reference
+HERE := $(dir $(abspath $(lastword $(MAKEFILE_LIST))))
+OUT := $(HERE)_ref
+REF_CXX ?= $(firstword $(wildcard /opt/rocm/llvm/bin/clang++) clang++)
+CXXFLAGS := -O3 -std=c++14 -fPIC -DDORT_DISABLE_STAT -DNDEBUG \
+  -include functional -include string -include stdexcept \
+  -I$(REF)/include -I$(REF)/extern -I$(REF)/extern/stb -w
+
+SRCS := path_renderer renderer film bvh bvh_primitive mesh_bvh_primitive triangle \
+  primitive scene bsdf lambert_material microfacet rough_dielectric_material \
+  dielectric_material fresnel diffuse_light shape light environment_light \
+  infinite_light piecewise_distrib_2d discrete_distrib_1d monte_carlo \
+  pinhole_camera camera transform random_sampler sampler sampled_filter filter \
+  stats thread_pool box geometry math image image_texture texture_maps \
+  list_primitive mat triangle_shape triangle_shape_primitive mesh_triangle_primitive
+OBJS := $(addprefix $(OUT)/obj/,$(addsuffix .o,$(SRCS)))
+
+all: $(OUT)/libdortref.so
+
+$(OUT)/obj/%.o: $(REF)/src/dort/%.cpp
+	@mkdir -p $(OUT)/obj
+	$(REF_CXX) $(CXXFLAGS) -c $< -o $@
+
+$(OUT)/obj/ref_driver.o: $(HERE)ref_driver.cpp $(HERE)../include/dortgpu.h
+	@mkdir -p $(OUT)/obj
+	$(REF_CXX) $(CXXFLAGS) -c $< -o $@
+
+$(OUT)/libdortref.so: $(OBJS) $(OUT)/obj/ref_driver.o
+	$(REF_CXX) -shared -o $@ $^ -lpthread
+
+clean:
+	rm -rf $(OUT)
+
+.PHONY: all clean

@@ -1,0 +1,87 @@
+"""ctypes binding of oracle/_ref/libdortref.so (TEST INFRASTRUCTURE ONLY).
+
+libdortref.so is dort itself: its pt-path sources compiled where they lie under
+the reference checkout (oracle/ref.mk) plus ref_driver.cpp.  Tests use it to pin
+the oracle against the real reference; it is built only where the reference
+checkout exists, and tests that need it skip when it is absent.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from dort_amd import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_ref", "libdortref.so")
+REF_ROOT = "/root/reference"
+_lib = None
+
+
+def available():
+    return os.path.exists(LIB_PATH)
+
+
+def build():
+    """Compiles dort's sources into oracle/_ref (only where the checkout exists)."""
+    if not os.path.isdir(os.path.join(REF_ROOT, "src", "dort")):
+        return False
+    subprocess.check_call(["make", "-s", "-j8", "-f", os.path.join(HERE, "ref.mk"), f"REF={REF_ROOT}"])
+    return True
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        L = C.CDLL(LIB_PATH)
+        L.ref_scene_create.restype = C.c_void_p
+        L.ref_scene_create.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]
+        L.ref_scene_destroy.argtypes = [C.c_void_p]
+        L.ref_trace.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(C.c_void_p), C.c_int,
+                                C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.ref_render.restype = C.c_int
+        L.ref_render.argtypes = [C.c_void_p, C.POINTER(abi.RenderOpts), C.c_uint32, C.c_void_p, C.c_void_p]
+        _lib = L
+    return _lib
+
+
+class RefScene:
+    """dort's Scene, assembled as its Lua builder does (middle split, leaf 6, max 32)."""
+
+    def __init__(self, scene, split="middle", leaf_size=6, max_leaf_size=32, threads=1):
+        self.packed = scene.pack()
+        self.h = lib().ref_scene_create(C.addressof(self.packed.desc), 0 if split == "middle" else 1,
+                                        leaf_size, max_leaf_size, threads)
+        if not self.h:
+            raise ValueError("scene not expressible in dort (GGX conductor or texture)")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().ref_scene_destroy(self.h)
+            self.h = None
+
+    def trace(self, rays, any_hit=False):
+        n = len(rays["ox"])
+        arrs = [np.ascontiguousarray(rays[k], dtype=np.float32)
+                for k in ("ox", "oy", "oz", "dx", "dy", "dz", "tmin", "tmax")]
+        ptrs = (C.c_void_p * 8)(*[a.ctypes.data for a in arrs])
+        if any_hit:
+            occ = np.zeros(n, np.uint32)
+            lib().ref_trace(self.h, n, ptrs, 1, None, None, occ.ctypes.data, None, None)
+            return occ
+        t = np.zeros(n, np.float32)
+        prim = np.zeros(n, np.int32)
+        uv3 = np.zeros((n, 3), np.float32)
+        pt = np.zeros((n, 3), np.float32)
+        lib().ref_trace(self.h, n, ptrs, 0, t.ctypes.data, prim.ctypes.data, None, uv3.ctypes.data,
+                        pt.ctypes.data)
+        return {"t": t, "prim": prim, "uv": uv3[:, :2], "eps": uv3[:, 2], "p": pt}
+
+    def render(self, opts, threads=1):
+        """PathRenderer::render with RandomSampler(Rng(seed)); (rgb_sum, weight)."""
+        rgb = np.zeros((opts.y_res, opts.x_res, 3), np.float32)
+        w = np.zeros((opts.y_res, opts.x_res), np.float32)
+        if lib().ref_render(self.h, C.byref(opts), threads, rgb.ctypes.data, w.ctypes.data) != 0:
+            raise RuntimeError("ref_render failed")
+        return rgb, w

@@ -76,9 +76,8 @@ def test_convergence_rejects_a_wrong_image(test_box):
         if m.type == abi.MAT_LAMBERT:
             m.albedo = abi.f3(*[0.9 * a for a in m.albedo])
     img = oracle.OracleScene(sc).image(abi.render_opts(512, 512, spp=16, min_depth=1, max_depth=1), rng="mt")
-    # dort's own rule (variation 8) is too loose to see a 10% albedo error here ...
-    assert convergence.compare_with_ref(ref, img)[0] != "err"
-    # ... the tightened tile test and the mean check both catch it
+    # dort's own rule (variation 8) sees a 10% albedo error only on some random
+    # streams; the tightened tile test and the mean check both catch it
     assert convergence.test_convergence(ref, img, variation=2.0)
     assert not golden_ok(ref, img)[0]
 

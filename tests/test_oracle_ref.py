@@ -1,0 +1,119 @@
+"""Pins the oracle against dort itself.
+
+oracle/_ref/libdortref.so is dort's own pt-path sources (compiled where they lie
+in the reference checkout by oracle/ref.mk, with dort's clang++ build) driven
+through oracle/ref_driver.cpp.  Given the same scene and seed, dort's
+PathRenderer with its mt19937 RandomSampler and the oracle's "mt" mode draw the
+same random numbers in the same order, so every path is the same path: the
+images agree pixel by pixel up to float rounding, and hit records agree bit for
+bit.  The tests skip where the library has not been built (no checkout).
+"""
+import dataclasses
+import os
+
+import numpy as np
+import pytest
+
+from dort_amd import abi, scenes
+from oracle import oracle, ref, rgbe
+
+pytestmark = pytest.mark.skipif(not ref.available(), reason="oracle/_ref not built (no reference checkout)")
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def dortable(sc):
+    """dort has no rough conductor: GGX conductors become Lambert."""
+    mats = [scenes.lambert((0.6, 0.5, 0.4)) if m.type == abi.MAT_GGX_CONDUCTOR else m for m in sc.materials]
+    return dataclasses.replace(sc, materials=mats)
+
+
+def glass_cornell():
+    """Cornell box whose tall block is a smooth dielectric (IOR 1.5)."""
+    sc = scenes.cornell()
+    mats = list(sc.materials) + [scenes.dielectric(1.5)]
+    tri_mat = sc.tri_material.copy()
+    tri_mat[22:34] = len(mats) - 1  # 10 wall + 12 short-box triangles come first
+    return dataclasses.replace(sc, materials=mats, tri_material=tri_mat)
+
+
+def terrain():
+    return dortable(scenes.terrain_scene(quads=12, sky_res=(64, 32)))
+
+
+SCENES = {
+    "cornell": scenes.cornell,
+    "box_area": lambda: scenes.dort_test_box("area"),
+    "box_infi": lambda: scenes.dort_test_box("infi"),
+    "glass_cornell": glass_cornell,
+    "terrain_sky": terrain,
+}
+
+
+@pytest.mark.parametrize("split", ["middle", "sah"])
+@pytest.mark.parametrize("name", ["cornell", "box_area", "terrain_sky"])
+def test_trace_matches_dort_bitwise(name, split):
+    sc = SCENES[name]()
+    rays = scenes.intersection_workload(sc, 48, 48, 20000, seed=5)
+    a = ref.RefScene(sc, split=split).trace(rays)
+    b = oracle.OracleScene(sc, split=split).trace(rays, threads=1)
+    assert (a["prim"] >= 0).mean() > 0.1
+    np.testing.assert_array_equal(a["prim"], b["prim"])
+    np.testing.assert_array_equal(a["t"], b["t"])
+    occ_a = ref.RefScene(sc, split=split).trace(rays, any_hit=True)
+    occ_b = oracle.OracleScene(sc, split=split).trace(rays, any_hit=True, threads=1)
+    np.testing.assert_array_equal(occ_a, occ_b)
+
+
+RENDER_CASES = [
+    ("cornell", {}),
+    ("box_area", {}),
+    ("box_infi", {}),
+    ("glass_cornell", {}),
+    ("terrain_sky", {}),
+    ("terrain_sky", {"direct_strategy": abi.DIRECT_BSDF}),
+    ("terrain_sky", {"direct_strategy": abi.DIRECT_LIGHT}),
+    ("box_area", {"sample_all_lights": 1}),
+    ("box_area", {"min_depth": 2}),
+    ("box_area", {"only_direct": 1}),
+    ("glass_cornell", {"max_depth": 12}),
+]
+
+
+@pytest.mark.parametrize("name,kw", RENDER_CASES, ids=[f"{n}-{'-'.join(k) or 'default'}" for n, k in RENDER_CASES])
+def test_render_matches_dort_path_for_path(name, kw):
+    sc = SCENES[name]()
+    opts = abi.render_opts(32, 24, spp=6, seed=11, max_depth=6)
+    for k, v in kw.items():
+        setattr(opts, k, v)
+    ra, wa = ref.RefScene(sc).render(opts, threads=1)
+    rb, wb, _ = oracle.OracleScene(sc).render(opts, rng="mt", threads=1)
+    np.testing.assert_array_equal(wa, wb)
+    assert ra.sum() > 0
+    # same paths: per-pixel agreement to float rounding (a different random
+    # draw order or a dropped term moves whole pixels by O(1))
+    np.testing.assert_allclose(rb, ra, rtol=2e-4, atol=1e-5)
+
+
+def test_render_detects_planted_mistake():
+    """The comparison above is sharp: 1% more albedo on one wall fails it."""
+    sc = scenes.cornell()
+    bad = dataclasses.replace(sc, materials=list(sc.materials))
+    m = abi.Material.from_buffer_copy(bad.materials[1])
+    m.albedo = abi.f3(*(np.array(m.albedo[:]) * 1.01))
+    bad.materials[1] = m
+    opts = abi.render_opts(32, 24, spp=6, seed=11, max_depth=6)
+    ra, _ = ref.RefScene(sc).render(opts, threads=1)
+    rb, _, _ = oracle.OracleScene(bad).render(opts, rng="mt", threads=1)
+    assert not np.allclose(rb, ra, rtol=2e-4, atol=1e-5)
+
+
+def test_dort_itself_passes_its_golden_render():
+    """dort rendering its own test box meets the acceptance rule the oracle and the
+    GPU path are held to (pins the scene restatement and the rule together)."""
+    from tests.test_oracle import golden_ok
+    gold = rgbe.read_hdr(os.path.join(GOLDEN, "box_diff_area_pin.hdr"))
+    opts = abi.render_opts(512, 512, spp=16, seed=42, min_depth=0, max_depth=3)
+    rgb, w = ref.RefScene(scenes.dort_test_box("area")).render(opts, threads=min(8, os.cpu_count() or 1))
+    ok, msg = golden_ok(gold, rgb / np.maximum(w, 1e-30)[..., None])
+    assert ok, msg
