@@ -52,8 +52,8 @@ def parse():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--calib", action="store_true",
                     help="also run a 256 MiB device-to-device copy (counter calibration for rocprofv3 --pmc)")
-    ap.add_argument("--width", type=int, default=2)
-    ap.add_argument("--stack", type=int, default=16)
+    ap.add_argument("--width", type=int, default=4)
+    ap.add_argument("--stack", type=int, default=32)
     ap.add_argument("--block", type=int, default=128)
     ap.add_argument("--leaf", type=int, default=4)
     ap.add_argument("--tile", type=int, default=0)

@@ -258,8 +258,8 @@ dgpu_status dgpu_scene_create(dgpu_ctx* ctx, const dgpu_scene_desc* d, const dgp
   dgpu_bvh_opts o{};
   if (opts) o = *opts;
   KernelCfg cfg;
-  cfg.width = o.width ? o.width : 2;
-  cfg.lds = o.stack_lds ? o.stack_lds : 16;
+  cfg.width = o.width ? o.width : 4;
+  cfg.lds = o.stack_lds ? o.stack_lds : 32;
   cfg.block = o.block_size ? o.block_size : 128;
   cfg.grid = 0;
   if (!variant_supported(cfg.width, cfg.lds, cfg.block))

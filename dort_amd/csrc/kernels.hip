@@ -48,8 +48,7 @@ struct BvhRef {
 
 template <bool ANY, int WIDTH>
 __device__ __forceinline__ bool trace_one(BvhRef b, const TraceRay& r, LaneStack& st, TraceHit& h) {
-  if (WIDTH == 4) return traverse_bvh4<ANY>(b.nodes, b.tris, r, st, h);
-  return traverse_bvh2<ANY>(b.nodes, b.tris, r, st, h);
+  return traverse<ANY, WIDTH>(b.nodes, b.tris, r, st, h);
 }
 
 // Shared persistent loop: `io.load(i, ray)` returns the ray's BVH, `io.store(i, hit, found)`.
@@ -837,9 +836,9 @@ hipError_t launch_trace_api(const DevScene& S, const KernelCfg& cfg, const dgpu_
 #define X(W, L, B)                                                                               \
   if (cfg.width == W && cfg.lds == L && cfg.block == B) {                                        \
     if (any) hipLaunchKernelGGL((k_trace_api<true, W, L, B>), dim3(cfg.grid), dim3(B), 0, st, io, n, \
-                                fetch, spill, S.stack_cap);                                      \
+                                fetch, spill, S.stack_cap);                          \
     else hipLaunchKernelGGL((k_trace_api<false, W, L, B>), dim3(cfg.grid), dim3(B), 0, st, io, n, \
-                            fetch, spill, S.stack_cap);                                          \
+                            fetch, spill, S.stack_cap);                              \
     return hipGetLastError();                                                                    \
   }
   DG_VARIANTS(X)
@@ -858,9 +857,9 @@ hipError_t launch_extend(const WaveParams& P, const KernelCfg& cfg, uint32_t bou
 #define X(W, L, B)                                                                               \
   if (cfg.width == W && cfg.lds == L && cfg.block == B) {                                        \
     if (batch) hipLaunchKernelGGL((k_extend<W, L, B, true>), dim3(cfg.grid), dim3(B), 0, st, iob, \
-                                  count, fetch, P.spill, P.stack_cap, P.batch, P.num_scenes);     \
+                                  count, fetch, P.spill, P.stack_cap, P.batch, P.num_scenes); \
     else hipLaunchKernelGGL((k_extend<W, L, B, false>), dim3(cfg.grid), dim3(B), 0, st, io, count, \
-                            fetch, P.spill, P.stack_cap, P.batch, P.num_scenes);                  \
+                            fetch, P.spill, P.stack_cap, P.batch, P.num_scenes);      \
     return hipGetLastError();                                                                    \
   }
   DG_VARIANTS(X)
@@ -877,9 +876,9 @@ hipError_t launch_connect(const WaveParams& P, const KernelCfg& cfg, uint32_t bo
 #define X(W, L, B)                                                                                \
   if (cfg.width == W && cfg.lds == L && cfg.block == B) {                                         \
     if (batch) hipLaunchKernelGGL((k_connect<W, L, B, true>), dim3(cfg.grid), dim3(B), 0, st, iob, \
-                                  count, fetch, P.spill, P.stack_cap, P.batch, P.num_scenes);      \
+                                  count, fetch, P.spill, P.stack_cap, P.batch, P.num_scenes); \
     else hipLaunchKernelGGL((k_connect<W, L, B, false>), dim3(cfg.grid), dim3(B), 0, st, io, count, \
-                            fetch, P.spill, P.stack_cap, P.batch, P.num_scenes);                   \
+                            fetch, P.spill, P.stack_cap, P.batch, P.num_scenes);       \
     return hipGetLastError();                                                                     \
   }
   DG_VARIANTS(X)

@@ -156,7 +156,7 @@ class DeviceArray:
 class Scene:
     """A scene uploaded to the GPU (BVH built on the host, flattened to the GPU layout)."""
 
-    def __init__(self, ctx, scene, width=2, max_leaf_size=4, sah_bins=16, stack_lds=16,
+    def __init__(self, ctx, scene, width=4, max_leaf_size=4, sah_bins=16, stack_lds=32,
                  block_size=128, queue_tile=0):
         self.ctx = ctx
         self.packed = scene.pack()

@@ -112,10 +112,10 @@ typedef struct {
 
 /* BVH / kernel knobs (the autotuner's search space). Zero fields take defaults. */
 typedef struct {
-  uint32_t width;          /* 2 or 4 (default 2) */
+  uint32_t width;          /* 2 or 4 (default 4) */
   uint32_t max_leaf_size;  /* triangles per leaf, 1..8 (default 4) */
   uint32_t sah_bins;       /* binned-SAH bins (default 16) */
-  uint32_t stack_lds;      /* traversal stack entries kept in LDS per lane (default 16) */
+  uint32_t stack_lds;      /* traversal stack entries kept in LDS per lane (default 32) */
   uint32_t block_size;     /* threads per traversal workgroup: 64,128,256 (default 128) */
   uint32_t queue_tile;     /* paths per wavefront chunk (default: sized from film) */
 } dgpu_bvh_opts;

@@ -39,7 +39,25 @@ $(OUT)/obj/ref_driver.o: $(HERE)ref_driver.cpp $(HERE)../include/dortgpu.h
 $(OUT)/libdortref.so: $(OBJS) $(OUT)/obj/ref_driver.o
 	$(REF_CXX) -shared -o $@ $^ -lpthread
 
+# dort with dort_amd::GpuPathRenderer (dort_amd/host) in its Renderer seam, over
+# libdortgpu.so; built when that library exists (target `gpu`).
+GPU_LIB := $(HERE)../dort_amd/libdortgpu.so
+HOST := $(HERE)../dort_amd/host
+
+$(OUT)/obj/gpu_path_renderer.o: $(HOST)/gpu_path_renderer.cpp $(HOST)/gpu_path_renderer.hpp $(HERE)../include/dortgpu.h
+	@mkdir -p $(OUT)/obj
+	$(REF_CXX) $(CXXFLAGS) -I$(HERE)../include -c $< -o $@
+
+$(OUT)/obj/ref_gpu_glue.o: $(HERE)ref_gpu_glue.cpp $(HOST)/gpu_path_renderer.hpp
+	@mkdir -p $(OUT)/obj
+	$(REF_CXX) $(CXXFLAGS) -I$(HERE)../include -c $< -o $@
+
+$(OUT)/libdortref_gpu.so: $(OBJS) $(OUT)/obj/ref_driver.o $(OUT)/obj/gpu_path_renderer.o $(OUT)/obj/ref_gpu_glue.o $(GPU_LIB)
+	$(REF_CXX) -shared -o $@ $(filter %.o,$^) -L$(dir $(GPU_LIB)) -ldortgpu -Wl,-rpath,'$$ORIGIN/../../dort_amd' -lpthread
+
+gpu: $(OUT)/libdortref_gpu.so
+
 clean:
 	rm -rf $(OUT)
 
-.PHONY: all clean
+.PHONY: all clean gpu

@@ -15,19 +15,29 @@ from dort_amd import abi
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_ref", "libdortref.so")
+GPU_LIB_PATH = os.path.join(HERE, "_ref", "libdortref_gpu.so")
 REF_ROOT = "/root/reference"
 _lib = None
+_gpu_lib = None
 
 
 def available():
     return os.path.exists(LIB_PATH)
 
 
+def gpu_available():
+    return os.path.exists(GPU_LIB_PATH)
+
+
 def build():
-    """Compiles dort's sources into oracle/_ref (only where the checkout exists)."""
+    """Compiles dort's sources into oracle/_ref (only where the checkout exists), and
+    dort with dort_amd's GpuPathRenderer when libdortgpu.so has been built."""
     if not os.path.isdir(os.path.join(REF_ROOT, "src", "dort")):
         return False
-    subprocess.check_call(["make", "-s", "-j8", "-f", os.path.join(HERE, "ref.mk"), f"REF={REF_ROOT}"])
+    targets = ["all"]
+    if os.path.exists(os.path.join(HERE, "..", "dort_amd", "libdortgpu.so")):
+        targets.append("gpu")
+    subprocess.check_call(["make", "-s", "-j8", "-f", os.path.join(HERE, "ref.mk"), f"REF={REF_ROOT}", *targets])
     return True
 
 
@@ -85,3 +95,47 @@ class RefScene:
         if lib().ref_render(self.h, C.byref(opts), threads, rgb.ctypes.data, w.ctypes.data) != 0:
             raise RuntimeError("ref_render failed")
         return rgb, w
+
+
+def gpu_lib():
+    global _gpu_lib
+    if _gpu_lib is None:
+        L = C.CDLL(GPU_LIB_PATH)
+        L.ref_scene_create.restype = C.c_void_p
+        L.ref_scene_create.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]
+        L.ref_scene_destroy.argtypes = [C.c_void_p]
+        L.refgpu_render.restype = C.c_int
+        L.refgpu_render.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(abi.RenderOpts), C.c_uint32, C.c_uint32,
+                                    C.c_uint32, C.c_int, C.c_void_p, C.c_void_p, C.POINTER(C.c_uint64),
+                                    C.POINTER(C.c_uint64), C.POINTER(C.c_float), C.c_char_p, C.c_size_t]
+        _gpu_lib = L
+    return _gpu_lib
+
+
+class GpuRendererInDort:
+    """A dort::Scene plus dort_amd::GpuPathRenderer plugged into dort's Renderer seam."""
+
+    def __init__(self, scene):
+        self.packed = scene.pack()
+        self.h = gpu_lib().ref_scene_create(C.addressof(self.packed.desc), 0, 6, 32, 1)
+        if not self.h:
+            raise ValueError("scene not expressible in dort (GGX conductor or texture)")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            gpu_lib().ref_scene_destroy(self.h)
+            self.h = None
+
+    def render(self, opts, sampler_seed=42, cancel_after=0, chunk=0, filter_kind=0):
+        """Renderer::render of the GPU renderer; returns (rgb_sum, weight, info dict).
+        Raises RuntimeError with the renderer's exception message."""
+        rgb = np.zeros((opts.y_res, opts.x_res, 3), np.float32)
+        w = np.zeros((opts.y_res, opts.x_res), np.float32)
+        seed, paths, pct = C.c_uint64(), C.c_uint64(), C.c_float()
+        err = C.create_string_buffer(512)
+        rc = gpu_lib().refgpu_render(self.h, C.addressof(self.packed.desc), C.byref(opts), sampler_seed,
+                                     cancel_after, chunk, filter_kind, rgb.ctypes.data, w.ctypes.data,
+                                     C.byref(seed), C.byref(paths), C.byref(pct), err, len(err))
+        if rc != 0:
+            raise RuntimeError(err.value.decode())
+        return rgb, w, {"seed": seed.value, "paths": paths.value, "percent": pct.value}

@@ -219,6 +219,14 @@ void* ref_scene_create(const dgpu_scene_desc* d, uint32_t split_method, uint32_t
 
 void ref_scene_destroy(void* p) { delete (RefScene*)p; }
 
+}  // extern "C"
+
+// For other test drivers linked with this one (oracle/ref_gpu_glue.cpp).
+std::shared_ptr<Scene> ref_scene_scene(void* p) { return ((RefScene*)p)->scene; }
+std::shared_ptr<Camera> ref_scene_camera(void* p) { return ((RefScene*)p)->camera; }
+
+extern "C" {
+
 // rays: 8 pointers (ox, oy, oz, dx, dy, dz, tmin, tmax).  Closest hits also give
 // the surface uv and point of DiffGeom (uv3 = u, v, ray epsilon; pt = p), or NULL.
 void ref_trace(void* p, uint32_t n, const float* const* rays, int any, float* t, int32_t* prim,
