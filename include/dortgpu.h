@@ -5,9 +5,10 @@
  * (src/dort/path_renderer.cpp:11-21) and everything it calls per sample
  * (scene intersection, BSDFs, lights, film accumulation).  The seam in dort is
  * the abstract `Renderer` (include/dort/renderer.hpp:12-34), selected by
- * `lua_render_make` (src/dort/lua_render.cpp:102-221).  A host adapter
- * (`dort_amd/host/gpu_path_renderer.hpp`) flattens a dort-style scene into a
- * `dgpu_scene_desc` and calls the functions below.
+ * `lua_render_make` (src/dort/lua_render.cpp:102-221).  The host adapter
+ * `dort_amd::GpuPathRenderer` (dort_amd/host/gpu_path_renderer.hpp) is such a
+ * Renderer: it takes the scene as a typed `dgpu_scene_desc` recorded next to the
+ * dort::Scene and calls the functions below.
  *
  * Conventions
  *  - C99, no exceptions cross this boundary.  Every call returns a dgpu_status;
