@@ -117,3 +117,21 @@ def test_dort_itself_passes_its_golden_render():
     rgb, w = ref.RefScene(scenes.dort_test_box("area")).render(opts, threads=min(8, os.cpu_count() or 1))
     ok, msg = golden_ok(gold, rgb / np.maximum(w, 1e-30)[..., None])
     assert ok, msg
+
+
+def test_counter_rng_estimator_has_dorts_relmse():
+    """The counter-based random numbers the GPU draws give the same error as dort's
+    mt19937 streams: relMSE against dort's 65536-spp render, 48 seeds each (the
+    relMSE of one render is heavy-tailed; 48 seeds put the ratio's noise near 2%)."""
+    r = np.load(os.path.join(GOLDEN, "cornell_64x48_65536spp_dort.npy"))
+    sc = scenes.cornell()
+    o, d = oracle.OracleScene(sc), ref.RefScene(sc)
+    a, b = [], []
+    for s in range(48):
+        opts = abi.render_opts(64, 48, spp=16, seed=300 + s, max_depth=5, rr_depth=2)
+        a.append(np.mean((o.image(opts, rng="counter") - r) ** 2 / (r ** 2 + 1e-2)))
+        rgb, w = d.render(opts, threads=min(8, os.cpu_count() or 1))
+        b.append(np.mean((rgb / w[..., None] - r) ** 2 / (r ** 2 + 1e-2)))
+    ratio = np.mean(a) / np.mean(b)
+    print(f"relMSE counter {np.mean(a):.5f}, dort {np.mean(b):.5f}, ratio {ratio:.4f}")
+    assert 0.92 < ratio < 1.08
