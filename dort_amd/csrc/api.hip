@@ -94,6 +94,31 @@ struct dgpu_ctx {
     return b.p;
   }
 
+  // Paths per wavefront chunk when the scene sets none: as many as fit a quarter of
+  // the free device memory, at most 16 GiB of queues and path state.  Bigger chunks
+  // amortise the ramp-up and drain of every per-bounce launch (on the 2M-triangle
+  // terrain the traversal kernels ran twice as fast at 64M paths as at 2M).
+  uint64_t default_tile(uint32_t slots) {
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
+    const uint64_t per_path = 120ull + 57ull * slots;  // state, queues, hit, shadow queue
+    uint64_t budget = std::min<uint64_t>(16ull << 30, free_b / 4);
+    return std::max<uint64_t>(1ull << 20, budget / per_path);
+  }
+
+  // Workgroups of the strided shade kernel: enough to fill every CU.
+  uint32_t shade_blocks(bool batch) {
+    int& cached = batch ? shade_blocks_batch : shade_blocks_single;
+    if (cached == 0) {
+      int nb = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, shade_kernel_ptr(batch), 256, 0) != hipSuccess || nb <= 0)
+        nb = 2;
+      cached = nb * num_cus * 4;
+    }
+    return (uint32_t)cached;
+  }
+  int shade_blocks_single = 0, shade_blocks_batch = 0;
+
   uint32_t grid_for(int which, const KernelCfg& c) {
     uint64_t key = ((uint64_t)which << 48) | ((uint64_t)c.width << 32) | ((uint64_t)c.lds << 16) | c.block;
     auto it = grid_cache.find(key);
@@ -399,6 +424,21 @@ dgpu_status dgpu_scene_create(dgpu_ctx* ctx, const dgpu_scene_desc* d, const dgp
     UP(marg.data(), marg.size(), S.env_marg);
     UP(cond.data(), cond.size(), S.env_cond);
     UP(pdfs.data(), pdfs.size(), S.env_pdf);
+    // guide tables: for u = g / G, the bin the bisection would return (a start hint;
+    // the device scan corrects it in either direction)
+    auto guide = [](const float* cdf, uint32_t size, uint32_t G, uint32_t* out) {
+      uint32_t k = 0;  // entries of cdf[0 .. size-2] below u
+      for (uint32_t g = 0; g < G; ++g) {
+        float u = float(g) / float(G);
+        while (k < size - 1 && cdf[k] < u) ++k;
+        out[g] = k == 0 ? 0u : std::min(k - 1u, size - 2u);
+      }
+    };
+    std::vector<uint32_t> mg(h), cg((size_t)w * h);
+    guide(marg.data(), h + 1, h, mg.data());
+    for (uint32_t y = 0; y < h; ++y) guide(cond.data() + (size_t)(w + 1) * y, w + 1, w, cg.data() + (size_t)w * y);
+    UP(mg.data(), mg.size(), S.env_marg_guide);
+    UP(cg.data(), cg.size(), S.env_cond_guide);
     DevLight L;
     memset(&L, 0, sizeof L);
     L.kind = LK_ENV;
@@ -574,9 +614,10 @@ dgpu_status dgpu_render(dgpu_scene* s, const dgpu_render_opts* o, float* film_rg
   dgpu_status st = DGPU_OK;
   const uint32_t npix = o->x_res * o->y_res;
   const uint64_t total = (uint64_t)npix * (o->sample_end - o->sample_begin);
-  uint32_t tile = s->queue_tile ? s->queue_tile : (1u << 21);
-  uint32_t C = (uint32_t)std::min<uint64_t>(total ? total : 1, tile);
   uint32_t slots = o->sample_all_lights ? 2 * std::max(1u, s->S.num_lights) : 2;
+  uint64_t tile = s->queue_tile ? s->queue_tile : ctx->default_tile(slots);
+  uint32_t C = (uint32_t)std::min<uint64_t>(std::min<uint64_t>(total ? total : 1, tile), 0xFFFFFFFFull / slots);
+  const uint32_t shade_grid = ctx->shade_blocks(false);
 
   KernelCfg ce = s->cfg, cc = s->cfg;
   ce.grid = ctx->grid_for(2, ce);
@@ -646,7 +687,7 @@ dgpu_status dgpu_render(dgpu_scene* s, const dgpu_render_opts* o, float* film_rg
     for (uint32_t b = 0; b <= o->max_depth; ++b) {
       if (cancel && *cancel) { cancelled = true; break; }
       HIP_TRY(timed(0, [&] { return launch_extend(P, ce, b, false, stream); }));
-      HIP_TRY(timed(1, [&] { return launch_shade(P, P.n_chunk, b, false, stream); }));
+      HIP_TRY(timed(1, [&] { return launch_shade(P, P.n_chunk, b, false, stream, shade_grid); }));
       HIP_TRY(timed(2, [&] { return launch_connect(P, cc, b, false, stream); }));
     }
     if (!cancelled) HIP_TRY(timed(3, [&] { return launch_finalize(P, P.n_chunk, false, stream); }));
@@ -868,8 +909,8 @@ dgpu_status dgpu_render_batch(dgpu_ctx* ctx, const dgpu_batch_item* items, uint3
   }
 
   // chunks over the scene-major concatenation of every scene's paths
-  uint32_t C = s0->queue_tile ? s0->queue_tile : (1u << 21);
-  C = std::min(C, 1u << 23);  // slot indices (x2 NEE slots) must fit in 24 bits
+  uint32_t C = (uint32_t)std::min<uint64_t>(s0->queue_tile ? s0->queue_tile : ctx->default_tile(2), 1u << 23);
+  // slot indices (x2 NEE slots) must fit in 24 bits
   std::vector<ChunkDesc> chunks;
   {
     uint32_t si = 0;
@@ -904,6 +945,7 @@ dgpu_status dgpu_render_batch(dgpu_ctx* ctx, const dgpu_batch_item* items, uint3
   if (chunks.empty()) return DGPU_OK;
   uint32_t cap = 0;
   for (const ChunkDesc& cd : chunks) cap = std::max(cap, cd.n_chunk);
+  const uint32_t shade_grid = ctx->shade_blocks(true);
 
   KernelCfg ce = s0->cfg, cc = s0->cfg;
   ce.grid = ctx->grid_for(4, ce);
@@ -954,7 +996,7 @@ dgpu_status dgpu_render_batch(dgpu_ctx* ctx, const dgpu_batch_item* items, uint3
     hipError_t e = launch_raygen(P, cap, true, stream);
     for (uint32_t b = 0; e == hipSuccess && b <= o0.max_depth; ++b) {
       e = launch_extend(P, ce, b, true, stream);
-      if (e == hipSuccess) e = launch_shade(P, cap, b, true, stream);
+      if (e == hipSuccess) e = launch_shade(P, cap, b, true, stream, shade_grid);
       if (e == hipSuccess) e = launch_connect(P, cc, b, true, stream);
     }
     if (e == hipSuccess) e = launch_finalize(P, cap, true, stream);

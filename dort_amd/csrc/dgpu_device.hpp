@@ -116,6 +116,8 @@ struct DevScene {
   const float* env_marg;     // h+1
   const float* env_cond;     // (w+1)*h
   const float* env_pdf;      // w*h
+  const uint32_t* env_marg_guide;  // h: start row of the search for u in [g/h, (g+1)/h)
+  const uint32_t* env_cond_guide;  // w*h: start column per row, likewise
   V3 env_up, env_s, env_t;
   int32_t has_inf;
   V3 inf_rad;
@@ -458,13 +460,31 @@ DG_INLINE V3 env_background(const DevScene& S, V3 d) {
   return env_radiance(S, u, v);
 }
 
-// PiecewiseDistrib2d::sample_cdf_slice (piecewise_distrib_2d.cpp:58-80)
-DG_INLINE float sample_slice(const float* cdf, uint32_t size, float u, uint32_t& out_idx) {
+// PiecewiseDistrib2d::sample_cdf_slice (piecewise_distrib_2d.cpp:58-80): the bin
+// b = the last index below size-1 with cdf[b] < u (0 if none), plus the linear
+// remainder.  dort bisects; here a guide table (Chen & Asau 1974, "On generating
+// random variates from an empirical distribution") gives a start index near b and
+// short scans in both directions reach exactly the bisection's b, replacing ~log2(size)
+// dependent loads by one or two.
+#ifndef DGPU_ENV_GUIDE
+#define DGPU_ENV_GUIDE 1
+#endif
+DG_INLINE float sample_slice(const float* cdf, uint32_t size, const uint32_t* guide, uint32_t G, float u,
+                             uint32_t& out_idx) {
+#if DGPU_ENV_GUIDE
+  uint32_t g = (uint32_t)(u * float(G));
+  if (g >= G) g = G - 1u;
+  uint32_t b = guide[g];
+  while (b > 0u && cdf[b] >= u) --b;
+  while (b + 2u < size && cdf[b + 1u] < u) ++b;
+#else
+  (void)guide; (void)G;
   uint32_t b = 0, e = size - 1;
   while (b + 1 < e) {
     uint32_t m = b + (e - b) / 2;
     if (cdf[m] < u) b = m; else e = m;
   }
+#endif
   float c0 = cdf[b], c1 = cdf[b + 1];
   out_idx = b;
   return float(b) + (u - c0) / (c1 - c0);
@@ -473,8 +493,9 @@ DG_INLINE float sample_slice(const float* cdf, uint32_t size, float u, uint32_t&
 // EnvironmentLight::sample_dir (environment_light.cpp:134-146)
 DG_INLINE V3 env_sample(const DevScene& S, float su, float sv, float& u, float& v, float& pdf) {
   uint32_t yi, xi;
-  float y = sample_slice(S.env_marg, S.env_h + 1, sv, yi);
-  float x = sample_slice(S.env_cond + (size_t)(S.env_w + 1) * yi, S.env_w + 1, su, xi);
+  float y = sample_slice(S.env_marg, S.env_h + 1, S.env_marg_guide, S.env_h, sv, yi);
+  float x = sample_slice(S.env_cond + (size_t)(S.env_w + 1) * yi, S.env_w + 1,
+                         S.env_cond_guide + (size_t)S.env_w * yi, S.env_w, su, xi);
   u = x / float(S.env_w);
   v = y / float(S.env_h);
   float theta = v * PI_F, phi = (u - 0.5f) * TWO_PI_F;

@@ -94,6 +94,7 @@ constexpr uint32_t ADJ_CHUNK = 1u << 20;
 bool variant_supported(uint32_t width, uint32_t lds, uint32_t block);
 // which: 0 closest-hit API, 1 any-hit API, 2 extend, 3 connect, 4/5 batched extend/connect
 const void* trace_kernel_ptr(int which, uint32_t width, uint32_t lds, uint32_t block);
+const void* shade_kernel_ptr(bool batch);
 
 hipError_t launch_trace_api(const DevScene& S, const KernelCfg& cfg, const dgpu_rays& rays,
                             const dgpu_hits* hits, uint32_t* occluded, uint32_t n, uint32_t* fetch,
@@ -101,7 +102,9 @@ hipError_t launch_trace_api(const DevScene& S, const KernelCfg& cfg, const dgpu_
 hipError_t launch_raygen(const WaveParams& P, uint32_t capacity, bool batch, hipStream_t st);
 hipError_t launch_extend(const WaveParams& P, const KernelCfg& cfg, uint32_t bounce, bool batch,
                          hipStream_t st);
-hipError_t launch_shade(const WaveParams& P, uint32_t capacity, uint32_t bounce, bool batch, hipStream_t st);
+// max_blocks: the shade grid (workgroups of 256 that stride over the queue)
+hipError_t launch_shade(const WaveParams& P, uint32_t capacity, uint32_t bounce, bool batch, hipStream_t st,
+                        uint32_t max_blocks);
 hipError_t launch_connect(const WaveParams& P, const KernelCfg& cfg, uint32_t bounce, bool batch,
                           hipStream_t st);
 hipError_t launch_finalize(const WaveParams& P, uint32_t capacity, bool batch, hipStream_t st);

@@ -17,6 +17,8 @@
 //   Both give the same estimate except at exact ties.
 // * NEE contributions are staged per path (pend) and added by the path's owner
 //   thread once the visibility is known, so per-pixel sums are deterministic.
+#include <algorithm>
+
 #include "dgpu_internal.hpp"
 #include "traverse.hpp"
 
@@ -507,28 +509,33 @@ __device__ __forceinline__ void shade_body(const WaveParams& P, const DevScene& 
   }
 }
 
+// A fixed grid strides over the queue, whose length is known only on the device;
+// the loop bound is uniform per workgroup, so every wave runs the same number of
+// queue appends.
 template <bool BATCH>
 __global__ __launch_bounds__(256) void k_shade(WaveParams P, uint32_t bounce) {
   const uint32_t cur = bounce & 1u;
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  uint32_t n = P.counters[CNT_QUEUE + bounce];
-  bool active = i < n;
-  if (BATCH) {
-    uint32_t p = 0, scene = 0;
-    uint2 ps = make_uint2(0u, 0u);
-    if (active) {
-      p = P.qpath[cur][i] & LOW24;
-      scene = __float_as_uint(P.thr[p].w) >> SCENE_SHIFT;
-      ps = P.pid[p];
+  const uint32_t n = P.counters[CNT_QUEUE + bounce];
+  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+    uint32_t i = base + threadIdx.x;
+    bool active = i < n;
+    if (BATCH) {
+      uint32_t p = 0, scene = 0;
+      uint2 ps = make_uint2(0u, 0u);
+      if (active) {
+        p = P.qpath[cur][i] & LOW24;
+        scene = __float_as_uint(P.thr[p].w) >> SCENE_SHIFT;
+        ps = P.pid[p];
+      }
+      const BatchEntry& E = P.batch[scene];
+      shade_body<true>(P, E.S, bounce, active, i, p, scene, ps.x, ps.y, E.k0, E.k1);
+    } else {
+      uint32_t p = active ? P.qpath[cur][i] : 0u;
+      uint64_t gidx = P.g0 + p;
+      uint32_t sample = P.sample_begin + (uint32_t)(gidx / P.npix);
+      uint32_t pixel = (uint32_t)(gidx % P.npix);
+      shade_body<false>(P, P.S, bounce, active, i, p, 0u, pixel, sample, P.k0, P.k1);
     }
-    const BatchEntry& E = P.batch[scene];
-    shade_body<true>(P, E.S, bounce, active, i, p, scene, ps.x, ps.y, E.k0, E.k1);
-  } else {
-    uint32_t p = active ? P.qpath[cur][i] : 0u;
-    uint64_t gidx = P.g0 + p;
-    uint32_t sample = P.sample_begin + (uint32_t)(gidx / P.npix);
-    uint32_t pixel = (uint32_t)(gidx % P.npix);
-    shade_body<false>(P, P.S, bounce, active, i, p, 0u, pixel, sample, P.k0, P.k1);
   }
 }
 
@@ -893,8 +900,13 @@ hipError_t launch_raygen(const WaveParams& P, uint32_t capacity, bool batch, hip
   return hipGetLastError();
 }
 
-hipError_t launch_shade(const WaveParams& P, uint32_t capacity, uint32_t bounce, bool batch, hipStream_t st) {
-  uint32_t blocks = (capacity + 255u) / 256u;
+const void* shade_kernel_ptr(bool batch) {
+  return batch ? (const void*)&k_shade<true> : (const void*)&k_shade<false>;
+}
+
+hipError_t launch_shade(const WaveParams& P, uint32_t capacity, uint32_t bounce, bool batch, hipStream_t st,
+                        uint32_t max_blocks) {
+  uint32_t blocks = std::min((capacity + 255u) / 256u, std::max(1u, max_blocks));
   if (batch) hipLaunchKernelGGL(k_shade<true>, dim3(blocks), dim3(256), 0, st, P, bounce);
   else hipLaunchKernelGGL(k_shade<false>, dim3(blocks), dim3(256), 0, st, P, bounce);
   return hipGetLastError();

@@ -4,6 +4,12 @@ libdortref.so is dort itself: its pt-path sources compiled where they lie under
 the reference checkout (oracle/ref.mk) plus ref_driver.cpp.  Tests use it to pin
 the oracle against the real reference; it is built only where the reference
 checkout exists, and tests that need it skip when it is absent.
+
+Run dort with one thread.  Its ThreadPool (thread_pool.cpp) lets a worker that
+found a loop exhausted take the pool lock and read the Loop object after the
+thread that owns it (on its stack) has seen the loop finish and returned
+(ThreadPool::loop :54-75 against work_on_loop :111-146): a use after free that
+crashes now and then with many threads.  With one thread the pool has no workers.
 """
 import ctypes as C
 import os
@@ -89,7 +95,8 @@ class RefScene:
         return {"t": t, "prim": prim, "uv": uv3[:, :2], "eps": uv3[:, 2], "p": pt}
 
     def render(self, opts, threads=1):
-        """PathRenderer::render with RandomSampler(Rng(seed)); (rgb_sum, weight)."""
+        """PathRenderer::render with RandomSampler(Rng(seed)); (rgb_sum, weight).
+        threads > 1 runs into dort's thread-pool race (see the module docstring)."""
         rgb = np.zeros((opts.y_res, opts.x_res, 3), np.float32)
         w = np.zeros((opts.y_res, opts.x_res), np.float32)
         if lib().ref_render(self.h, C.byref(opts), threads, rgb.ctypes.data, w.ctypes.data) != 0:

@@ -6,6 +6,7 @@ test compares GPU and CPU renders against.
 dort is oracle/_ref/libdortref.so (its sources compiled in place by oracle/ref.mk).
 Run from the repository root:  python tests/golden/make_cornell_reference.py
 """
+import multiprocessing
 import os
 import sys
 import time
@@ -21,21 +22,27 @@ from oracle import ref  # noqa: E402
 OUT = os.path.join(ROOT, "tests", "golden", "cornell_64x48_65536spp_dort.npy")
 
 
+def batch(k):
+    # dort runs single-threaded (its thread pool has a race); batches run in parallel
+    # processes instead
+    opts = abi.render_opts(64, 48, spp=4096, seed=1000 + k, max_depth=5, rr_depth=2)
+    rgb, w = ref.RefScene(scenes.cornell()).render(opts, threads=1)
+    return rgb.astype(np.float64), w.astype(np.float64)
+
+
 def main():
     if not ref.available():
         ref.build()
-    sc = ref.RefScene(scenes.cornell())
     total = np.zeros((48, 64, 3), np.float64)
     weight = np.zeros((48, 64), np.float64)
     t0 = time.time()
     # 16 renders of 4096 iterations with different sampler seeds (dort's tiles
     # draw their streams from the seed), summed
-    for k in range(16):
-        opts = abi.render_opts(64, 48, spp=4096, seed=1000 + k, max_depth=5, rr_depth=2)
-        rgb, w = sc.render(opts, threads=os.cpu_count() or 1)
-        total += rgb
-        weight += w
-        print(f"batch {k}: {time.time() - t0:.0f}s", flush=True)
+    with multiprocessing.Pool(min(16, os.cpu_count() or 1)) as pool:
+        for k, (rgb, w) in enumerate(pool.imap(batch, range(16))):
+            total += rgb
+            weight += w
+            print(f"batch {k}: {time.time() - t0:.0f}s", flush=True)
     img = (total / weight[..., None]).astype(np.float32)
     np.save(OUT, img)
     print(f"wrote {OUT}: mean {img.reshape(-1, 3).mean(0)}, weight per pixel {weight[0, 0]:.0f}")

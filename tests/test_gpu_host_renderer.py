@@ -49,7 +49,7 @@ def test_renderer_seam_agrees_with_dorts_path_renderer(name):
     sc = scenes.cornell() if name == "cornell" else scenes.dort_test_box("area")
     opts = _opts(256)
     rgb, w, _ = ref.GpuRendererInDort(sc).render(opts, sampler_seed=7)
-    crgb, cw = ref.RefScene(sc).render(opts, threads=8)
+    crgb, cw = ref.RefScene(sc).render(opts)
     img, cimg = rgb / w[..., None], crgb / cw[..., None]
     # the same estimator with different random numbers: image means agree to the
     # noise of 786k paths, 8x8-pixel tile means to a few percent

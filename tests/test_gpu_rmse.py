@@ -5,8 +5,9 @@ The reference is dort's own render of the Cornell scene at 65536 spp
 tests/golden/make_cornell_reference.py).  relMSE = mean over pixels and channels of
 (x - r)^2 / (r^2 + 1e-2).  Both sides render 64x48 at 64 spp, max depth 5, MIS,
 RR from bounce 2, over 64 seeds each; the GPU's mean relMSE must be within 5% of
-the CPU's.  The CPU side is dort itself (oracle/_ref) where it was built, else the
-oracle's mt19937 mode, which follows dort path for path.
+the CPU's.  The CPU side is the oracle's mt19937 mode on all host cores: it
+follows dort path for path (tests/test_oracle_ref.py), and dort itself runs
+single-threaded only (its thread pool has a race).
 """
 import os
 
@@ -14,7 +15,7 @@ import numpy as np
 import pytest
 
 from dort_amd import abi, scenes
-from oracle import oracle, ref
+from oracle import oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -26,24 +27,22 @@ def rel_mse(x, r):
     return float(np.mean((x - r) ** 2 / (r ** 2 + 1e-2)))
 
 
-def cpu_image(sc, opts):
-    if ref.available():
-        rgb, w = ref.RefScene(sc).render(opts, threads=min(16, os.cpu_count() or 1))
-        return rgb / w[..., None]
-    return oracle.OracleScene(sc).image(opts, rng="mt", threads=min(16, os.cpu_count() or 1))
+def cpu_image(osc, opts):
+    return osc.image(opts, rng="mt", threads=min(16, os.cpu_count() or 1))
 
 
 def test_relmse_matches_cpu_at_equal_spp(gpu_ctx):
     r = np.load(REF)
     sc = scenes.cornell()
     gsc = gpu_ctx.scene(sc)
+    osc = oracle.OracleScene(sc)
     g, c, gsum = [], [], np.zeros_like(r)
     for s in range(SEEDS):
         opts = abi.render_opts(64, 48, spp=64, seed=500 + s, max_depth=5, rr_depth=2)
         img = gsc.image(opts)
         gsum += img
         g.append(rel_mse(img, r))
-        c.append(rel_mse(cpu_image(sc, opts), r))
+        c.append(rel_mse(cpu_image(osc, opts), r))
     g, c = np.array(g), np.array(c)
     ratio = g.mean() / c.mean()
     print(f"relMSE at 64 spp: GPU {g.mean():.5f} +- {g.std() / np.sqrt(SEEDS):.5f}, "
