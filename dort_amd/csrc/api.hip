@@ -101,7 +101,7 @@ struct dgpu_ctx {
   uint64_t default_tile(uint32_t slots) {
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
-    const uint64_t per_path = 120ull + 57ull * slots;  // state, queues, hit, shadow queue
+    const uint64_t per_path = 120ull + 56ull * slots;  // state, queues, hit, shadow queue
     uint64_t budget = std::min<uint64_t>(16ull << 30, free_b / 4);
     return std::max<uint64_t>(1ull << 20, budget / per_path);
   }
@@ -321,8 +321,6 @@ dgpu_status dgpu_scene_create(dgpu_ctx* ctx, const dgpu_scene_desc* d, const dgp
     if (us_ != DGPU_OK) { delete s; return us_; }                             \
   } while (0)
   UP((const float4*)bvh.nodes.data(), bvh.nodes.size() / 4, S.nodes);
-  UP((const float4*)bvh.tris.data(), bvh.tris.size() / 4, S.tris);
-  UP(d->tri_material, (size_t)d->num_triangles, S.tri_mat);
 
   // scene bounds (Scene::bounds -> centroid / radius, lua_builder.cpp:124-128)
   float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
@@ -333,12 +331,16 @@ dgpu_status dgpu_scene_create(dgpu_ctx* ctx, const dgpu_scene_desc* d, const dgp
   float radius = 0.5f * std::sqrt((hi[0] - lo[0]) * (hi[0] - lo[0]) + (hi[1] - lo[1]) * (hi[1] - lo[1]) +
                                   (hi[2] - lo[2]) * (hi[2] - lo[2]));
 
-  // per-triangle uvs
+  // per-triangle uvs, in leaf order
   if (d->uvs) {
     std::vector<float2> tuv(3 * (size_t)d->num_triangles);
-    for (size_t t = 0; t < 3 * (size_t)d->num_triangles; ++t) {
-      uint32_t v = d->indices[t];
-      tuv[t] = make_float2(d->uvs[2 * (size_t)v], d->uvs[2 * (size_t)v + 1]);
+    for (size_t k = 0; k < (size_t)d->num_triangles; ++k) {
+      uint32_t t;
+      memcpy(&t, &bvh.tris[12 * k + 3], 4);
+      for (int j = 0; j < 3; ++j) {
+        uint32_t v = d->indices[3 * (size_t)t + j];
+        tuv[3 * k + j] = make_float2(d->uvs[2 * (size_t)v], d->uvs[2 * (size_t)v + 1]);
+      }
     }
     UP(tuv.data(), tuv.size(), S.tri_uv);
   }
@@ -394,7 +396,7 @@ dgpu_status dgpu_scene_create(dgpu_ctx* ctx, const dgpu_scene_desc* d, const dgp
       float ct = clampf(std::cos(theta), 0.f, 1.f);
       for (uint32_t x = 0; x < w; ++x) {
         const float* px = e.rgb + 3 * ((size_t)y * w + x);
-        rgb[(size_t)y * w + x] = make_float4(px[0] * e.scale[0], px[1] * e.scale[1], px[2] * e.scale[2], 0.f);
+        rgb[(size_t)y * w + x] = make_float4(px[0] * e.scale[0], px[1] * e.scale[1], px[2] * e.scale[2], 0.f);  // .w below
         values[(size_t)y * w + x] = std::fabs((px[0] + px[1] + px[2]) / 3.f * ct);
         for (int k = 0; k < 3; ++k) vsum[k] += px[k] * ct;
         wsum += ct;
@@ -419,11 +421,11 @@ dgpu_status dgpu_scene_create(dgpu_ctx* ctx, const dgpu_scene_desc* d, const dgp
     float inv_sum = 1.f / sum;
     for (uint32_t y = 0; y <= h; ++y) marg[y] *= inv_sum;
     for (size_t i = 0; i < (size_t)w * h; ++i) pdfs[i] = values[i] * inv_sum;
+    for (size_t i = 0; i < (size_t)w * h; ++i) rgb[i].w = pdfs[i];  // radiance and probability in one fetch
     if (!(sum > 0.f)) { delete s; return fail(DGPU_EINVAL, "environment map has no energy in the upper hemisphere"); }
     UP(rgb.data(), rgb.size(), S.env_rgb);
     UP(marg.data(), marg.size(), S.env_marg);
     UP(cond.data(), cond.size(), S.env_cond);
-    UP(pdfs.data(), pdfs.size(), S.env_pdf);
     // guide tables: for u = g / G, the bin the bisection would return (a start hint;
     // the device scan corrects it in either direction)
     auto guide = [](const float* cdf, uint32_t size, uint32_t G, uint32_t* out) {
@@ -474,7 +476,17 @@ dgpu_status dgpu_scene_create(dgpu_ctx* ctx, const dgpu_scene_desc* d, const dgp
     UP(lights.data(), lights.size(), S.lights);
     UP(cdf.data(), cdf.size(), S.light_cdf);
   }
-  UP(tri_light.data(), tri_light.size(), S.tri_light);
+  // leaf-order triangle records: v0.w scene id (from the builder), v1.w material,
+  // v2.w light index or -1, so a hit reaches its attributes in one fetch
+  for (size_t k = 0; k < (size_t)d->num_triangles; ++k) {
+    uint32_t t;
+    memcpy(&t, &bvh.tris[12 * k + 3], 4);
+    uint32_t m = d->tri_material[t];
+    uint32_t l = (uint32_t)tri_light[t];
+    memcpy(&bvh.tris[12 * k + 7], &m, 4);
+    memcpy(&bvh.tris[12 * k + 11], &l, 4);
+  }
+  UP((const float4*)bvh.tris.data(), bvh.tris.size() / 4, S.tris);
   // textures
   if (d->num_textures) {
     std::vector<float4> texels;
@@ -635,7 +647,6 @@ dgpu_status dgpu_render(dgpu_scene* s, const dgpu_render_opts* o, float* film_rg
   P.thr = (float4*)ctx->buf("thr", (size_t)C * 16, &st);
   P.rad = (float4*)ctx->buf("rad", (size_t)C * 16, &st);
   P.pend = (float4*)ctx->buf("pend", (size_t)C * slots * 16, &st);
-  P.vis = (uint8_t*)ctx->buf("vis", (size_t)C * slots, &st);
   P.qo[0] = (float4*)ctx->buf("qo0", (size_t)C * 16, &st);
   P.qo[1] = (float4*)ctx->buf("qo1", (size_t)C * 16, &st);
   P.qd[0] = (float4*)ctx->buf("qd0", (size_t)C * 16, &st);
@@ -960,7 +971,6 @@ dgpu_status dgpu_render_batch(dgpu_ctx* ctx, const dgpu_batch_item* items, uint3
   P.thr = (float4*)ctx->buf("thr", (size_t)cap * 16, &st);
   P.rad = (float4*)ctx->buf("rad", (size_t)cap * 16, &st);
   P.pend = (float4*)ctx->buf("pend", (size_t)cap * 2 * 16, &st);
-  P.vis = (uint8_t*)ctx->buf("vis", (size_t)cap * 2, &st);
   P.qo[0] = (float4*)ctx->buf("qo0", (size_t)cap * 16, &st);
   P.qo[1] = (float4*)ctx->buf("qo1", (size_t)cap * 16, &st);
   P.qd[0] = (float4*)ctx->buf("qd0", (size_t)cap * 16, &st);

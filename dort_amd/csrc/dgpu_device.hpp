@@ -99,9 +99,8 @@ struct DevScene {
   uint32_t width;            // 2 or 4
   uint32_t num_tris;
   uint32_t stack_cap;        // traversal stack entries per lane (LDS + spill)
-  // attributes by scene triangle id
-  const uint32_t* tri_mat;
-  const int32_t* tri_light;  // light index or -1
+  // per-triangle attributes in leaf order (the leaf-order triangle record holds the
+  // scene id in v0.w, the material in v1.w and the light index or -1 in v2.w)
   const float2* tri_uv;      // 3 per triangle, or null (default uvs)
   const DevMaterial* mats;
   uint32_t num_mats;
@@ -112,10 +111,9 @@ struct DevScene {
   int32_t env_light;         // index of the environment light or -1
   // environment map (environment_light.cpp, piecewise_distrib_2d.cpp)
   uint32_t env_w, env_h;
-  const float4* env_rgb;     // w*h, premultiplied by scale
+  const float4* env_rgb;     // w*h: rgb premultiplied by scale, w = the texel's probability
   const float* env_marg;     // h+1
   const float* env_cond;     // (w+1)*h
-  const float* env_pdf;      // w*h
   const uint32_t* env_marg_guide;  // h: start row of the search for u in [g/h, (g+1)/h)
   const uint32_t* env_cond_guide;  // w*h: start column per row, likewise
   V3 env_up, env_s, env_t;
@@ -355,6 +353,8 @@ struct Surface {
   V3 p, nn, dpdu;
   float u, v, eps;
   uint32_t tri;    // scene triangle id
+  uint32_t mat;    // material index
+  int32_t light;   // light index or -1
 };
 
 DG_INLINE Surface surface_at(const DevScene& S, uint32_t leaf_idx, float t, float b1, float b2) {
@@ -363,7 +363,9 @@ DG_INLINE Surface surface_at(const DevScene& S, uint32_t leaf_idx, float t, floa
   uint32_t tri = __float_as_uint(a.w);
   V3 e1 = p1 - p0, e2 = p2 - p0;
   float2 uv0 = make_float2(0.f, 0.f), uv1 = make_float2(1.f, 0.f), uv2 = make_float2(0.f, 1.f);
-  if (S.tri_uv) { uv0 = S.tri_uv[3 * tri]; uv1 = S.tri_uv[3 * tri + 1]; uv2 = S.tri_uv[3 * tri + 2]; }
+  if (S.tri_uv) {
+    uv0 = S.tri_uv[3 * leaf_idx]; uv1 = S.tri_uv[3 * leaf_idx + 1]; uv2 = S.tri_uv[3 * leaf_idx + 2];
+  }
   float du1 = uv1.x - uv0.x, du2 = uv2.x - uv0.x, dv1 = uv1.y - uv0.y, dv2 = uv2.y - uv0.y;
   float det = du1 * dv2 - dv1 * du2;
   Surface s;
@@ -376,6 +378,8 @@ DG_INLINE Surface surface_at(const DevScene& S, uint32_t leaf_idx, float t, floa
   s.nn = normalize(cross(e2, e1));
   s.eps = fabsf(1e-3f * t);
   s.tri = tri;
+  s.mat = __float_as_uint(bb.w);
+  s.light = (int32_t)__float_as_uint(cc.w);
   return s;
 }
 
@@ -394,7 +398,7 @@ DG_INLINE Bsdf make_bsdf(const DevScene& S, const Surface& g, int32_t* texel_out
   float len = length(g.dpdu);
   if (len < 1e-9f) coordinate_system(b.nn, b.sn, b.tn);
   else { b.sn = g.dpdu / len; b.tn = cross(b.nn, b.sn); }
-  const DevMaterial m = S.mats[S.tri_mat[g.tri]];
+  const DevMaterial m = S.mats[g.mat];
   b.kind = DGPU_MAT_NONE;
   b.flags = 0;
   if (texel_out) *texel_out = -1;
@@ -434,30 +438,32 @@ DG_INLINE void env_dir_to_uv(const DevScene& S, V3 d, float& u, float& v, float&
   v = theta * INV_PI_F;
 }
 
-DG_INLINE V3 env_radiance(const DevScene& S, float u, float v) {
+// The texel of (u, v) (nearest, environment_light.cpp:153-159): rgb and probability;
+// zero outside the map.
+DG_INLINE float4 env_texel(const DevScene& S, float u, float v) {
   int32_t x = (int32_t)floorf(u * float(S.env_w)), y = (int32_t)floorf(v * float(S.env_h));
-  if (x < 0 || y < 0 || x >= (int32_t)S.env_w || y >= (int32_t)S.env_h) return v3(0.f, 0.f, 0.f);
-  return xyz(S.env_rgb[(uint32_t)y * S.env_w + (uint32_t)x]);
+  if (x < 0 || y < 0 || x >= (int32_t)S.env_w || y >= (int32_t)S.env_h) return make_float4(0.f, 0.f, 0.f, 0.f);
+  return S.env_rgb[(uint32_t)y * S.env_w + (uint32_t)x];
 }
 
-DG_INLINE float env_uv_pdf(const DevScene& S, float u, float v) {
-  float x = u * float(S.env_w), y = v * float(S.env_h);
-  int32_t xi = (int32_t)floorf(x), yi = (int32_t)floorf(y);
-  if (xi < 0 || yi < 0 || xi >= (int32_t)S.env_w || yi >= (int32_t)S.env_h) return 0.f;
-  return S.env_pdf[(uint32_t)yi * S.env_w + (uint32_t)xi] * (float(S.env_w) * float(S.env_h));
+// Solid-angle pdf of a direction whose texel has probability `prob` (environment_light.cpp:99-118).
+DG_INLINE float env_solid_angle_pdf(const DevScene& S, float prob, float sin_theta) {
+  return sin_theta == 0.f ? 0.f : prob * (float(S.env_w) * float(S.env_h)) / (2.f * sqf(PI_F) * sin_theta);
 }
 
-DG_INLINE float env_dir_pdf(const DevScene& S, V3 d) {
+// Radiance toward d and the pdf of sampling d, from one texel fetch.
+DG_INLINE V3 env_eval(const DevScene& S, V3 d, float& pdf) {
   float u, v, st;
   env_dir_to_uv(S, d, u, v, st);
-  if (st == 0.f) return 0.f;
-  return env_uv_pdf(S, u, v) / (2.f * sqf(PI_F) * st);
+  float4 t = env_texel(S, u, v);
+  pdf = env_solid_angle_pdf(S, t.w, st);
+  return xyz(t);
 }
 
 DG_INLINE V3 env_background(const DevScene& S, V3 d) {
   float u, v, st;
   env_dir_to_uv(S, d, u, v, st);
-  return env_radiance(S, u, v);
+  return xyz(env_texel(S, u, v));
 }
 
 // PiecewiseDistrib2d::sample_cdf_slice (piecewise_distrib_2d.cpp:58-80): the bin
@@ -490,18 +496,20 @@ DG_INLINE float sample_slice(const float* cdf, uint32_t size, const uint32_t* gu
   return float(b) + (u - c0) / (c1 - c0);
 }
 
-// EnvironmentLight::sample_dir (environment_light.cpp:134-146)
-DG_INLINE V3 env_sample(const DevScene& S, float su, float sv, float& u, float& v, float& pdf) {
+// EnvironmentLight::sample_dir (environment_light.cpp:134-146); also returns the
+// radiance of the sampled texel.
+DG_INLINE V3 env_sample(const DevScene& S, float su, float sv, float& pdf, V3& radiance) {
   uint32_t yi, xi;
   float y = sample_slice(S.env_marg, S.env_h + 1, S.env_marg_guide, S.env_h, sv, yi);
   float x = sample_slice(S.env_cond + (size_t)(S.env_w + 1) * yi, S.env_w + 1,
                          S.env_cond_guide + (size_t)S.env_w * yi, S.env_w, su, xi);
-  u = x / float(S.env_w);
-  v = y / float(S.env_h);
+  float u = x / float(S.env_w);
+  float v = y / float(S.env_h);
   float theta = v * PI_F, phi = (u - 0.5f) * TWO_PI_F;
   float ct = cosf(theta), st = sinf(theta), cp = cosf(phi), sp = sinf(phi);
-  float sin_theta = clampf(st, 0.f, 1.f);
-  pdf = sin_theta == 0.f ? 0.f : env_uv_pdf(S, u, v) / (2.f * sqf(PI_F) * sin_theta);
+  float4 t = env_texel(S, u, v);
+  pdf = env_solid_angle_pdf(S, t.w, clampf(st, 0.f, 1.f));
+  radiance = xyz(t);
   return S.env_up * ct + S.env_s * (st * cp) + S.env_t * (st * sp);
 }
 

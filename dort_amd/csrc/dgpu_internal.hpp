@@ -57,8 +57,7 @@ struct WaveParams {
   uint64_t g0;             // first global path index of the chunk
   float4* thr;             // throughput rgb + flags
   float4* rad;             // radiance accumulated so far
-  float4* pend;            // pending NEE contributions [path * slots + k]
-  uint8_t* vis;            // 1 = pending contribution visible
+  float4* pend;            // pending NEE contributions [path * slots + k]; w = 1 while visible
   float4* qo[2];           // extension queues: origin + tmin
   float4* qd[2];           //                   direction + tmax
   uint32_t* qpath[2];      //                   path slot

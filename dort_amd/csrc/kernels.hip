@@ -140,7 +140,7 @@ struct ConnectIO {
   const float4* so;
   const float4* sd;
   const uint2* meta;   // (pending slot [| scene << 24], ignored triangle)
-  uint8_t* vis;
+  float* pend_w;        // .w of the pending NEE record: 1 = visible so far
   BvhRef bvh;
   const BvhRef* table;
   __device__ __forceinline__ BvhRef load(uint32_t i, TraceRay& r) const {
@@ -153,7 +153,7 @@ struct ConnectIO {
     return bvh;
   }
   __device__ __forceinline__ void store(uint32_t i, const TraceHit&, bool found) const {
-    if (found) vis[BATCH ? (meta[i].x & LOW24) : meta[i].x] = 0;
+    if (found) pend_w[4 * (size_t)(BATCH ? (meta[i].x & LOW24) : meta[i].x)] = 0.f;
   }
 };
 
@@ -256,7 +256,7 @@ __global__ void k_raygen(WaveParams P) {
   P.qpath[0][i] = i | (scene << SCENE_SHIFT);
   P.thr[i] = make_float4(1.f, 1.f, 1.f, __uint_as_float(flags));
   P.rad[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-  for (uint32_t k = 0; k < P.slots; ++k) P.vis[(size_t)i * P.slots + k] = 0;
+  for (uint32_t k = 0; k < P.slots; ++k) P.pend[(size_t)i * P.slots + k] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
 struct NeeOut {
@@ -287,10 +287,10 @@ __device__ __forceinline__ V3 light_sample(const DevScene& S, const DevLight& L,
     sh.ignore = 0xFFFFFFFFu;
     return dot(sw, n) < 0.f ? xyz(L.radiance) : zero;
   } else if (L.kind == LK_ENV) {
-    float eu, ev;
-    wi = env_sample(S, u, v, eu, ev, pdf);
+    V3 rad;
+    wi = env_sample(S, u, v, pdf, rad);
     sh.o = p; sh.d = normalize(wi); sh.tmin = eps; sh.tmax = INFINITY; sh.ignore = 0xFFFFFFFFu;
-    return env_radiance(S, eu, ev);
+    return rad;
   } else {
     float z = 1.f - 2.f * u, r = sqrtf(fmaxf(0.f, 1.f - z * z)), phi = TWO_PI_F * v;
     wi = v3(cosf(phi) * r, sinf(phi) * r, z);
@@ -341,8 +341,7 @@ __device__ __forceinline__ void estimate_direct(const DevScene& S, const WavePar
           ignore = L.tri;
         }
       } else if (L.kind == LK_ENV) {
-        rad = env_background(S, wi);
-        lp = env_dir_pdf(S, wi);
+        rad = env_eval(S, wi, lp);
       } else {
         rad = xyz(L.radiance);
         lp = INV_FOUR_PI_F;
@@ -404,7 +403,8 @@ __device__ __forceinline__ void shade_body(const WaveParams& P, const DevScene& 
     // NEE of the previous vertex, now that its visibility is known
     for (uint32_t k = 0; k < SL; ++k) {
       size_t s = (size_t)p * SL + k;
-      if (P.vis[s]) { rad = rad + xyz(P.pend[s]); P.vis[s] = 0; }
+      float4 pe = P.pend[s];
+      if (pe.w != 0.f) { rad = rad + xyz(pe); P.pend[s].w = 0.f; }
     }
     bounces = flags & F_BOUNCES;
     bool last_delta = (flags & F_DELTA) != 0u;
@@ -415,7 +415,7 @@ __device__ __forceinline__ void shade_body(const WaveParams& P, const DevScene& 
     if (is) g = surface_at(S, leaf, h4.x, h4.y, h4.z);
     if ((bounces == 0u || last_delta) && bounces >= P.min_depth && bounces <= P.max_depth) {
       if (is) {
-        int32_t li = S.tri_light[g.tri];
+        int32_t li = g.light;
         // Intersection::eval_radiance -> DiffuseLight::eval_radiance (diffuse_light.cpp:83-93)
         if (li >= 0 && dot(g.nn, ro - g.p) >= 0.f) rad = rad + thr * xyz(S.lights[li].radiance);
       } else {
@@ -459,14 +459,12 @@ __device__ __forceinline__ void shade_body(const WaveParams& P, const DevScene& 
     uint32_t slot_b = (uint32_t)((size_t)p * SL + 2 * li), slot_l = slot_b + 1;
     uint32_t qb = wave_append(&P.counters[CNT_SHADOW + bounce], nb.emit);
     if (nb.emit) {
-      P.pend[slot_b] = make_float4(nb.contrib.x, nb.contrib.y, nb.contrib.z, 0.f);
-      P.vis[slot_b] = 1;
+      P.pend[slot_b] = make_float4(nb.contrib.x, nb.contrib.y, nb.contrib.z, 1.f);
       emit_shadow(P, nb, slot_b | scene_bits, qb);
     }
     uint32_t ql = wave_append(&P.counters[CNT_SHADOW + bounce], nl.emit);
     if (nl.emit) {
-      P.pend[slot_l] = make_float4(nl.contrib.x, nl.contrib.y, nl.contrib.z, 0.f);
-      P.vis[slot_l] = 1;
+      P.pend[slot_l] = make_float4(nl.contrib.x, nl.contrib.y, nl.contrib.z, 1.f);
       emit_shadow(P, nl, slot_l | scene_bits, ql);
     }
   }
@@ -555,7 +553,8 @@ __device__ __forceinline__ void film_accumulate(const WaveParams& P, float* film
     V3 rad = xyz(r4);
     for (uint32_t k = 0; k < P.slots; ++k) {
       size_t s = (size_t)slot * P.slots + k;
-      if (P.vis[s]) rad = rad + xyz(P.pend[s]);
+      float4 pe = P.pend[s];
+      if (pe.w != 0.f) rad = rad + xyz(pe);
     }
     bool ok = isfinite(rad.x) && isfinite(rad.y) && isfinite(rad.z) && rad.x >= 0.f &&
               rad.y >= 0.f && rad.z >= 0.f;
@@ -685,7 +684,7 @@ __device__ V3 replay_path(const WaveParams& P, uint32_t pixel, uint32_t sample, 
     if (is) g = surface_at(S, h.tri, h.t, h.b1, h.b2);
     if ((bounces == 0u || last_delta) && bounces >= P.min_depth && bounces <= P.max_depth) {
       if (is) {
-        int32_t li = S.tri_light[g.tri];
+        int32_t li = g.light;
         if (li >= 0 && dot(g.nn, ray.o - g.p) >= 0.f) rad = rad + thr * xyz(S.lights[li].radiance);
       } else {
         rad = rad + thr * background_radiance(S, ray.d);
@@ -696,7 +695,7 @@ __device__ V3 replay_path(const WaveParams& P, uint32_t pixel, uint32_t sample, 
     V3 wo = normalize(-ray.d);
     int32_t texel = -1;
     Bsdf bsdf = make_bsdf(S, g, &texel);
-    uint32_t mat = S.tri_mat[g.tri];
+    uint32_t mat = g.mat;
     float rb0[4], rb1[4], rb2[4];
     dgpu_rng_block(P.k0, P.k1, pixel, sample, bounce, 0, rb0);
     dgpu_rng_block(P.k0, P.k1, pixel, sample, bounce, 1, rb1);
@@ -878,8 +877,9 @@ hipError_t launch_connect(const WaveParams& P, const KernelCfg& cfg, uint32_t bo
                           hipStream_t st) {
   const uint32_t* count = P.counters + CNT_SHADOW + bounce;
   uint32_t* fetch = P.counters + CNT_FETCH_SHD + bounce;
-  ConnectIO<false> io{P.so, P.sd, P.smeta, P.vis, BvhRef{P.S.nodes, P.S.tris}, nullptr};
-  ConnectIO<true> iob{P.so, P.sd, P.smeta, P.vis, BvhRef{nullptr, nullptr}, nullptr};
+  float* pend_w = (float*)P.pend + 3;
+  ConnectIO<false> io{P.so, P.sd, P.smeta, pend_w, BvhRef{P.S.nodes, P.S.tris}, nullptr};
+  ConnectIO<true> iob{P.so, P.sd, P.smeta, pend_w, BvhRef{nullptr, nullptr}, nullptr};
 #define X(W, L, B)                                                                                \
   if (cfg.width == W && cfg.lds == L && cfg.block == B) {                                         \
     if (batch) hipLaunchKernelGGL((k_connect<W, L, B, true>), dim3(cfg.grid), dim3(B), 0, st, iob, \
