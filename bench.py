@@ -53,7 +53,7 @@ def parse():
     ap.add_argument("--calib", action="store_true",
                     help="also run a 256 MiB device-to-device copy (counter calibration for rocprofv3 --pmc)")
     ap.add_argument("--width", type=int, default=4)
-    ap.add_argument("--stack", type=int, default=32)
+    ap.add_argument("--stack", type=int, default=16)
     ap.add_argument("--block", type=int, default=128)
     ap.add_argument("--leaf", type=int, default=4)
     ap.add_argument("--tile", type=int, default=0)
@@ -66,7 +66,7 @@ def committed_traffic(args, kernel):
     uses the profiled configuration; otherwise None."""
     path = os.path.join(ROOT, "profiles", "terrain_2M_1024x1024_64spp_pmc.md")
     default = (args.quads, args.res, args.spp, args.width, args.stack, args.block, args.leaf) == \
-        (1000, 1024, 64, 4, 32, 128, 4)
+        (1000, 1024, 64, 4, 16, 128, 4)
     if not default or not os.path.exists(path):
         return None, None
     for line in open(path):

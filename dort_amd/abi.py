@@ -16,6 +16,7 @@ DIRECT_MIS, DIRECT_BSDF, DIRECT_LIGHT = range(3)
 BUF_DEVICE = 1
 RENDER_PROFILE = 2
 BATCH_NO_GRAPH = 4
+TRACE_BOUNDED = 8
 
 f3 = C.c_float * 3
 
@@ -57,7 +58,8 @@ class SceneDesc(C.Structure):
 
 class BvhOpts(C.Structure):
     _fields_ = [("width", C.c_uint32), ("max_leaf_size", C.c_uint32), ("sah_bins", C.c_uint32),
-                ("stack_lds", C.c_uint32), ("block_size", C.c_uint32), ("queue_tile", C.c_uint32)]
+                ("stack_lds", C.c_uint32), ("block_size", C.c_uint32), ("queue_tile", C.c_uint32),
+                ("sah_node_cost", C.c_float)]
 
 
 class Rays(C.Structure):

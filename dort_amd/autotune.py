@@ -56,7 +56,7 @@ def autotune(ctx, scene, opts, repeats=2, log=None):
             log(f"autotune {knobs}: {ms:.2f} ms ({time.perf_counter() - t0:.1f}s)")
         return ms
 
-    best = {"width": 4, "max_leaf_size": 4, "stack_lds": 32, "block_size": 128, "queue_tile": 0}
+    best = {"width": 4, "max_leaf_size": 4, "stack_lds": 16, "block_size": 128, "queue_tile": 0}
     best_ms = trial(best)
     for w in WIDTHS:
         for leaf in LEAVES:

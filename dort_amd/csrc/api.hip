@@ -107,17 +107,18 @@ struct dgpu_ctx {
   }
 
   // Workgroups of the strided shade kernel: enough to fill every CU.
-  uint32_t shade_blocks(bool batch) {
-    int& cached = batch ? shade_blocks_batch : shade_blocks_single;
+  uint32_t shade_blocks(bool batch, bool all) {
+    int& cached = batch ? shade_blocks_batch : (all ? shade_blocks_all : shade_blocks_single);
     if (cached == 0) {
       int nb = 0;
-      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, shade_kernel_ptr(batch), 256, 0) != hipSuccess || nb <= 0)
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, shade_kernel_ptr(batch, all), 256, 0) != hipSuccess ||
+          nb <= 0)
         nb = 2;
       cached = nb * num_cus * 4;
     }
     return (uint32_t)cached;
   }
-  int shade_blocks_single = 0, shade_blocks_batch = 0;
+  int shade_blocks_single = 0, shade_blocks_all = 0, shade_blocks_batch = 0;
 
   uint32_t grid_for(int which, const KernelCfg& c) {
     uint64_t key = ((uint64_t)which << 48) | ((uint64_t)c.width << 32) | ((uint64_t)c.lds << 16) | c.block;
@@ -284,7 +285,7 @@ dgpu_status dgpu_scene_create(dgpu_ctx* ctx, const dgpu_scene_desc* d, const dgp
   if (opts) o = *opts;
   KernelCfg cfg;
   cfg.width = o.width ? o.width : 4;
-  cfg.lds = o.stack_lds ? o.stack_lds : 32;
+  cfg.lds = o.stack_lds ? o.stack_lds : 16;
   cfg.block = o.block_size ? o.block_size : 128;
   cfg.grid = 0;
   if (!variant_supported(cfg.width, cfg.lds, cfg.block))
@@ -295,6 +296,21 @@ dgpu_status dgpu_scene_create(dgpu_ctx* ctx, const dgpu_scene_desc* d, const dgp
   bp.width = cfg.width;
   bp.max_leaf_size = o.max_leaf_size ? o.max_leaf_size : 4;
   bp.sah_bins = o.sah_bins ? o.sah_bins : 16;
+  bp.node_cost = o.sah_node_cost > 0.f ? o.sah_node_cost : 1.f;
+  {
+    // Box padding for the path tracer's FMA slab test: M bounds every coordinate of the
+    // scene and of the camera position (where all of its rays start); the test's error
+    // is below 3 * 2^-24 * M, the padding is M * 2^-21.
+    float m = 0.f;
+    for (size_t t = 0; t < 3 * (size_t)d->num_triangles; ++t) {
+      const float* p = d->positions + 3 * (size_t)d->indices[t];
+      m = std::max(m, std::max(std::fabs(p[0]), std::max(std::fabs(p[1]), std::fabs(p[2]))));
+    }
+    const float* c2w = d->camera.camera_to_world;
+    m = std::max(m, std::max(std::fabs(c2w[3]), std::max(std::fabs(c2w[7]), std::fabs(c2w[11]))));
+    bp.box_pad = std::isfinite(m) ? std::ldexp(m, -21) : 0.f;
+  }
+  if (!(o.sah_node_cost >= 0.f) || o.sah_node_cost > 64.f) return fail(DGPU_EINVAL, "sah_node_cost must be in [0, 64]");
   if (bp.max_leaf_size > 16) return fail(DGPU_EINVAL, "max_leaf_size must be <= 16");
   dgpu_host::BuiltBvh bvh;
   try {
@@ -570,7 +586,8 @@ dgpu_status trace_common(dgpu_scene* s, const dgpu_rays* rays, uint32_t n, dgpu_
     dh = *hits;
   }
   HIP_TRY(hipEventRecord(ctx->ev0, stream));
-  HIP_TRY(launch_trace_api(s->S, cfg, dr, occluded ? nullptr : &dh, docc, n, fetch, spill, stream));
+  HIP_TRY(launch_trace_api(s->S, cfg, dr, occluded ? nullptr : &dh, docc, n, fetch, spill,
+                           (flags & DGPU_TRACE_BOUNDED) != 0, stream));
   HIP_TRY(hipEventRecord(ctx->ev1, stream));
   if (!dev) {
     if (occluded) {
@@ -629,7 +646,7 @@ dgpu_status dgpu_render(dgpu_scene* s, const dgpu_render_opts* o, float* film_rg
   uint32_t slots = o->sample_all_lights ? 2 * std::max(1u, s->S.num_lights) : 2;
   uint64_t tile = s->queue_tile ? s->queue_tile : ctx->default_tile(slots);
   uint32_t C = (uint32_t)std::min<uint64_t>(std::min<uint64_t>(total ? total : 1, tile), 0xFFFFFFFFull / slots);
-  const uint32_t shade_grid = ctx->shade_blocks(false);
+  const uint32_t shade_grid = ctx->shade_blocks(false, o->sample_all_lights != 0);
 
   KernelCfg ce = s->cfg, cc = s->cfg;
   ce.grid = ctx->grid_for(2, ce);
@@ -956,7 +973,7 @@ dgpu_status dgpu_render_batch(dgpu_ctx* ctx, const dgpu_batch_item* items, uint3
   if (chunks.empty()) return DGPU_OK;
   uint32_t cap = 0;
   for (const ChunkDesc& cd : chunks) cap = std::max(cap, cd.n_chunk);
-  const uint32_t shade_grid = ctx->shade_blocks(true);
+  const uint32_t shade_grid = ctx->shade_blocks(true, false);
 
   KernelCfg ce = s0->cfg, cc = s0->cfg;
   ce.grid = ctx->grid_for(4, ce);

@@ -46,7 +46,6 @@ struct BNode {
   uint32_t first = 0, count = 0;  // leaf range into the prim index array
 };
 
-constexpr float C_TRAV = 1.0f;
 constexpr float C_ISECT = 1.0f;
 constexpr int SAH_DEPTH_LIMIT = 32;
 constexpr uint32_t PAR_MIN = 1u << 16;
@@ -58,6 +57,7 @@ struct Builder {
   uint32_t max_leaf;
   uint32_t bins;
   int par_levels;
+  float c_trav;         // SAH cost of a node visit (C_ISECT = 1 per triangle)
 
   Aabb range_box(uint32_t b, uint32_t e, Aabb* cbox) const {
     Aabb r, c;
@@ -120,7 +120,7 @@ struct Builder {
     float parent_area = box.area();
     float leaf_cost = C_ISECT * float(n);
     if (best_axis < 0) return (n <= max_leaf && !force) ? 0 : median();
-    float split_cost = C_TRAV + C_ISECT * (parent_area > 0.f ? best_cost / parent_area : float(n));
+    float split_cost = c_trav + C_ISECT * (parent_area > 0.f ? best_cost / parent_area : float(n));
     if (n <= max_leaf && leaf_cost <= split_cost && !force) return 0;
     int k = best_axis;
     float scale = float(bins) / ext[k];
@@ -214,7 +214,8 @@ BuiltBvh build_bvh(const float* positions, uint32_t num_vertices, const uint32_t
     root.grow(b);
     rootc.grow(&cent[3 * (size_t)t]);
   }
-  Builder bld{cent.data(), pbox.data(), idx.data(), max_leaf, std::max(4u, std::min(prm.sah_bins, 64u)), 4};
+  Builder bld{cent.data(), pbox.data(), idx.data(), max_leaf, std::max(4u, std::min(prm.sah_bins, 64u)), 4,
+              prm.node_cost > 0.f ? prm.node_cost : 1.f};
   std::vector<BNode> nodes;
   nodes.reserve(2 * (size_t)num_tris / std::max(1u, max_leaf) + 16);
   uint32_t depth = 0;
@@ -263,9 +264,10 @@ BuiltBvh build_bvh(const float* positions, uint32_t num_vertices, const uint32_t
       const Aabb& a = nodes[n.left].box;
       const Aabb& b = nodes[n.right].box;
       float* o = out.nodes.data() + 16 * i;
-      o[0] = a.lo[0]; o[1] = a.hi[0]; o[2] = a.lo[1]; o[3] = a.hi[1];
-      o[4] = b.lo[0]; o[5] = b.hi[0]; o[6] = b.lo[1]; o[7] = b.hi[1];
-      o[8] = a.lo[2]; o[9] = a.hi[2]; o[10] = b.lo[2]; o[11] = b.hi[2];
+      const float p = prm.box_pad;
+      o[0] = a.lo[0] - p; o[1] = a.hi[0] + p; o[2] = a.lo[1] - p; o[3] = a.hi[1] + p;
+      o[4] = b.lo[0] - p; o[5] = b.hi[0] + p; o[6] = b.lo[1] - p; o[7] = b.hi[1] + p;
+      o[8] = a.lo[2] - p; o[9] = a.hi[2] + p; o[10] = b.lo[2] - p; o[11] = b.hi[2] + p;
       uint32_t c0 = nodes[n.left].left < 0 ? child_code(n.left) : (uint32_t)flat_of[n.left];
       uint32_t c1 = nodes[n.right].left < 0 ? child_code(n.right) : (uint32_t)flat_of[n.right];
       leaves += (nodes[n.left].left < 0) + (nodes[n.right].left < 0);
@@ -317,7 +319,7 @@ BuiltBvh build_bvh(const float* positions, uint32_t num_vertices, const uint32_t
         if (s < (int)wide[i].kids.size()) {
           int32_t k = wide[i].kids[s];
           const BNode& n = nodes[k];
-          for (int a = 0; a < 3; ++a) { lo[a] = n.box.lo[a]; hi[a] = n.box.hi[a]; }
+          for (int a = 0; a < 3; ++a) { lo[a] = n.box.lo[a] - prm.box_pad; hi[a] = n.box.hi[a] + prm.box_pad; }
           if (n.left < 0) { code = leaf_code(n.first, n.count); ++leaves; }
           else code = (uint32_t)wide_of[k];
         }

@@ -9,6 +9,8 @@ struct BvhParams {
   uint32_t width = 2;          // 2 or 4
   uint32_t max_leaf_size = 4;  // 1..16
   uint32_t sah_bins = 16;
+  float node_cost = 1.f;       // SAH: cost of one node visit relative to one triangle test
+  float box_pad = 0.f;         // every node box is grown by this much on each side
 };
 
 struct BuiltBvh {

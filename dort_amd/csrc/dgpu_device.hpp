@@ -61,6 +61,26 @@ DG_INLINE float average(V3 c) { return (c.x + c.y + c.z) / 3.0f; }
 DG_INLINE float comp(V3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
 DG_INLINE V3 xyz(float4 a) { return v3(a.x, a.y, a.z); }
 
+// sin and cos of x for |x| <= 8; every caller passes an angle in [-2 pi, 2 pi].
+// Cody-Waite reduction by pi/2 (two-part constant, fused multiply-adds) to
+// r in [-pi/4, pi/4], then the Taylor series of sin to r^9 and of cos to r^10, whose
+// truncation errors (< 2e-9 and < 1e-10) are below half an ulp of the result.  The
+// library's sinf/cosf carry a large-argument reduction that this range never needs
+// and that costs registers in every kernel that inlines it.
+DG_INLINE void sincos_small(float x, float& s, float& c) {
+  float k = rintf(x * 0.63661977236758134f);           // x * 2/pi
+  float r = fmaf(-k, 1.57079637050628662f, x);          // pi/2 rounded to float ...
+  r = fmaf(-k, -4.37113900018624283e-08f, r);           // ... and the rest of pi/2
+  float r2 = r * r;
+  float sp = r + r * r2 * (-1.f / 6.f + r2 * (1.f / 120.f + r2 * (-1.f / 5040.f + r2 * (1.f / 362880.f))));
+  float cp = 1.f + r2 * (-0.5f + r2 * (1.f / 24.f + r2 * (-1.f / 720.f +
+                                                     r2 * (1.f / 40320.f + r2 * (-1.f / 3628800.f)))));
+  int q = (int)k & 3;
+  float ss = (q & 1) ? cp : sp, cc = (q & 1) ? sp : cp;
+  s = (q & 2) ? -ss : ss;
+  c = ((q + 1) & 2) ? -cc : cc;
+}
+
 // geometry.cpp:4-11
 DG_INLINE void coordinate_system(V3 v0, V3& v1, V3& v2) {
   if (fabsf(v0.x) > fabsf(v0.y)) v1 = normalize(v3(-v0.z, 0.f, v0.x));
@@ -142,20 +162,39 @@ struct PathRng {
 enum { BSDF_REFLECTION = 1, BSDF_TRANSMISSION = 2, BSDF_DIFFUSE = 4, BSDF_GLOSSY = 8,
        BSDF_DELTA = 16, BSDF_MODES = 3, BSDF_LOBES = 28, BSDF_ALL = 31 };
 
+// The lobe parameters share three colour slots and two scalars; which kind owns them:
+//   Lambert:        c0 = albedo
+//   GGX conductor:  c0 = tint, c1 = eta, c2 = k, a0 = alpha
+//   dielectric:     c0 = reflect tint, c1 = transmit tint, a0 = ior inside, a1 = ior outside
+// Every field is always written, so the record lives in registers, never on the stack.
 struct Bsdf {
-  V3 nn, sn, tn;   // geometric == shading normal (no shading normals on the GPU path)
+  V3 nn, sn;       // geometric == shading normal (no shading normals on the GPU path); tn = nn x sn
   uint32_t kind;   // DGPU_MAT_*; NONE = no bxdf
-  uint32_t flags;
-  V3 albedo;       // lambert albedo / conductor tint
-  float alpha;
-  V3 eta, k;
-  float ior_in, ior_out;
-  V3 refl, trans;
-  DG_INLINE bool matches(uint32_t req) const {
-    return kind != DGPU_MAT_NONE && (flags & (req & BSDF_MODES)) && (flags & (req & BSDF_LOBES));
+  V3 c0, c1, c2;
+  float a0, a1;
+  // lobe flags by kind: NONE -, LAMBERT reflection|diffuse, GGX reflection|glossy,
+  // DIELECTRIC reflection|transmission|delta
+  DG_INLINE uint32_t flags() const {
+    constexpr uint32_t T = ((uint32_t)(BSDF_REFLECTION | BSDF_TRANSMISSION | BSDF_DELTA) << 24) |
+                           ((uint32_t)(BSDF_REFLECTION | BSDF_GLOSSY) << 16) |
+                           ((uint32_t)(BSDF_REFLECTION | BSDF_DIFFUSE) << 8);
+    return (T >> (8u * kind)) & 0xFFu;
   }
-  DG_INLINE V3 to_local(V3 v) const { return v3(dot(sn, v), dot(tn, v), dot(nn, v)); }
-  DG_INLINE V3 to_world(V3 v) const { return sn * v.x + tn * v.y + nn * v.z; }
+  DG_INLINE V3 tn() const { return cross(nn, sn); }
+  DG_INLINE V3 albedo() const { return c0; }
+  DG_INLINE float alpha() const { return a0; }
+  DG_INLINE V3 eta() const { return c1; }
+  DG_INLINE V3 k() const { return c2; }
+  DG_INLINE V3 refl() const { return c0; }
+  DG_INLINE V3 trans() const { return c1; }
+  DG_INLINE float ior_in() const { return a0; }
+  DG_INLINE float ior_out() const { return a1; }
+  DG_INLINE bool matches(uint32_t req) const {
+    uint32_t f = flags();
+    return kind != DGPU_MAT_NONE && (f & (req & BSDF_MODES)) && (f & (req & BSDF_LOBES));
+  }
+  DG_INLINE V3 to_local(V3 v) const { return v3(dot(sn, v), dot(tn(), v), dot(nn, v)); }
+  DG_INLINE V3 to_world(V3 v) const { return sn * v.x + tn() * v.y + nn * v.z; }
 };
 
 // microfacet.cpp:55-100 (GGX)
@@ -216,7 +255,9 @@ DG_INLINE V3 ggx_sample_vndf(float alpha, V3 wo, float u1, float u2) {
   V3 t1 = lensq > 0.f ? v3(-vh.y, vh.x, 0.f) / sqrtf(lensq) : v3(1.f, 0.f, 0.f);
   V3 t2 = cross(vh, t1);
   float r = sqrtf(u1), phi = TWO_PI_F * u2;
-  float p1 = r * cosf(phi), p2 = r * sinf(phi);
+  float sphi, cphi;
+  sincos_small(phi, sphi, cphi);
+  float p1 = r * cphi, p2 = r * sphi;
   float s = 0.5f * (1.f + vh.z);
   p2 = (1.f - s) * sqrtf(fmaxf(0.f, 1.f - p1 * p1)) + s * p2;
   V3 nh = t1 * p1 + t2 * p2 + vh * sqrtf(fmaxf(0.f, 1.f - p1 * p1 - p2 * p2));
@@ -229,12 +270,13 @@ DG_INLINE V3 conductor_f(const Bsdf& b, V3 wi, V3 wo) {
   V3 h = wi + wo;
   if (length_sq(h) == 0.f) return v3(0.f, 0.f, 0.f);
   h = normalize(h);
-  float g = ggx_g1(b.alpha, wi, h) * ggx_g1(b.alpha, wo, h);
-  float d = ggx_d(b.alpha, h);
+  float g = ggx_g1(b.alpha(), wi, h) * ggx_g1(b.alpha(), wo, h);
+  float d = ggx_d(b.alpha(), h);
   float c = fabsf(dot(wi, h));
-  V3 F = v3(fresnel_conductor1(c, b.eta.x, b.k.x), fresnel_conductor1(c, b.eta.y, b.k.y),
-            fresnel_conductor1(c, b.eta.z, b.k.z));
-  return b.albedo * F * (d * g / (4.f * wi.z * wo.z));
+  V3 eta = b.eta(), k = b.k();
+  V3 F = v3(fresnel_conductor1(c, eta.x, k.x), fresnel_conductor1(c, eta.y, k.y),
+            fresnel_conductor1(c, eta.z, k.z));
+  return b.albedo() * F * (d * g / (4.f * wi.z * wo.z));
 }
 DG_INLINE float conductor_pdf(const Bsdf& b, V3 wi, V3 wo) {
   if (wi.z * wo.z <= 0.f) return 0.f;
@@ -243,12 +285,12 @@ DG_INLINE float conductor_pdf(const Bsdf& b, V3 wi, V3 wo) {
   if (length_sq(h) == 0.f) return 0.f;
   h = normalize(h);
   if (dot(wi, h) <= 0.f) return 0.f;
-  return ggx_g1(b.alpha, wo, h) * ggx_d(b.alpha, h) / (4.f * wo.z);
+  return ggx_g1(b.alpha(), wo, h) * ggx_d(b.alpha(), h) / (4.f * wo.z);
 }
 DG_INLINE float conductor_dlnf_dalpha(const Bsdf& b, V3 wi, V3 wo) {
   if (wo.z < 0.f) { wi.z = -wi.z; wo.z = -wo.z; }
   V3 h = normalize(wi + wo);
-  return ggx_dlnd(b.alpha, h) + ggx_dlng1(b.alpha, wi) + ggx_dlng1(b.alpha, wo);
+  return ggx_dlnd(b.alpha(), h) + ggx_dlng1(b.alpha(), wi) + ggx_dlng1(b.alpha(), wo);
 }
 
 // Bsdf::eval_f (bsdf.cpp:72-93); shading frame == geometric frame, so shading_to_geom = 1
@@ -261,7 +303,7 @@ DG_INLINE V3 bsdf_eval(const Bsdf& b, V3 wi, V3 wo, uint32_t req) {
   if (!b.matches(ereq)) return v3(0.f, 0.f, 0.f);
   float ds = fabsf(dot(b.nn, wi));
   if (ds * ds == 0.f) return v3(0.f, 0.f, 0.f);  // shading_to_geom == 0
-  if (b.kind == DGPU_MAT_LAMBERT) return b.albedo * INV_PI_F;
+  if (b.kind == DGPU_MAT_LAMBERT) return b.albedo() * INV_PI_F;
   if (b.kind == DGPU_MAT_GGX_CONDUCTOR) return conductor_f(b, wil, wol);
   return v3(0.f, 0.f, 0.f);
 }
@@ -276,6 +318,8 @@ DG_INLINE float bsdf_pdf(const Bsdf& b, V3 wi, V3 wo, uint32_t req) {
 
 // Bsdf::sample_light_f with one lobe (bsdf.cpp:95-192) and the lobes'
 // sample functions (lambert_material.cpp:10-19, dielectric_material.cpp:50-104).
+// NON_DELTA: the caller requests no delta lobe (NEE), so the dielectric branch is not compiled.
+template <bool NON_DELTA = false>
 DG_INLINE V3 bsdf_sample(const Bsdf& b, V3 wo, uint32_t req, float u, float v, float c,
                          V3& wi, float& pdf, uint32_t& flags) {
   pdf = 0.f; flags = 0;
@@ -287,18 +331,20 @@ DG_INLINE V3 bsdf_sample(const Bsdf& b, V3 wo, uint32_t req, float u, float v, f
   uint32_t fl = 0;
   if (b.kind == DGPU_MAT_LAMBERT) {
     float r = sqrtf(u), th = TWO_PI_F * v;
-    float x = cosf(th) * r, y = sinf(th) * r;
+    float sth, cth;
+    sincos_small(th, sth, cth);
+    float x = cth * r, y = sth * r;
     float z = sqrtf(fmaxf(0.f, 1.f - x * x - y * y));
     wil = v3(x, y, z);
     p = fabsf(z * INV_PI_F);
     fl = BSDF_REFLECTION | BSDF_DIFFUSE;
     wil.z = copysignf(wil.z, wol.z);
-    f = b.albedo * INV_PI_F;
+    f = b.albedo() * INV_PI_F;
   } else if (b.kind == DGPU_MAT_GGX_CONDUCTOR) {
     if (wol.z == 0.f) return zero;
     float sgn = wol.z < 0.f ? -1.f : 1.f;
     V3 wu = v3(wol.x, wol.y, wol.z * sgn);
-    V3 m = ggx_sample_vndf(b.alpha, wu, u, v);
+    V3 m = ggx_sample_vndf(b.alpha(), wu, u, v);
     float wdm = dot(wu, m);
     V3 w = m * (2.f * wdm) - wu;
     if (w.z <= 0.f || wdm <= 0.f) return zero;
@@ -307,17 +353,17 @@ DG_INLINE V3 bsdf_sample(const Bsdf& b, V3 wo, uint32_t req, float u, float v, f
     if (!(p > 0.f)) return zero;
     fl = BSDF_REFLECTION | BSDF_GLOSSY;
     f = conductor_f(b, wil, wol);
-  } else if (b.kind == DGPU_MAT_DIELECTRIC) {
+  } else if (!NON_DELTA && b.kind == DGPU_MAT_DIELECTRIC) {
     bool s_refl = req & BSDF_REFLECTION, s_trans = req & BSDF_TRANSMISSION;
     float ir, it;
-    if (wol.z >= 0.f) { ir = b.ior_out; it = b.ior_in; } else { ir = b.ior_in; it = b.ior_out; }
+    if (wol.z >= 0.f) { ir = b.ior_out(); it = b.ior_in(); } else { ir = b.ior_in(); it = b.ior_out(); }
     float cos_refl = fabsf(wol.z);
     V3 w_reflect = v3(-wol.x, -wol.y, wol.z);
     V3 w_trans;
     if (!refract_dielectric(wol, w_trans, ir, it)) {
       if (!s_refl) return zero;
       fl = BSDF_DELTA | BSDF_REFLECTION; p = 1.f; wil = w_reflect;
-      f = b.refl / cos_refl;
+      f = b.refl() / cos_refl;
     } else {
       float cos_trans = fabsf(w_trans.z);
       float fr = fresnel_dielectric(cos_refl, cos_trans, ir, it);
@@ -326,10 +372,10 @@ DG_INLINE V3 bsdf_sample(const Bsdf& b, V3 wo, uint32_t req, float u, float v, f
       bool transmit = c > r && s_trans;
       if (reflect) {
         fl = BSDF_DELTA | BSDF_REFLECTION; p = s_trans ? r : 1.f; wil = w_reflect;
-        f = b.refl * (r / cos_refl);
+        f = b.refl() * (r / cos_refl);
       } else if (transmit) {
         fl = BSDF_DELTA | BSDF_TRANSMISSION; p = s_refl ? t : 1.f; wil = w_trans;
-        f = b.trans * (t / cos_trans);
+        f = b.trans() * (t / cos_trans);
       } else {
         return zero;
       }
@@ -396,11 +442,13 @@ DG_INLINE Bsdf make_bsdf(const DevScene& S, const Surface& g, int32_t* texel_out
   Bsdf b;
   b.nn = g.nn;
   float len = length(g.dpdu);
-  if (len < 1e-9f) coordinate_system(b.nn, b.sn, b.tn);
-  else { b.sn = g.dpdu / len; b.tn = cross(b.nn, b.sn); }
+  if (len < 1e-9f) { V3 t; coordinate_system(b.nn, b.sn, t); }  // its t is cross(nn, sn) too
+  else b.sn = g.dpdu / len;
   const DevMaterial m = S.mats[g.mat];
+  V3 zero = v3(0.f, 0.f, 0.f);
   b.kind = DGPU_MAT_NONE;
-  b.flags = 0;
+  b.c0 = zero; b.c1 = zero; b.c2 = zero;
+  b.a0 = 0.f; b.a1 = 0.f;
   if (texel_out) *texel_out = -1;
   if (m.type == DGPU_MAT_LAMBERT) {
     V3 a = xyz(m.albedo);
@@ -410,18 +458,18 @@ DG_INLINE Bsdf make_bsdf(const DevScene& S, const Surface& g, int32_t* texel_out
       a = xyz(S.tex_data[info.z + (uint32_t)ti]);
       if (texel_out && m.tex == 0) *texel_out = ti;
     }
-    if (!is_black(a)) { b.kind = DGPU_MAT_LAMBERT; b.flags = BSDF_REFLECTION | BSDF_DIFFUSE; b.albedo = a; }
+    if (!is_black(a)) { b.kind = DGPU_MAT_LAMBERT; b.c0 = a; }
   } else if (m.type == DGPU_MAT_GGX_CONDUCTOR) {
     V3 a = xyz(m.albedo);
     if (!is_black(a)) {
-      b.kind = DGPU_MAT_GGX_CONDUCTOR; b.flags = BSDF_REFLECTION | BSDF_GLOSSY;
-      b.albedo = a; b.alpha = m.roughness; b.eta = xyz(m.eta); b.k = xyz(m.k);
+      b.kind = DGPU_MAT_GGX_CONDUCTOR;
+      b.c0 = a; b.c1 = xyz(m.eta); b.c2 = xyz(m.k); b.a0 = m.roughness;
     }
   } else if (m.type == DGPU_MAT_DIELECTRIC) {
     V3 r = xyz(m.refl), t = xyz(m.trans);
     if (m.ior_in > 0.f && m.albedo.w > 0.f && !(is_black(r) && is_black(t))) {
-      b.kind = DGPU_MAT_DIELECTRIC; b.flags = BSDF_REFLECTION | BSDF_TRANSMISSION | BSDF_DELTA;
-      b.ior_in = m.ior_in; b.ior_out = m.albedo.w; b.refl = r; b.trans = t;
+      b.kind = DGPU_MAT_DIELECTRIC;
+      b.c0 = r; b.c1 = t; b.a0 = m.ior_in; b.a1 = m.albedo.w;
     }
   }
   return b;
@@ -506,7 +554,9 @@ DG_INLINE V3 env_sample(const DevScene& S, float su, float sv, float& pdf, V3& r
   float u = x / float(S.env_w);
   float v = y / float(S.env_h);
   float theta = v * PI_F, phi = (u - 0.5f) * TWO_PI_F;
-  float ct = cosf(theta), st = sinf(theta), cp = cosf(phi), sp = sinf(phi);
+  float ct, st, cp, sp;
+  sincos_small(theta, st, ct);
+  sincos_small(phi, sp, cp);
   float4 t = env_texel(S, u, v);
   pdf = env_solid_angle_pdf(S, t.w, clampf(st, 0.f, 1.f));
   radiance = xyz(t);

@@ -92,12 +92,14 @@ constexpr uint32_t ADJ_CHUNK = 1u << 20;
 
 bool variant_supported(uint32_t width, uint32_t lds, uint32_t block);
 // which: 0 closest-hit API, 1 any-hit API, 2 extend, 3 connect, 4/5 batched extend/connect
+// (the bounded API variants share the occupancy of 0/1)
 const void* trace_kernel_ptr(int which, uint32_t width, uint32_t lds, uint32_t block);
-const void* shade_kernel_ptr(bool batch);
+const void* shade_kernel_ptr(bool batch, bool all);
 
+// bounded: the ray origins lie within the range the box padding covers (fast box test)
 hipError_t launch_trace_api(const DevScene& S, const KernelCfg& cfg, const dgpu_rays& rays,
                             const dgpu_hits* hits, uint32_t* occluded, uint32_t n, uint32_t* fetch,
-                            uint32_t* spill, hipStream_t st);
+                            uint32_t* spill, bool bounded, hipStream_t st);
 hipError_t launch_raygen(const WaveParams& P, uint32_t capacity, bool batch, hipStream_t st);
 hipError_t launch_extend(const WaveParams& P, const KernelCfg& cfg, uint32_t bounce, bool batch,
                          hipStream_t st);

@@ -48,13 +48,23 @@ struct BvhRef {
   const float4* tris;
 };
 
-template <bool ANY, int WIDTH>
+// FAST: rays of the path tracer (origins inside the scene's padded range, see box_enter);
+// the intersection API takes rays from anywhere and uses the exact box test.
+template <bool ANY, int WIDTH, bool FAST>
 __device__ __forceinline__ bool trace_one(BvhRef b, const TraceRay& r, LaneStack& st, TraceHit& h) {
-  return traverse<ANY, WIDTH>(b.nodes, b.tris, r, st, h);
+  return traverse<ANY, WIDTH, FAST>(b.nodes, b.tris, r, st, h);
 }
 
 // Shared persistent loop: `io.load(i, ray)` returns the ray's BVH, `io.store(i, hit, found)`.
-template <bool ANY, int WIDTH, int LDS, int BLOCK, class IO>
+// Each wave takes 64 consecutive queue entries per atomic and traces them to the end.
+// Consecutive entries are neighbouring pixels (queues are compacted in order), so the
+// lanes of a wave fetch largely the same nodes and triangles.  Refilling finished
+// lanes from the queue as they drain (Aila & Laine 2009, dynamic fetch) was measured
+// 1.2-3.8x slower on MI355X: the refilled lanes hold rays from distant parts of the
+// queue and that coherence is lost.  Giving each XCD (blockIdx % 8) its own contiguous
+// eighth of the queue was also slower (7%): with one queue front the whole chip works
+// on one window of neighbouring rays, which the shared 256 MB Infinity Cache serves.
+template <bool ANY, int WIDTH, int LDS, int BLOCK, bool FAST, class IO>
 __device__ __forceinline__ void trace_loop(IO& io, uint32_t n, uint32_t* fetch, uint32_t* spill,
                                            uint32_t stack_cap) {
   __shared__ uint32_t stk[LDS * BLOCK];
@@ -76,7 +86,7 @@ __device__ __forceinline__ void trace_loop(IO& io, uint32_t n, uint32_t* fetch, 
       TraceRay r;
       BvhRef bvh = io.load(i, r);
       TraceHit h;
-      bool found = trace_one<ANY, WIDTH>(bvh, r, st, h);
+      bool found = trace_one<ANY, WIDTH, FAST>(bvh, r, st, h);
       io.store(i, h, found);
     }
   }
@@ -104,10 +114,10 @@ struct ApiIO {
   }
 };
 
-template <bool ANY, int WIDTH, int LDS, int BLOCK>
+template <bool ANY, int WIDTH, int LDS, int BLOCK, bool FAST>
 __global__ __launch_bounds__(BLOCK) void k_trace_api(ApiIO io, uint32_t n, uint32_t* fetch,
                                                      uint32_t* spill, uint32_t stack_cap) {
-  trace_loop<ANY, WIDTH, LDS, BLOCK>(io, n, fetch, spill, stack_cap);
+  trace_loop<ANY, WIDTH, LDS, BLOCK, FAST>(io, n, fetch, spill, stack_cap);
 }
 
 // Batched queues carry the scene index in the top byte of the path / slot word.
@@ -175,7 +185,7 @@ __global__ __launch_bounds__(BLOCK) void k_extend(ExtendIO<BATCH> io, const uint
                                                   const BatchEntry* batch, uint32_t nscenes) {
   __shared__ BvhRef tab[BATCH ? MAX_BATCH : 1];
   io.table = load_bvh_table<BATCH, BLOCK>(batch, nscenes, tab);
-  trace_loop<false, WIDTH, LDS, BLOCK>(io, *count, fetch, spill, stack_cap);
+  trace_loop<false, WIDTH, LDS, BLOCK, true>(io, *count, fetch, spill, stack_cap);
 }
 
 template <int WIDTH, int LDS, int BLOCK, bool BATCH>
@@ -184,7 +194,7 @@ __global__ __launch_bounds__(BLOCK) void k_connect(ConnectIO<BATCH> io, const ui
                                                    const BatchEntry* batch, uint32_t nscenes) {
   __shared__ BvhRef tab[BATCH ? MAX_BATCH : 1];
   io.table = load_bvh_table<BATCH, BLOCK>(batch, nscenes, tab);
-  trace_loop<true, WIDTH, LDS, BLOCK>(io, *count, fetch, spill, stack_cap);
+  trace_loop<true, WIDTH, LDS, BLOCK, true>(io, *count, fetch, spill, stack_cap);
 }
 
 __global__ void k_zero_u32(uint32_t* p, uint32_t n) {
@@ -293,7 +303,9 @@ __device__ __forceinline__ V3 light_sample(const DevScene& S, const DevLight& L,
     return rad;
   } else {
     float z = 1.f - 2.f * u, r = sqrtf(fmaxf(0.f, 1.f - z * z)), phi = TWO_PI_F * v;
-    wi = v3(cosf(phi) * r, sinf(phi) * r, z);
+    float sphi, cphi;
+    sincos_small(phi, sphi, cphi);
+    wi = v3(cphi * r, sphi * r, z);
     pdf = INV_FOUR_PI_F;
     sh.o = p; sh.d = normalize(wi); sh.tmin = eps; sh.tmax = INFINITY; sh.ignore = 0xFFFFFFFFu;
     return xyz(L.radiance);
@@ -313,7 +325,68 @@ __device__ __forceinline__ V3 light_tri_normal(const DevScene& S, uint32_t tri) 
   return normalize(cross(xyz(c) - p0, xyz(b) - p0));
 }
 
-// estimate_direct (path_renderer.cpp:135-206) for one light; writes up to two shadow rays.
+// estimate_direct_from_bsdf (path_renderer.cpp:149-182) for light L: samples the BSDF
+// with (u, v, c) and, if the direction reaches L, returns the shadow ray and the
+// contribution it adds when unoccluded.  use_light: MIS against light sampling.
+__device__ __forceinline__ NeeOut nee_from_bsdf(const DevScene& S, const Surface& g, V3 wo, const Bsdf& bsdf,
+                                                const DevLight& L, float scale, V3 thr, bool use_light,
+                                                float u, float v, float c) {
+  const uint32_t NON_DELTA = BSDF_ALL & ~BSDF_DELTA;
+  NeeOut nb;
+  nb.emit = false;
+  V3 wi; float pdf; uint32_t fl;
+  V3 f = bsdf_sample<true>(bsdf, wo, NON_DELTA, u, v, c, wi, pdf, fl);
+  if (pdf == 0.f || is_black(f)) return nb;
+  V3 rad = v3(0.f, 0.f, 0.f);
+  float tmax = INFINITY, lp = 0.f;
+  uint32_t ignore = 0xFFFFFFFFu;
+  if (L.kind == LK_AREA) {
+    float tl;
+    if (light_tri_hit(S, L.tri, g.p, wi, tl) && tl >= g.eps) {
+      V3 nl_ = light_tri_normal(S, L.tri);
+      if (dot(nl_, -wi) >= 0.f) rad = xyz(L.radiance);  // eval_radiance, diffuse_light.cpp:83-93
+      lp = (1.f / L.area) * (tl * tl * length_sq(wi)) / fabsf(dot(wi, nl_));
+      tmax = tl;
+      ignore = L.tri;
+    }
+  } else if (L.kind == LK_ENV) {
+    rad = env_eval(S, wi, lp);
+  } else {
+    rad = xyz(L.radiance);
+    lp = INV_FOUR_PI_F;
+  }
+  if (is_black(rad)) return nb;
+  float w = use_light ? pdf / (pdf + lp) : 1.f;
+  nb.emit = true;
+  nb.o = g.p; nb.d = wi; nb.tmin = g.eps; nb.tmax = tmax; nb.ignore = ignore;
+  nb.contrib = thr * f * rad * (w * fabsf(dot(g.nn, wi)) / pdf * scale);
+  return nb;
+}
+
+// estimate_direct_from_light (path_renderer.cpp:184-206) for light L with the light
+// sample (u, v).  use_bsdf: MIS against BSDF sampling.
+__device__ __forceinline__ NeeOut nee_from_light(const DevScene& S, const Surface& g, V3 wo, const Bsdf& bsdf,
+                                                 const DevLight& L, float scale, V3 thr, bool use_bsdf,
+                                                 float u, float v) {
+  const uint32_t NON_DELTA = BSDF_ALL & ~BSDF_DELTA;
+  NeeOut nl;
+  nl.emit = false;
+  V3 wi; float pdf;
+  V3 rad = light_sample(S, L, g.p, g.eps, u, v, wi, pdf, nl);
+  if (pdf == 0.f || is_black(rad)) return nl;
+  V3 f = bsdf_eval(bsdf, wi, wo, NON_DELTA);
+  if (is_black(f)) return nl;
+  float w = 1.f;
+  if (use_bsdf) {
+    float bp = bsdf_pdf(bsdf, wi, wo, NON_DELTA);
+    w = pdf / (pdf + bp);
+  }
+  nl.emit = true;
+  nl.contrib = thr * f * rad * (w * fabsf(dot(g.nn, wi)) / pdf * scale);
+  return nl;
+}
+
+// estimate_direct (path_renderer.cpp:135-206) for light li: up to two shadow rays.
 __device__ __forceinline__ void estimate_direct(const DevScene& S, const WaveParams& P, const Surface& g,
                                                 V3 wo, const Bsdf& bsdf, uint32_t li, float scale, V3 thr,
                                                 const float rb0[4], const float rb1[4],
@@ -323,53 +396,8 @@ __device__ __forceinline__ void estimate_direct(const DevScene& S, const WavePar
   bool use_light = P.strategy != DGPU_DIRECT_BSDF;
   nb.emit = false;
   nl.emit = false;
-  const uint32_t NON_DELTA = BSDF_ALL & ~BSDF_DELTA;
-  if (use_bsdf) {  // estimate_direct_from_bsdf, path_renderer.cpp:149-182
-    V3 wi; float pdf; uint32_t fl;
-    V3 f = bsdf_sample(bsdf, wo, NON_DELTA, rb0[1], rb0[2], rb0[3], wi, pdf, fl);
-    if (pdf != 0.f && !is_black(f)) {
-      V3 rad = v3(0.f, 0.f, 0.f);
-      float tmax = INFINITY, lp = 0.f;
-      uint32_t ignore = 0xFFFFFFFFu;
-      if (L.kind == LK_AREA) {
-        float tl;
-        if (light_tri_hit(S, L.tri, g.p, wi, tl) && tl >= g.eps) {
-          V3 nl_ = light_tri_normal(S, L.tri);
-          if (dot(nl_, -wi) >= 0.f) rad = xyz(L.radiance);  // eval_radiance, diffuse_light.cpp:83-93
-          lp = (1.f / L.area) * (tl * tl * length_sq(wi)) / fabsf(dot(wi, nl_));
-          tmax = tl;
-          ignore = L.tri;
-        }
-      } else if (L.kind == LK_ENV) {
-        rad = env_eval(S, wi, lp);
-      } else {
-        rad = xyz(L.radiance);
-        lp = INV_FOUR_PI_F;
-      }
-      if (!is_black(rad)) {
-        float w = use_light ? pdf / (pdf + lp) : 1.f;
-        nb.emit = true;
-        nb.o = g.p; nb.d = wi; nb.tmin = g.eps; nb.tmax = tmax; nb.ignore = ignore;
-        nb.contrib = thr * f * rad * (w * fabsf(dot(g.nn, wi)) / pdf * scale);
-      }
-    }
-  }
-  if (use_light) {  // estimate_direct_from_light, path_renderer.cpp:184-206
-    V3 wi; float pdf;
-    V3 rad = light_sample(S, L, g.p, g.eps, rb1[0], rb1[1], wi, pdf, nl);
-    if (pdf != 0.f && !is_black(rad)) {
-      V3 f = bsdf_eval(bsdf, wi, wo, NON_DELTA);
-      if (!is_black(f)) {
-        float w = 1.f;
-        if (use_bsdf) {
-          float bp = bsdf_pdf(bsdf, wi, wo, NON_DELTA);
-          w = pdf / (pdf + bp);
-        }
-        nl.emit = true;
-        nl.contrib = thr * f * rad * (w * fabsf(dot(g.nn, wi)) / pdf * scale);
-      }
-    }
-  }
+  if (use_bsdf) nb = nee_from_bsdf(S, g, wo, bsdf, L, scale, thr, use_light, rb0[1], rb0[2], rb0[3]);
+  if (use_light) nl = nee_from_light(S, g, wo, bsdf, L, scale, thr, use_bsdf, rb1[0], rb1[1]);
 }
 
 __device__ __forceinline__ void emit_shadow(const WaveParams& P, const NeeOut& s, uint32_t slot_word,
@@ -382,7 +410,7 @@ __device__ __forceinline__ void emit_shadow(const WaveParams& P, const NeeOut& s
 // One bounce of PathRenderer::sample for every path in the current queue.
 // Every lane of a wave runs the same number of queue appends (inactive lanes
 // append nothing), so the ballot-based compaction sees the whole wave.
-template <bool BATCH>
+template <bool BATCH, bool ALL>
 __device__ __forceinline__ void shade_body(const WaveParams& P, const DevScene& S, uint32_t bounce,
                                            bool active, uint32_t i, uint32_t p, uint32_t scene,
                                            uint32_t pixel, uint32_t sample, uint32_t k0, uint32_t k1) {
@@ -394,6 +422,8 @@ __device__ __forceinline__ void shade_body(const WaveParams& P, const DevScene& 
   Bsdf bsdf;
   bsdf.kind = DGPU_MAT_NONE;
   bool shading = false;
+  // random blocks are drawn where they are first needed (they are pure functions of
+  // the path, bounce and block), so fewer of them are live across the NEE code
   float rb0[4] = {0.f, 0.f, 0.f, 0.f}, rb1[4] = {0.f, 0.f, 0.f, 0.f}, rb2[4] = {0.f, 0.f, 0.f, 0.f};
   if (active) {
     float4 t4 = P.thr[p], r4 = P.rad[p];
@@ -428,24 +458,22 @@ __device__ __forceinline__ void shade_body(const WaveParams& P, const DevScene& 
       wo = normalize(-rd);
       bsdf = make_bsdf(S, g);
       dgpu_rng_block(k0, k1, pixel, sample, bounce, 0, rb0);
-      dgpu_rng_block(k0, k1, pixel, sample, bounce, 1, rb1);
-      dgpu_rng_block(k0, k1, pixel, sample, bounce, 2, rb2);
     }
   }
   // direct lighting (path_renderer.cpp:68-76, 113-132): up to 2 shadow rays per light
   bool do_nee = shading && bounces >= P.min_depth && bsdf.matches(BSDF_ALL & ~BSDF_DELTA) &&
                 S.num_lights > 0u;
-  uint32_t lights_loop = (!BATCH && P.sample_all) ? P.S.num_lights : 1u;
+  uint32_t lights_loop = ALL ? P.S.num_lights : 1u;
   const uint32_t scene_bits = scene << SCENE_SHIFT;
+  const bool use_bsdf = P.strategy != DGPU_DIRECT_LIGHT, use_light = P.strategy != DGPU_DIRECT_BSDF;
   for (uint32_t li = 0; li < lights_loop; ++li) {
-    NeeOut nb, nl;
-    nb.emit = false;
-    nl.emit = false;
+    // one light: all of them in turn (sample_all_lights), or one picked by power with
+    // DiscreteDistrib1d::sample / pdf (discrete_distrib_1d.cpp:22-42)
+    uint32_t light = li;
+    float scale = 0.f;
     if (do_nee) {
-      if (!BATCH && P.sample_all) {
-        estimate_direct(S, P, g, wo, bsdf, li, 1.f, thr, rb0, rb1, nb, nl);
-      } else {
-        // DiscreteDistrib1d::sample / pdf (discrete_distrib_1d.cpp:22-42)
+      scale = 1.f;
+      if (!ALL) {
         float u = rb0[0];
         uint32_t b = 0, e = S.num_lights;
         while (b + 1u < e) {
@@ -453,19 +481,37 @@ __device__ __forceinline__ void shade_body(const WaveParams& P, const DevScene& 
           if (S.light_cdf[m] < u) b = m; else e = m;
         }
         float pp = S.light_cdf[b + 1] - S.light_cdf[b];
-        if (pp != 0.f) estimate_direct(S, P, g, wo, bsdf, b, 1.f / pp, thr, rb0, rb1, nb, nl);
+        light = b;
+        scale = pp != 0.f ? 1.f / pp : 0.f;
       }
     }
+    const bool nee = scale != 0.f;
+    DevLight L;
+    if (nee) L = S.lights[light];
+    // the two strategies one after the other, each shadow ray queued as soon as it exists
     uint32_t slot_b = (uint32_t)((size_t)p * SL + 2 * li), slot_l = slot_b + 1;
-    uint32_t qb = wave_append(&P.counters[CNT_SHADOW + bounce], nb.emit);
-    if (nb.emit) {
-      P.pend[slot_b] = make_float4(nb.contrib.x, nb.contrib.y, nb.contrib.z, 1.f);
-      emit_shadow(P, nb, slot_b | scene_bits, qb);
+    {
+      NeeOut nb;
+      nb.emit = false;
+      if (nee && use_bsdf) nb = nee_from_bsdf(S, g, wo, bsdf, L, scale, thr, use_light, rb0[1], rb0[2], rb0[3]);
+      uint32_t qb = wave_append(&P.counters[CNT_SHADOW + bounce], nb.emit);
+      if (nb.emit) {
+        P.pend[slot_b] = make_float4(nb.contrib.x, nb.contrib.y, nb.contrib.z, 1.f);
+        emit_shadow(P, nb, slot_b | scene_bits, qb);
+      }
     }
-    uint32_t ql = wave_append(&P.counters[CNT_SHADOW + bounce], nl.emit);
-    if (nl.emit) {
-      P.pend[slot_l] = make_float4(nl.contrib.x, nl.contrib.y, nl.contrib.z, 1.f);
-      emit_shadow(P, nl, slot_l | scene_bits, ql);
+    {
+      NeeOut nl;
+      nl.emit = false;
+      if (nee && use_light) {
+        dgpu_rng_block(k0, k1, pixel, sample, bounce, 1, rb1);
+        nl = nee_from_light(S, g, wo, bsdf, L, scale, thr, use_bsdf, rb1[0], rb1[1]);
+      }
+      uint32_t ql = wave_append(&P.counters[CNT_SHADOW + bounce], nl.emit);
+      if (nl.emit) {
+        P.pend[slot_l] = make_float4(nl.contrib.x, nl.contrib.y, nl.contrib.z, 1.f);
+        emit_shadow(P, nl, slot_l | scene_bits, ql);
+      }
     }
   }
   // continuation: Bsdf::sample_light_f, throughput, Russian roulette (path_renderer.cpp:78-107)
@@ -475,6 +521,8 @@ __device__ __forceinline__ void shade_body(const WaveParams& P, const DevScene& 
   if (shading) {
     uint32_t req = (P.only_direct || bounces == P.max_depth) ? (BSDF_MODES | BSDF_DELTA) : BSDF_ALL;
     V3 wi; float pdf; uint32_t fl;
+    dgpu_rng_block(k0, k1, pixel, sample, bounce, 1, rb1);
+    dgpu_rng_block(k0, k1, pixel, sample, bounce, 2, rb2);
     V3 f = bsdf_sample(bsdf, wo, req, rb1[2], rb1[3], rb2[0], wi, pdf, fl);
     if (!is_black(f) && pdf != 0.f) {
       V3 contrib = f * (fabsf(dot(wi, g.nn)) / pdf);
@@ -510,8 +558,13 @@ __device__ __forceinline__ void shade_body(const WaveParams& P, const DevScene& 
 // A fixed grid strides over the queue, whose length is known only on the device;
 // the loop bound is uniform per workgroup, so every wave runs the same number of
 // queue appends.
-template <bool BATCH>
-__global__ __launch_bounds__(256) void k_shade(WaveParams P, uint32_t bounce) {
+#ifndef DGPU_SHADE_WAVES
+#define DGPU_SHADE_WAVES 4
+#endif
+// ALL: sample_all_lights (one NEE per light and vertex) instead of one light picked by power.
+template <bool BATCH, bool ALL>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DGPU_SHADE_WAVES)))
+void k_shade(WaveParams P, uint32_t bounce) {
   const uint32_t cur = bounce & 1u;
   const uint32_t n = P.counters[CNT_QUEUE + bounce];
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
@@ -526,13 +579,13 @@ __global__ __launch_bounds__(256) void k_shade(WaveParams P, uint32_t bounce) {
         ps = P.pid[p];
       }
       const BatchEntry& E = P.batch[scene];
-      shade_body<true>(P, E.S, bounce, active, i, p, scene, ps.x, ps.y, E.k0, E.k1);
+      shade_body<true, false>(P, E.S, bounce, active, i, p, scene, ps.x, ps.y, E.k0, E.k1);
     } else {
       uint32_t p = active ? P.qpath[cur][i] : 0u;
       uint64_t gidx = P.g0 + p;
       uint32_t sample = P.sample_begin + (uint32_t)(gidx / P.npix);
       uint32_t pixel = (uint32_t)(gidx % P.npix);
-      shade_body<false>(P, P.S, bounce, active, i, p, 0u, pixel, sample, P.k0, P.k1);
+      shade_body<false, ALL>(P, P.S, bounce, active, i, p, 0u, pixel, sample, P.k0, P.k1);
     }
   }
 }
@@ -645,9 +698,9 @@ __device__ __forceinline__ void adj_deposit(const AdjSink& k, const Bsdf& b, int
   if (b.kind == DGPU_MAT_LAMBERT && texel >= 0 && k.g_tex) {
     // f = albedo / pi: d ln f_c / d albedo_c = 1 / albedo_c
     float* g = k.g_tex + 3 * (size_t)texel;
-    if (b.albedo.x > 0.f) atomicAdd(g + 0, k.dl[0] * carried.x / b.albedo.x);
-    if (b.albedo.y > 0.f) atomicAdd(g + 1, k.dl[1] * carried.y / b.albedo.y);
-    if (b.albedo.z > 0.f) atomicAdd(g + 2, k.dl[2] * carried.z / b.albedo.z);
+    if (b.albedo().x > 0.f) atomicAdd(g + 0, k.dl[0] * carried.x / b.albedo().x);
+    if (b.albedo().y > 0.f) atomicAdd(g + 1, k.dl[1] * carried.y / b.albedo().y);
+    if (b.albedo().z > 0.f) atomicAdd(g + 2, k.dl[2] * carried.z / b.albedo().z);
   } else if (b.kind == DGPU_MAT_GGX_CONDUCTOR && k.g_rough) {
     float r = conductor_dlnf_dalpha(b, b.to_local(wi), b.to_local(wo));
     float v = r * (k.dl[0] * carried.x + k.dl[1] * carried.y + k.dl[2] * carried.z);
@@ -679,7 +732,7 @@ __device__ V3 replay_path(const WaveParams& P, uint32_t pixel, uint32_t sample, 
   const uint32_t NON_DELTA = BSDF_ALL & ~BSDF_DELTA;
   for (uint32_t bounce = 0; bounce < MAX_BOUNCES; ++bounce) {
     TraceHit h;
-    bool is = trace_one<false, WIDTH>(BvhRef{S.nodes, S.tris}, ray, st, h);
+    bool is = trace_one<false, WIDTH, true>(BvhRef{S.nodes, S.tris}, ray, st, h);
     Surface g;
     if (is) g = surface_at(S, h.tri, h.t, h.b1, h.b2);
     if ((bounces == 0u || last_delta) && bounces >= P.min_depth && bounces <= P.max_depth) {
@@ -718,17 +771,17 @@ __device__ V3 replay_path(const WaveParams& P, uint32_t pixel, uint32_t sample, 
           float pp = S.light_cdf[b + 1] - S.light_cdf[b];
           if (pp != 0.f) estimate_direct(S, P, g, wo, bsdf, b, 1.f / pp, thr, rb0, rb1, nb, nl);
         }
-        NeeOut* outs[2] = {&nb, &nl};
-        for (int k = 0; k < 2; ++k) {
-          const NeeOut& n = *outs[k];
-          if (!n.emit) continue;
+        auto shadow = [&](const NeeOut& n) {
+          if (!n.emit) return;
           TraceRay sr;
           sr.o = n.o; sr.d = n.d; sr.tmin = n.tmin; sr.tmax = n.tmax; sr.ignore = n.ignore;
           TraceHit sh;
-          if (trace_one<true, WIDTH>(BvhRef{S.nodes, S.tris}, sr, st, sh)) continue;
+          if (trace_one<true, WIDTH, true>(BvhRef{S.nodes, S.tris}, sr, st, sh)) return;
           rad = rad + n.contrib;
           if (ADJ) adj_deposit(sink, bsdf, texel, mat, n.d, wo, n.contrib);
-        }
+        };
+        shadow(nb);
+        shadow(nl);
       }
     }
     uint32_t req = (P.only_direct || bounces == P.max_depth) ? (BSDF_MODES | BSDF_DELTA) : BSDF_ALL;
@@ -817,8 +870,8 @@ bool variant_supported(uint32_t width, uint32_t lds, uint32_t block) {
 const void* trace_kernel_ptr(int which, uint32_t width, uint32_t lds, uint32_t block) {
 #define X(W, L, B)                                                                         \
   if (width == W && lds == L && block == B) {                                              \
-    if (which == 0) return (const void*)&k_trace_api<false, W, L, B>;                      \
-    if (which == 1) return (const void*)&k_trace_api<true, W, L, B>;                       \
+    if (which == 0) return (const void*)&k_trace_api<false, W, L, B, false>;               \
+    if (which == 1) return (const void*)&k_trace_api<true, W, L, B, false>;                \
     if (which == 2) return (const void*)&k_extend<W, L, B, false>;                         \
     if (which == 3) return (const void*)&k_connect<W, L, B, false>;                        \
     if (which == 4) return (const void*)&k_extend<W, L, B, true>;                          \
@@ -831,7 +884,7 @@ const void* trace_kernel_ptr(int which, uint32_t width, uint32_t lds, uint32_t b
 
 hipError_t launch_trace_api(const DevScene& S, const KernelCfg& cfg, const dgpu_rays& rays,
                             const dgpu_hits* hits, uint32_t* occluded, uint32_t n, uint32_t* fetch,
-                            uint32_t* spill, hipStream_t st) {
+                            uint32_t* spill, bool bounded, hipStream_t st) {
   ApiIO io;
   io.rays = rays;
   if (hits) io.hits = *hits; else io.hits = dgpu_hits{nullptr, nullptr, nullptr, nullptr};
@@ -839,13 +892,21 @@ hipError_t launch_trace_api(const DevScene& S, const KernelCfg& cfg, const dgpu_
   io.bvh = BvhRef{S.nodes, S.tris};
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, fetch, 1u);
   bool any = occluded != nullptr;
-#define X(W, L, B)                                                                               \
-  if (cfg.width == W && cfg.lds == L && cfg.block == B) {                                        \
-    if (any) hipLaunchKernelGGL((k_trace_api<true, W, L, B>), dim3(cfg.grid), dim3(B), 0, st, io, n, \
-                                fetch, spill, S.stack_cap);                          \
-    else hipLaunchKernelGGL((k_trace_api<false, W, L, B>), dim3(cfg.grid), dim3(B), 0, st, io, n, \
-                            fetch, spill, S.stack_cap);                              \
-    return hipGetLastError();                                                                    \
+#define X(W, L, B)                                                                                 \
+  if (cfg.width == W && cfg.lds == L && cfg.block == B) {                                          \
+    if (any && bounded)                                                                            \
+      hipLaunchKernelGGL((k_trace_api<true, W, L, B, true>), dim3(cfg.grid), dim3(B), 0, st, io, n,  \
+                         fetch, spill, S.stack_cap);                                               \
+    else if (any)                                                                                  \
+      hipLaunchKernelGGL((k_trace_api<true, W, L, B, false>), dim3(cfg.grid), dim3(B), 0, st, io, n, \
+                         fetch, spill, S.stack_cap);                                               \
+    else if (bounded)                                                                              \
+      hipLaunchKernelGGL((k_trace_api<false, W, L, B, true>), dim3(cfg.grid), dim3(B), 0, st, io, n, \
+                         fetch, spill, S.stack_cap);                                               \
+    else                                                                                           \
+      hipLaunchKernelGGL((k_trace_api<false, W, L, B, false>), dim3(cfg.grid), dim3(B), 0, st, io, n, \
+                         fetch, spill, S.stack_cap);                                               \
+    return hipGetLastError();                                                                      \
   }
   DG_VARIANTS(X)
 #undef X
@@ -900,15 +961,17 @@ hipError_t launch_raygen(const WaveParams& P, uint32_t capacity, bool batch, hip
   return hipGetLastError();
 }
 
-const void* shade_kernel_ptr(bool batch) {
-  return batch ? (const void*)&k_shade<true> : (const void*)&k_shade<false>;
+const void* shade_kernel_ptr(bool batch, bool all) {
+  if (batch) return (const void*)&k_shade<true, false>;
+  return all ? (const void*)&k_shade<false, true> : (const void*)&k_shade<false, false>;
 }
 
 hipError_t launch_shade(const WaveParams& P, uint32_t capacity, uint32_t bounce, bool batch, hipStream_t st,
                         uint32_t max_blocks) {
   uint32_t blocks = std::min((capacity + 255u) / 256u, std::max(1u, max_blocks));
-  if (batch) hipLaunchKernelGGL(k_shade<true>, dim3(blocks), dim3(256), 0, st, P, bounce);
-  else hipLaunchKernelGGL(k_shade<false>, dim3(blocks), dim3(256), 0, st, P, bounce);
+  if (batch) hipLaunchKernelGGL((k_shade<true, false>), dim3(blocks), dim3(256), 0, st, P, bounce);
+  else if (P.sample_all) hipLaunchKernelGGL((k_shade<false, true>), dim3(blocks), dim3(256), 0, st, P, bounce);
+  else hipLaunchKernelGGL((k_shade<false, false>), dim3(blocks), dim3(256), 0, st, P, bounce);
   return hipGetLastError();
 }
 
@@ -936,3 +999,4 @@ hipError_t launch_adjoint(const WaveParams& P, const AdjArgs& A, uint32_t width,
 }
 
 }  // namespace dgpu
+

@@ -7,7 +7,7 @@
 //
 // * Triangle test: Woop, Benthin, Wald, "Watertight ray/triangle intersection",
 //   JCGT 2(1) 2013, two-sided like dort, with the FP64 re-evaluation of zero
-//   edge functions.  Accepts t in [tmin, tmax]; a later hit at equal t replaces
+//   edge functions and the ray-space transform applied as three FMA rows.  Accepts t in [tmin, tmax]; a later hit at equal t replaces
 //   the current one, as dort's `t > t_max` rejection does (triangle.cpp:131).
 // * Box test: slabs with (lo - o) * inv_dir and the far distance scaled by
 //   1 + 2*gamma(3) (Ize 2013, "Robust BVH ray traversal"), so a box the ray
@@ -57,11 +57,21 @@ struct LaneStack {
   }
 };
 
+// Per-ray part of the watertight test: the ray-space transform of Woop et al. as three
+// row vectors, x = rx . (v - o), y = ry . (v - o), z = rz . (v - o), with
+// rx = e_kx - sx e_kz, ry = e_ky - sy e_kz, rz = sz e_kz (kz the dominant axis of the
+// direction).  Applying the rows with fused multiply-adds replaces the per-vertex
+// component permutation by three FMAs per coordinate; the zero entries add exact
+// zeros, so x = fma(-sx, A[kz], A[kx]) etc.  Every vertex is transformed the same way
+// in every triangle that shares it, which is all that watertightness needs.
 struct WtRay {
-  int kx, ky, kz;
-  float sx, sy, sz;
   V3 o;
+  V3 rx, ry, rz;
 };
+
+DG_INLINE V3 axis_scaled(int k, float v) {
+  return v3(k == 0 ? v : 0.f, k == 1 ? v : 0.f, k == 2 ? v : 0.f);
+}
 
 DG_INLINE WtRay wt_setup(V3 o, V3 d) {
   WtRay w;
@@ -71,25 +81,28 @@ DG_INLINE WtRay wt_setup(V3 o, V3 d) {
   int ky = kx + 1 == 3 ? 0 : kx + 1;
   float dz = comp(d, kz);
   if (dz < 0.f) { int t = kx; kx = ky; ky = t; }
-  w.kx = kx; w.ky = ky; w.kz = kz;
-  w.sx = comp(d, kx) / dz;
-  w.sy = comp(d, ky) / dz;
-  w.sz = 1.0f / dz;
+  float sx = comp(d, kx) / dz, sy = comp(d, ky) / dz, sz = 1.0f / dz;
   w.o = o;
+  w.rx = axis_scaled(kx, 1.f) + axis_scaled(kz, -sx);
+  w.ry = axis_scaled(ky, 1.f) + axis_scaled(kz, -sy);
+  w.rz = axis_scaled(kz, sz);
   return w;
 }
+
+DG_INLINE float row_dot(V3 r, V3 a) { return fmaf(r.z, a.z, fmaf(r.y, a.y, r.x * a.x)); }
 
 // Watertight test; on a hit inside [tmin, tmax] returns true with t, b1, b2.
 // Contraction into FMA is off here: an edge shared by two triangles must give edge
 // functions of exactly opposite sign, which needs both products rounded the same way.
+// The range test runs first on T against det * [tmin, tmax] (widened by 2^-20), so
+// the reciprocal is taken only for candidates; the final test is on t itself.
 DG_INLINE bool wt_hit(const WtRay& w, float4 va, float4 vb, float4 vc, float tmin, float tmax,
                       float& t_out, float& b1_out, float& b2_out) {
 #pragma clang fp contract(off)
   V3 A = xyz(va) - w.o, B = xyz(vb) - w.o, C = xyz(vc) - w.o;
-  float Az = comp(A, w.kz), Bz = comp(B, w.kz), Cz = comp(C, w.kz);
-  float Ax = comp(A, w.kx) - w.sx * Az, Ay = comp(A, w.ky) - w.sy * Az;
-  float Bx = comp(B, w.kx) - w.sx * Bz, By = comp(B, w.ky) - w.sy * Bz;
-  float Cx = comp(C, w.kx) - w.sx * Cz, Cy = comp(C, w.ky) - w.sy * Cz;
+  float Ax = row_dot(w.rx, A), Ay = row_dot(w.ry, A);
+  float Bx = row_dot(w.rx, B), By = row_dot(w.ry, B);
+  float Cx = row_dot(w.rx, C), Cy = row_dot(w.ry, C);
   float U = Cx * By - Cy * Bx;
   float V = Ax * Cy - Ay * Cx;
   float W = Bx * Ay - By * Ax;
@@ -101,10 +114,14 @@ DG_INLINE bool wt_hit(const WtRay& w, float4 va, float4 vb, float4 vc, float tmi
   if ((U < 0.f || V < 0.f || W < 0.f) && (U > 0.f || V > 0.f || W > 0.f)) return false;
   float det = U + V + W;
   if (det == 0.f) return false;
-  float T = U * (w.sz * Az) + V * (w.sz * Bz) + W * (w.sz * Cz);
-  float t = T / det;
-  if (!(t >= tmin && t <= tmax)) return false;
+  float Az = row_dot(w.rz, A), Bz = row_dot(w.rz, B), Cz = row_dot(w.rz, C);
+  float T = fmaf(U, Az, fmaf(V, Bz, W * Cz));
+  uint32_t sgn = __float_as_uint(det) & 0x80000000u;
+  float Tn = __uint_as_float(__float_as_uint(T) ^ sgn), dn = fabsf(det);
+  if (Tn > tmax * dn * 1.000001f || Tn < tmin * dn * (tmin > 0.f ? 0.999999f : 1.000001f)) return false;
   float inv = 1.0f / det;
+  float t = T * inv;
+  if (!(t >= tmin && t <= tmax)) return false;
   t_out = t;
   b1_out = V * inv;
   b2_out = W * inv;
@@ -122,11 +139,25 @@ DG_INLINE float robust_inv(float d) {
 }
 
 // Slab test of one box against [tmin, tmax]; returns the entry distance or +inf on a miss.
+// Exact form (any ray origin): (b - o) * inv, far distance scaled by 1 + 2 gamma(3).
+// FAST form: fma(b, inv, -o * inv), one instruction per plane instead of two.  Its
+// error in distance along an axis is below 3u M (u = 2^-24) where M bounds |o| and
+// |b|; the builder pads every box by M 2^-21 > 3u M for M covering the scene and the
+// camera, so for rays that start inside that range (all rays of the path tracer) the
+// FAST test never culls a box that the ray touches.  `o` holds -o * inv for FAST.
+template <bool FAST>
 DG_INLINE float box_enter(float lox, float hix, float loy, float hiy, float loz, float hiz,
                           V3 o, V3 inv, float tmin, float tmax) {
-  float tx0 = (lox - o.x) * inv.x, tx1 = (hix - o.x) * inv.x;
-  float ty0 = (loy - o.y) * inv.y, ty1 = (hiy - o.y) * inv.y;
-  float tz0 = (loz - o.z) * inv.z, tz1 = (hiz - o.z) * inv.z;
+  float tx0, tx1, ty0, ty1, tz0, tz1;
+  if (FAST) {
+    tx0 = fmaf(lox, inv.x, o.x); tx1 = fmaf(hix, inv.x, o.x);
+    ty0 = fmaf(loy, inv.y, o.y); ty1 = fmaf(hiy, inv.y, o.y);
+    tz0 = fmaf(loz, inv.z, o.z); tz1 = fmaf(hiz, inv.z, o.z);
+  } else {
+    tx0 = (lox - o.x) * inv.x; tx1 = (hix - o.x) * inv.x;
+    ty0 = (loy - o.y) * inv.y; ty1 = (hiy - o.y) * inv.y;
+    tz0 = (loz - o.z) * inv.z; tz1 = (hiz - o.z) * inv.z;
+  }
   float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), tmin));
   float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tmax));
   return tn <= tf * ROBUST_FAR ? tn : INFINITY;
@@ -134,84 +165,92 @@ DG_INLINE float box_enter(float lox, float hix, float loy, float hiy, float loz,
 
 // Traversal state of one ray.
 struct TravState {
-  V3 o, inv;
+  V3 o, inv;        // origin (FAST: -origin * inv), reciprocal direction
   WtRay w;
   float tmin, tmax;
   uint32_t ignore;  // leaf-order triangle to skip
   uint32_t node;    // next node to visit
+  uint32_t visits;  // steps taken
   TraceHit h;
 };
 
+template <bool FAST>
 DG_INLINE void trav_begin(TravState& s, const TraceRay& r, LaneStack& st) {
-  s.o = r.o;
   s.inv = v3(robust_inv(r.d.x), robust_inv(r.d.y), robust_inv(r.d.z));
+  s.o = FAST ? v3(-r.o.x * s.inv.x, -r.o.y * s.inv.y, -r.o.z * s.inv.z) : r.o;
   s.w = wt_setup(r.o, r.d);
   s.tmin = r.tmin;
   s.tmax = r.tmax;
   s.ignore = r.ignore;
   s.node = 0;
+  s.visits = 0;
   s.h.t = INFINITY; s.h.b1 = 0.f; s.h.b2 = 0.f; s.h.tri = 0xFFFFFFFFu;
   st.sp = 0;
 }
 
-// Tests the triangles of a leaf; returns true if (ANY) a hit was found.
+// Tests the first triangle of leaf `code`; returns true if (ANY) a hit was found.
+// A leaf is walked one triangle per traversal step (see trav_step).
 template <bool ANY>
 DG_INLINE bool leaf_test(const float4* __restrict__ tris, uint32_t code, TravState& s) {
-  uint32_t first = leaf_first(code), cnt = leaf_count(code);
-  for (uint32_t k = 0; k < cnt; ++k) {
-    uint32_t ti = first + k;
-    if (ti == s.ignore) continue;
-    float4 va = tris[3 * ti], vb = tris[3 * ti + 1], vc = tris[3 * ti + 2];
-    float t, b1, b2;
-    if (wt_hit(s.w, va, vb, vc, s.tmin, s.tmax, t, b1, b2)) {
-      s.h.t = t; s.h.b1 = b1; s.h.b2 = b2; s.h.tri = ti;
-      if (ANY) return true;
-      s.tmax = t;
-    }
-  }
-  return false;
+  uint32_t ti = leaf_first(code);
+  if (ti == s.ignore) return false;
+  float4 va = tris[3 * ti], vb = tris[3 * ti + 1], vc = tris[3 * ti + 2];
+  float t, b1, b2;
+  if (!wt_hit(s.w, va, vb, vc, s.tmin, s.tmax, t, b1, b2)) return false;
+  s.h.t = t; s.h.b1 = b1; s.h.b2 = b2; s.h.tri = ti;
+  if (!ANY) s.tmax = t;
+  return ANY;
 }
 
 constexpr uint32_t TRAV_END = 0xFFFFFFFFu;  // no node left (the builder keeps leaf codes below it)
 
 DG_INLINE void trav_pop(TravState& s, LaneStack& st) { s.node = st.sp ? st.pop() : TRAV_END; }
 
+// Orders (distance, child) pairs a and b by distance with selects, no branches.
+DG_INLINE void order_pair(float& da, uint32_t& ca, float& db, uint32_t& cb) {
+  bool sw = db < da;
+  float d0 = sw ? db : da, d1 = sw ? da : db;
+  uint32_t c0 = sw ? cb : ca, c1 = sw ? ca : cb;
+  da = d0; db = d1; ca = c0; cb = c1;
+}
+
 // Visits inner node s.node.  BVH2: both child boxes in the parent node (64 B);
 // BVH4: 128 B nodes.  s.node becomes the nearest child hit (the others pushed far
-// to near), or the next stacked node, or TRAV_END.
-template <int WIDTH>
+// to near), or the next stacked node, or TRAV_END.  Any-hit rays (ANY) stop at the
+// first hit anywhere, so only the nearest child is picked (a three-comparison
+// tournament) and the other hit children are pushed unsorted.
+template <int WIDTH, bool ANY, bool FAST>
 DG_INLINE void trav_inner(const float4* __restrict__ nodes, TravState& s, LaneStack& st) {
   if (WIDTH == 4) {
     const float4* n = nodes + 8 * (size_t)s.node;
     float4 lx = n[0], hx = n[1], ly = n[2], hy = n[3], lz = n[4], hz = n[5], cc = n[6];
-    float td[4];
-    uint32_t ch[4];
-    td[0] = box_enter(lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, s.o, s.inv, s.tmin, s.tmax);
-    td[1] = box_enter(lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, s.o, s.inv, s.tmin, s.tmax);
-    td[2] = box_enter(lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, s.o, s.inv, s.tmin, s.tmax);
-    td[3] = box_enter(lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, s.o, s.inv, s.tmin, s.tmax);
-    ch[0] = __float_as_uint(cc.x); ch[1] = __float_as_uint(cc.y);
-    ch[2] = __float_as_uint(cc.z); ch[3] = __float_as_uint(cc.w);
-    // sort the four (distance, child) pairs, misses (inf) last: 5-comparator network
-#define DG_CSWAP(a, b)                                                      \
-    if (td[b] < td[a]) {                                                    \
-      float tt = td[a]; td[a] = td[b]; td[b] = tt;                          \
-      uint32_t cx = ch[a]; ch[a] = ch[b]; ch[b] = cx;                       \
+    float d0 = box_enter<FAST>(lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, s.o, s.inv, s.tmin, s.tmax);
+    float d1 = box_enter<FAST>(lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, s.o, s.inv, s.tmin, s.tmax);
+    float d2 = box_enter<FAST>(lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, s.o, s.inv, s.tmin, s.tmax);
+    float d3 = box_enter<FAST>(lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, s.o, s.inv, s.tmin, s.tmax);
+    uint32_t c0 = __float_as_uint(cc.x), c1 = __float_as_uint(cc.y);
+    uint32_t c2 = __float_as_uint(cc.z), c3 = __float_as_uint(cc.w);
+    // (0,1) (2,3) then the two winners: the nearest is in slot 0
+    order_pair(d0, c0, d1, c1);
+    order_pair(d2, c2, d3, c3);
+    order_pair(d0, c0, d2, c2);
+    if (!ANY) {
+      // complete the 5-comparator network: slots 1..3 far to near after (1,3), (1,2)
+      order_pair(d1, c1, d3, c3);
+      order_pair(d1, c1, d2, c2);
     }
-    DG_CSWAP(0, 1) DG_CSWAP(2, 3) DG_CSWAP(0, 2) DG_CSWAP(1, 3) DG_CSWAP(1, 2)
-#undef DG_CSWAP
-    if (td[0] != INFINITY) {
-      if (td[3] != INFINITY) st.push(ch[3]);
-      if (td[2] != INFINITY) st.push(ch[2]);
-      if (td[1] != INFINITY) st.push(ch[1]);
-      s.node = ch[0];
+    if (d0 != INFINITY) {
+      if (d3 != INFINITY) st.push(c3);
+      if (d2 != INFINITY) st.push(c2);
+      if (d1 != INFINITY) st.push(c1);
+      s.node = c0;
       return;
     }
   } else {
     const float4* n = nodes + 4 * (size_t)s.node;
     float4 n0 = n[0], n1 = n[1], n2 = n[2], n3 = n[3];
-    float t0 = box_enter(n0.x, n0.y, n0.z, n0.w, n2.x, n2.y, s.o, s.inv, s.tmin, s.tmax);
-    float t1 = box_enter(n1.x, n1.y, n1.z, n1.w, n2.z, n2.w, s.o, s.inv, s.tmin, s.tmax);
+    float t0 = box_enter<FAST>(n0.x, n0.y, n0.z, n0.w, n2.x, n2.y, s.o, s.inv, s.tmin, s.tmax);
+    float t1 = box_enter<FAST>(n1.x, n1.y, n1.z, n1.w, n2.z, n2.w, s.o, s.inv, s.tmin, s.tmax);
     uint32_t c0 = __float_as_uint(n3.x), c1 = __float_as_uint(n3.y);
     bool h0 = t0 != INFINITY, h1 = t1 != INFINITY;
     if (h0 && h1) {
@@ -228,25 +267,39 @@ DG_INLINE void trav_inner(const float4* __restrict__ nodes, TravState& s, LaneSt
   trav_pop(s, st);
 }
 
+// One traversal step of one ray: visits the inner node in s.node, or tests ONE
+// triangle of the leaf in s.node and leaves the rest of the leaf (first + 1,
+// count - 1) in s.node.  Lanes of a wave diverge between inner nodes and leaves in
+// nearly every step (each lane is at a leaf in about one step of eight, and there are
+// 64 lanes), so a step costs an inner-node visit plus one triangle test instead of
+// an inner-node visit plus the longest leaf of the wave.
+// s.node becomes TRAV_END when the ray is finished (stack empty, or any-hit found),
+// or when it exceeds MAX_VISITS steps.
+template <bool ANY, int WIDTH, bool FAST>
+DG_INLINE void trav_step(const float4* __restrict__ nodes, const float4* __restrict__ tris, TravState& s,
+                         LaneStack& st) {
+  if (++s.visits > MAX_VISITS) {
+    s.node = TRAV_END;
+  } else if (!(s.node & LEAF_BIT)) {
+    trav_inner<WIDTH, ANY, FAST>(nodes, s, st);
+  } else if (leaf_test<ANY>(tris, s.node, s)) {
+    s.node = TRAV_END;
+  } else if (leaf_count(s.node) > 1u) {
+    s.node += 1u - (1u << 27);  // next triangle: first + 1, count - 1
+  } else {
+    trav_pop(s, st);
+  }
+}
+
 // Whole traversal of one ray; returns true on a hit (closest) / any hit (ANY).
-// One loop, one node (inner or leaf) per iteration.  Measured on MI355X against the
-// "while-while" loop and speculative traversal of Aila & Laine 2009 ("Understanding
-// the efficiency of ray traversal on GPUs"), and against replacing finished rays
-// per lane: this plain loop was the fastest of the four on the terrain workload.
-template <bool ANY, int WIDTH>
+// One loop, one step (inner node or one leaf triangle) per iteration.  FAST: the ray
+// starts inside the range the box padding covers (see box_enter).
+template <bool ANY, int WIDTH, bool FAST>
 DG_INLINE bool traverse(const float4* __restrict__ nodes, const float4* __restrict__ tris,
                         const TraceRay& r, LaneStack& st, TraceHit& h) {
   TravState s;
-  trav_begin(s, r, st);
-  uint32_t visits = 0;
-  while (s.node != TRAV_END && visits++ < MAX_VISITS) {
-    if (!(s.node & LEAF_BIT)) {
-      trav_inner<WIDTH>(nodes, s, st);
-    } else {
-      if (leaf_test<ANY>(tris, s.node, s)) break;
-      trav_pop(s, st);
-    }
-  }
+  trav_begin<FAST>(s, r, st);
+  while (s.node != TRAV_END) trav_step<ANY, WIDTH, FAST>(nodes, tris, s, st);
   h = s.h;
   return h.tri != 0xFFFFFFFFu;
 }

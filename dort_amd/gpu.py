@@ -156,12 +156,12 @@ class DeviceArray:
 class Scene:
     """A scene uploaded to the GPU (BVH built on the host, flattened to the GPU layout)."""
 
-    def __init__(self, ctx, scene, width=4, max_leaf_size=4, sah_bins=16, stack_lds=32,
-                 block_size=128, queue_tile=0):
+    def __init__(self, ctx, scene, width=4, max_leaf_size=4, sah_bins=16, stack_lds=16,
+                 block_size=128, queue_tile=0, sah_node_cost=0.0):
         self.ctx = ctx
         self.packed = scene.pack()
         self.scene = scene
-        self.opts = abi.BvhOpts(width, max_leaf_size, sah_bins, stack_lds, block_size, queue_tile)
+        self.opts = abi.BvhOpts(width, max_leaf_size, sah_bins, stack_lds, block_size, queue_tile, sah_node_cost)
         self.h = C.c_void_p()
         check(lib().dgpu_scene_create(ctx.h, C.byref(self.packed.desc), C.byref(self.opts),
                                       C.byref(self.h)))
@@ -179,8 +179,9 @@ class Scene:
         check(lib().dgpu_scene_get_info(self.h, C.byref(i)))
         return {f: getattr(i, f) for f, _ in abi.SceneInfo._fields_}
 
-    def trace_closest(self, rays, out=None, device=False):
-        """rays: dict of 8 SoA arrays (numpy on host or torch tensors on the GPU)."""
+    def trace_closest(self, rays, out=None, device=False, bounded=False):
+        """rays: dict of 8 SoA arrays (numpy on host or torch tensors on the GPU).
+        bounded: the origins lie within the scene's coordinate range (DGPU_TRACE_BOUNDED)."""
         n = len(rays["ox"])
         r = abi.rays_struct(rays)
         if out is None:
@@ -189,20 +190,20 @@ class Scene:
         ptr = lambda a: a.data_ptr() if hasattr(a, "data_ptr") else a.ctypes.data
         h = abi.Hits(ptr(out["t"]), ptr(out["prim"]), ptr(out["b1"]), ptr(out["b2"]))
         ms = C.c_double()
-        check(lib().dgpu_trace_closest(self.h, C.byref(r), n, C.byref(h),
-                                       abi.BUF_DEVICE if device else 0, C.byref(ms)))
+        flags = (abi.BUF_DEVICE if device else 0) | (abi.TRACE_BOUNDED if bounded else 0)
+        check(lib().dgpu_trace_closest(self.h, C.byref(r), n, C.byref(h), flags, C.byref(ms)))
         out["kernel_ms"] = ms.value
         return out
 
-    def trace_any(self, rays, out=None, device=False):
+    def trace_any(self, rays, out=None, device=False, bounded=False):
         n = len(rays["ox"])
         r = abi.rays_struct(rays)
         if out is None:
             out = np.zeros(n, np.uint32)
         ptr = out.data_ptr() if hasattr(out, "data_ptr") else out.ctypes.data
         ms = C.c_double()
-        check(lib().dgpu_trace_any(self.h, C.byref(r), n, ptr, abi.BUF_DEVICE if device else 0,
-                                   C.byref(ms)))
+        flags = (abi.BUF_DEVICE if device else 0) | (abi.TRACE_BOUNDED if bounded else 0)
+        check(lib().dgpu_trace_any(self.h, C.byref(r), n, ptr, flags, C.byref(ms)))
         return out, ms.value
 
     def render(self, opts, film_rgb=None, film_w=None, device=False, cancel=None):

@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define DGPU_ABI_VERSION 1u
+#define DGPU_ABI_VERSION 2u
 
 typedef enum {
   DGPU_OK = 0,
@@ -116,9 +116,10 @@ typedef struct {
   uint32_t width;          /* 2 or 4 (default 4) */
   uint32_t max_leaf_size;  /* triangles per leaf, 1..8 (default 4) */
   uint32_t sah_bins;       /* binned-SAH bins (default 16) */
-  uint32_t stack_lds;      /* traversal stack entries kept in LDS per lane (default 32) */
+  uint32_t stack_lds;      /* traversal stack entries kept in LDS per lane (default 16) */
   uint32_t block_size;     /* threads per traversal workgroup: 64,128,256 (default 128) */
-  uint32_t queue_tile;     /* paths per wavefront chunk (default: sized from film) */
+  uint32_t queue_tile;     /* paths per wavefront chunk (default: sized from free memory) */
+  float sah_node_cost;     /* SAH cost of one node visit in triangle tests (default 1) */
 } dgpu_bvh_opts;
 
 /* ---- rays and hits (SoA, trace_* calls) -------------------------------- */
@@ -139,7 +140,10 @@ typedef struct {
 enum {
   DGPU_BUF_DEVICE = 1u,     /* buffers are device pointers */
   DGPU_RENDER_PROFILE = 2u, /* dgpu_render: time every kernel launch with HIP events */
-  DGPU_BATCH_NO_GRAPH = 4u  /* dgpu_render_batch: launch kernels directly, no hipGraph */
+  DGPU_BATCH_NO_GRAPH = 4u, /* dgpu_render_batch: launch kernels directly, no hipGraph */
+  DGPU_TRACE_BOUNDED = 8u   /* dgpu_trace_*: every ray origin lies within the scene's range (no
+                               coordinate beyond max |vertex or camera coordinate|), which
+                               allows the path tracer's faster box test */
 };
 
 /* ---- rendering ----------------------------------------------------------- */
@@ -211,7 +215,8 @@ dgpu_status dgpu_scene_get_info(const dgpu_scene* scene, dgpu_scene_info* out);
 
 /* ---- intersection (Scene::intersect / Scene::intersect_p, scene.cpp:8-18) - */
 
-/* Closest hit per ray; watertight FP32 triangle test, two-sided. */
+/* Closest hit per ray; watertight FP32 triangle test, two-sided.  flags: DGPU_BUF_DEVICE,
+ * DGPU_TRACE_BOUNDED (same results, faster; only for origins within the scene's range). */
 dgpu_status dgpu_trace_closest(dgpu_scene* scene, const dgpu_rays* rays, uint32_t n,
                                dgpu_hits* out, uint32_t flags, double* kernel_ms);
 /* Any hit per ray in [tmin, tmax]; occluded[i] = 1 if some triangle is hit. */

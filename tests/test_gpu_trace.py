@@ -68,9 +68,11 @@ def test_brute_force_small(gpu_ctx):
     assert c["agree"] >= 0.99 * c["n"]
 
 
-def test_watertight_closed_mesh(gpu_ctx):
+@pytest.mark.parametrize("bounded", [False, True])
+def test_watertight_closed_mesh(gpu_ctx, bounded):
     """Rays from inside a closed star-shaped mesh aimed at vertices and edge midpoints
-    must all hit (no leaks through shared edges)."""
+    must all hit (no leaks through shared edges), with the exact box test and with the
+    path tracer's padded FMA box test (bounded)."""
     p, t = scenes.displaced_sphere(16, seed=3, amp=0.05)
     b = scenes._Builder()
     m = b.material(scenes.lambert((0.5, 0.5, 0.5)))
@@ -85,10 +87,54 @@ def test_watertight_closed_mesh(gpu_ctx):
     d = tg - o
     d /= np.linalg.norm(d, axis=1, keepdims=True)
     rays = scenes.ray_batch(o.astype(np.float32), d.astype(np.float32))
-    g = gpu_ctx.scene(sc).trace_closest(rays)
+    g = gpu_ctx.scene(sc).trace_closest(rays, bounded=bounded)
     assert (g["prim"] >= 0).all(), int((g["prim"] < 0).sum())
-    occ, _ = gpu_ctx.scene(sc).trace_any(rays)
+    occ, _ = gpu_ctx.scene(sc).trace_any(rays, bounded=bounded)
     assert occ.all()
+
+
+def _bounded_rays(scene, n, seed):
+    """Rays that start inside the scene's coordinate range: half from points inside the
+    bounding box, half from surface points (hit points of random rays, tmin = 1e-3 t,
+    like the path tracer's secondary rays), with directions uniform on the sphere."""
+    rng = np.random.default_rng(seed)
+    p = scene.positions.reshape(-1, 3)
+    lo, hi = p.min(0), p.max(0)
+    o = rng.uniform(lo, hi, size=(n, 3)).astype(np.float32)
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = scenes.ray_batch(o, d.astype(np.float32))
+    ref = oracle.OracleScene(scene).trace(rays)
+    hit = np.isfinite(ref["t"])
+    o2 = (o + d * ref["t"][:, None]).astype(np.float32)[hit]
+    d2 = rng.normal(size=(len(o2), 3))
+    d2 /= np.linalg.norm(d2, axis=1, keepdims=True)
+    r2 = scenes.ray_batch(o2, d2.astype(np.float32))
+    r2["tmin"] = (1e-3 * ref["t"][hit]).astype(np.float32)
+    return {k: np.concatenate([rays[k], r2[k]]) for k in rays}
+
+
+@pytest.mark.parametrize("width", [2, 4])
+def test_bounded_box_test_parity(gpu_ctx, sphere, width):
+    """The padded FMA box test (DGPU_TRACE_BOUNDED, the path tracer's) against the oracle
+    and against the exact box test on rays that start inside the scene's range."""
+    rays = _bounded_rays(sphere, 32768, seed=21)
+    ref = oracle.OracleScene(sphere).trace(rays)
+    gsc = gpu_ctx.scene(sphere, width=width)
+    fast = gsc.trace_closest(rays, bounded=True)
+    exact = gsc.trace_closest(rays)
+    c = classify_hits(ref, fast)
+    # (hit distances are compared with the exact path below: rays from surface points
+    # include grazing hits, where dort's Moller-Trumbore t and the watertight t differ
+    # by more than 1e-5 relative without either being wrong)
+    assert c["agree"] >= 0.9999 * c["n"] and c["fail"] == 0, c
+    # the same triangle tests run either way: only ties may resolve differently
+    same = fast["prim"] == exact["prim"]
+    assert same.mean() >= 0.9999
+    assert np.array_equal(fast["t"][same], exact["t"][same])
+    occ_f, _ = gsc.trace_any(rays, bounded=True)
+    occ_e, _ = gsc.trace_any(rays)
+    assert np.array_equal(occ_f, occ_e)
 
 
 def test_device_buffers(gpu_ctx, sphere):
