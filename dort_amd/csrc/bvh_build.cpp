@@ -308,26 +308,73 @@ BuiltBvh build_bvh(const float* positions, uint32_t num_vertices, const uint32_t
       for (int i = (int)kids.size() - 1; i >= 0; --i)
         if (nodes[kids[i]].left >= 0) st.push_back(kids[i]);
     }
-    out.nodes.assign(32 * wide.size(), 0.f);
+    // Compressed 64-byte nodes: the child boxes are quantized to 8 bits per plane in
+    // the frame of the node's own box (Ylitie, Karras, Laine 2017, "Efficient incoherent
+    // ray traversal on GPUs through compressed wide BVHs"), rounded outward, so that a
+    // visit loads 4 x 16 B instead of 7:
+    //   [0]  origin x, y, z (float), exponents ex | ey << 8 | ez << 16 (int8 each)
+    //   [1]  lo.x[4] hi.x[4] lo.y[4] hi.y[4]   (one byte per child, 4 per word)
+    //   [2]  lo.z[4] hi.z[4] child[0] child[1]
+    //   [3]  child[2] child[3] 0 0
+    // A plane is origin + q * 2^e.  The boxes are grown by box_pad before quantizing.
+    out.nodes.assign(16 * wide.size(), 0.f);
     uint32_t leaves = 0;
     for (size_t i = 0; i < wide.size(); ++i) {
-      float* o = out.nodes.data() + 32 * i;
-      for (int s = 0; s < 4; ++s) {
-        // empty slot: a box at +infinity, which the slab test never enters
-        uint32_t code = 0x7FFFFFFFu;
-        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {INFINITY, INFINITY, INFINITY};
-        if (s < (int)wide[i].kids.size()) {
-          int32_t k = wide[i].kids[s];
-          const BNode& n = nodes[k];
-          for (int a = 0; a < 3; ++a) { lo[a] = n.box.lo[a] - prm.box_pad; hi[a] = n.box.hi[a] + prm.box_pad; }
-          if (n.left < 0) { code = leaf_code(n.first, n.count); ++leaves; }
-          else code = (uint32_t)wide_of[k];
+      float* o = out.nodes.data() + 16 * i;
+      uint32_t* w = reinterpret_cast<uint32_t*>(o);
+      const size_t nk = wide[i].kids.size();
+      double lo[4][3], hi[4][3], nlo[3] = {INFINITY, INFINITY, INFINITY}, nhi[3] = {-INFINITY, -INFINITY, -INFINITY};
+      for (size_t c = 0; c < nk; ++c) {
+        const BNode& n = nodes[wide[i].kids[c]];
+        for (int a = 0; a < 3; ++a) {
+          // padded child box, rounded outward to float
+          lo[c][a] = (double)std::nextafter(n.box.lo[a] - prm.box_pad, -INFINITY);
+          hi[c][a] = (double)std::nextafter(n.box.hi[a] + prm.box_pad, INFINITY);
+          nlo[a] = std::min(nlo[a], lo[c][a]);
+          nhi[a] = std::max(nhi[a], hi[c][a]);
         }
-        o[0 + s] = lo[0]; o[4 + s] = hi[0];
-        o[8 + s] = lo[1]; o[12 + s] = hi[1];
-        o[16 + s] = lo[2]; o[20 + s] = hi[2];
-        std::memcpy(&o[24 + s], &code, 4);
       }
+      uint32_t exps = 0;
+      int ex[3];
+      for (int a = 0; a < 3; ++a) {
+        // smallest e with 255 * 2^e >= extent, in [-100, 100]
+        double ext = nhi[a] - nlo[a];
+        int e = -100;
+        while (e < 100 && std::ldexp(255.0, e) < ext) ++e;
+        ex[a] = e;
+        o[a] = (float)nlo[a];  // a float already: nlo is a min of floats
+        exps |= (uint32_t)(uint8_t)(int8_t)e << (8 * a);
+      }
+      w[3] = exps;
+      uint32_t q[6] = {0, 0, 0, 0, 0, 0};  // lo.x hi.x lo.y hi.y lo.z hi.z, a byte per child
+      uint32_t code[4] = {EMPTY_CHILD_CODE, EMPTY_CHILD_CODE, EMPTY_CHILD_CODE, EMPTY_CHILD_CODE};
+      for (size_t c = 0; c < 4; ++c) {
+        for (int a = 0; a < 3; ++a) {
+          uint32_t ql = 255, qh = 0;  // empty slot: inverted box (and the child code says empty)
+          if (c < nk) {
+            // (child - origin) * 2^-e is exact in double unless the subtraction rounds;
+            // then one more quantum outward keeps the quantized box around the child
+            double sc = std::ldexp(1.0, -ex[a]);
+            double dl = lo[c][a] - nlo[a], dh = hi[c][a] - nlo[a];
+            double fl = std::floor(dl * sc), ce = std::ceil(dh * sc);
+            if (dl + nlo[a] != lo[c][a]) fl -= 1.0;
+            if (dh + nlo[a] != hi[c][a]) ce += 1.0;
+            ql = (uint32_t)std::min(std::max(fl, 0.0), 255.0);
+            qh = (uint32_t)std::min(std::max(ce, 0.0), 255.0);
+          }
+          q[2 * a] |= ql << (8 * c);
+          q[2 * a + 1] |= qh << (8 * c);
+        }
+        if (c < nk) {
+          int32_t k = wide[i].kids[c];
+          const BNode& n = nodes[k];
+          if (n.left < 0) { code[c] = leaf_code(n.first, n.count); ++leaves; }
+          else code[c] = (uint32_t)wide_of[k];
+        }
+      }
+      w[4] = q[0]; w[5] = q[1]; w[6] = q[2]; w[7] = q[3];
+      w[8] = q[4]; w[9] = q[5]; w[10] = code[0]; w[11] = code[1];
+      w[12] = code[2]; w[13] = code[3]; w[14] = 0; w[15] = 0;
     }
     out.num_nodes = (uint32_t)wide.size();
     out.num_leaves = leaves;

@@ -14,7 +14,7 @@ struct BvhParams {
 };
 
 struct BuiltBvh {
-  std::vector<float> nodes;          // 16 floats per BVH2 node, 32 per BVH4 node
+  std::vector<float> nodes;          // 16 floats (64 B) per node: BVH2, or compressed BVH4
   std::vector<float> tris;           // 12 floats per triangle, leaf order
   std::vector<uint32_t> tri_leaf_of; // scene triangle id -> leaf-order slot
   uint32_t width = 2;
@@ -22,6 +22,9 @@ struct BuiltBvh {
   uint32_t num_nodes = 0;
   uint32_t num_leaves = 0;
 };
+
+// Child code of an empty slot of a 4-wide node.
+constexpr uint32_t EMPTY_CHILD_CODE = 0x7FFFFFFFu;
 
 // Leaf child code: bit 31 set, bits 27..30 = count-1, bits 0..26 = first triangle.
 inline uint32_t leaf_code(uint32_t first, uint32_t count) {

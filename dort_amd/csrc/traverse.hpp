@@ -138,6 +138,15 @@ DG_INLINE float robust_inv(float d) {
   return 1.0f / (fabsf(d) > ooeps ? d : copysignf(ooeps, d));
 }
 
+// Entry distance of the slab intervals [t?0, t?1] (either order) clipped to [tmin, tmax],
+// or +inf when empty; the far distance is scaled by 1 + 2 gamma(3).
+DG_INLINE float slab_enter(float tx0, float tx1, float ty0, float ty1, float tz0, float tz1, float tmin,
+                           float tmax) {
+  float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), tmin));
+  float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tmax));
+  return tn <= tf * ROBUST_FAR ? tn : INFINITY;
+}
+
 // Slab test of one box against [tmin, tmax]; returns the entry distance or +inf on a miss.
 // Exact form (any ray origin): (b - o) * inv, far distance scaled by 1 + 2 gamma(3).
 // FAST form: fma(b, inv, -o * inv), one instruction per plane instead of two.  Its
@@ -158,9 +167,7 @@ DG_INLINE float box_enter(float lox, float hix, float loy, float hiy, float loz,
     ty0 = (loy - o.y) * inv.y; ty1 = (hiy - o.y) * inv.y;
     tz0 = (loz - o.z) * inv.z; tz1 = (hiz - o.z) * inv.z;
   }
-  float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), tmin));
-  float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tmax));
-  return tn <= tf * ROBUST_FAR ? tn : INFINITY;
+  return slab_enter(tx0, tx1, ty0, ty1, tz0, tz1, tmin, tmax);
 }
 
 // Traversal state of one ray.
@@ -206,6 +213,48 @@ constexpr uint32_t TRAV_END = 0xFFFFFFFFu;  // no node left (the builder keeps l
 
 DG_INLINE void trav_pop(TravState& s, LaneStack& st) { s.node = st.sp ? st.pop() : TRAV_END; }
 
+DG_INLINE float byte_f(uint32_t w, int k) { return (float)((w >> (8 * k)) & 0xFFu); }
+DG_INLINE float pow2f(uint32_t e8) {  // 2^e for the int8 e in the low byte of e8, |e| <= 100
+  return __uint_as_float((uint32_t)((int32_t)(int8_t)(e8 & 0xFFu) + 127) << 23);
+}
+
+// Entry distances of the four children of a compressed node (A: origin + exponents,
+// Q: lo.x hi.x lo.y hi.y bytes, R: lo.z hi.z bytes).  A plane is origin + q 2^e:
+//  FAST:  t = fma(q, 2^e inv, fma(origin, inv, -o inv)), two instructions per plane;
+//  exact: plane = fma(q, 2^e, origin) (one rounding, covered by the box padding), then
+//         (plane - o) * inv as in box_enter.
+template <bool FAST>
+DG_INLINE void decode_boxes(uint4 A, uint4 Q, uint4 R, const TravState& s, float& d0, float& d1, float& d2,
+                            float& d3) {
+  V3 org = v3(__uint_as_float(A.x), __uint_as_float(A.y), __uint_as_float(A.z));
+  V3 sc = v3(pow2f(A.w), pow2f(A.w >> 8), pow2f(A.w >> 16));
+  float t[4][6];
+  if (FAST) {
+    V3 si = v3(sc.x * s.inv.x, sc.y * s.inv.y, sc.z * s.inv.z);
+    V3 of = v3(fmaf(org.x, s.inv.x, s.o.x), fmaf(org.y, s.inv.y, s.o.y), fmaf(org.z, s.inv.z, s.o.z));
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      t[c][0] = fmaf(byte_f(Q.x, c), si.x, of.x); t[c][1] = fmaf(byte_f(Q.y, c), si.x, of.x);
+      t[c][2] = fmaf(byte_f(Q.z, c), si.y, of.y); t[c][3] = fmaf(byte_f(Q.w, c), si.y, of.y);
+      t[c][4] = fmaf(byte_f(R.x, c), si.z, of.z); t[c][5] = fmaf(byte_f(R.y, c), si.z, of.z);
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      t[c][0] = (fmaf(byte_f(Q.x, c), sc.x, org.x) - s.o.x) * s.inv.x;
+      t[c][1] = (fmaf(byte_f(Q.y, c), sc.x, org.x) - s.o.x) * s.inv.x;
+      t[c][2] = (fmaf(byte_f(Q.z, c), sc.y, org.y) - s.o.y) * s.inv.y;
+      t[c][3] = (fmaf(byte_f(Q.w, c), sc.y, org.y) - s.o.y) * s.inv.y;
+      t[c][4] = (fmaf(byte_f(R.x, c), sc.z, org.z) - s.o.z) * s.inv.z;
+      t[c][5] = (fmaf(byte_f(R.y, c), sc.z, org.z) - s.o.z) * s.inv.z;
+    }
+  }
+  float d[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) d[c] = slab_enter(t[c][0], t[c][1], t[c][2], t[c][3], t[c][4], t[c][5], s.tmin, s.tmax);
+  d0 = d[0]; d1 = d[1]; d2 = d[2]; d3 = d[3];
+}
+
 // Orders (distance, child) pairs a and b by distance with selects, no branches.
 DG_INLINE void order_pair(float& da, uint32_t& ca, float& db, uint32_t& cb) {
   bool sw = db < da;
@@ -222,14 +271,16 @@ DG_INLINE void order_pair(float& da, uint32_t& ca, float& db, uint32_t& cb) {
 template <int WIDTH, bool ANY, bool FAST>
 DG_INLINE void trav_inner(const float4* __restrict__ nodes, TravState& s, LaneStack& st) {
   if (WIDTH == 4) {
-    const float4* n = nodes + 8 * (size_t)s.node;
-    float4 lx = n[0], hx = n[1], ly = n[2], hy = n[3], lz = n[4], hz = n[5], cc = n[6];
-    float d0 = box_enter<FAST>(lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, s.o, s.inv, s.tmin, s.tmax);
-    float d1 = box_enter<FAST>(lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, s.o, s.inv, s.tmin, s.tmax);
-    float d2 = box_enter<FAST>(lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, s.o, s.inv, s.tmin, s.tmax);
-    float d3 = box_enter<FAST>(lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, s.o, s.inv, s.tmin, s.tmax);
-    uint32_t c0 = __float_as_uint(cc.x), c1 = __float_as_uint(cc.y);
-    uint32_t c2 = __float_as_uint(cc.z), c3 = __float_as_uint(cc.w);
+    // compressed node (64 B): origin, per-axis scale 2^e, 8-bit child planes
+    const uint4* n = reinterpret_cast<const uint4*>(nodes) + 4 * (size_t)s.node;
+    uint4 A = n[0], Q = n[1], R = n[2], K = n[3];
+    float d0, d1, d2, d3;
+    decode_boxes<FAST>(A, Q, R, s, d0, d1, d2, d3);
+    uint32_t c0 = R.z, c1 = R.w, c2 = K.x, c3 = K.y;
+    d0 = c0 == EMPTY_CHILD ? INFINITY : d0;
+    d1 = c1 == EMPTY_CHILD ? INFINITY : d1;
+    d2 = c2 == EMPTY_CHILD ? INFINITY : d2;
+    d3 = c3 == EMPTY_CHILD ? INFINITY : d3;
     // (0,1) (2,3) then the two winners: the nearest is in slot 0
     order_pair(d0, c0, d1, c1);
     order_pair(d2, c2, d3, c3);
