@@ -16,8 +16,8 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libdortgpu.so")
 ARCH = os.environ.get("DGPU_ARCH", "gfx950")
 
-HIP_SOURCES = ["kernels.hip", "api.hip"]
-CXX_SOURCES = ["bvh_build.cpp"]
+HIP_SOURCES = ["kernels.hip", "api.hip", "compare.hip"]
+CXX_SOURCES = ["bvh_build.cpp", "io.cpp"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-Wno-unused-variable"]
 HIP_FLAGS = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -19,14 +19,15 @@
 
 #include "bvh_build.hpp"
 #include "dgpu_internal.hpp"
+#include "status.hpp"
 
 using namespace dgpu;
 
 namespace {
-
 thread_local std::string g_err;
+}  // namespace
 
-dgpu_status fail(dgpu_status st, const char* fmt, ...) {
+dgpu_status dgpu::fail(dgpu_status st, const char* fmt, ...) {
   char buf[512];
   va_list ap;
   va_start(ap, fmt);
@@ -35,6 +36,8 @@ dgpu_status fail(dgpu_status st, const char* fmt, ...) {
   g_err = buf;
   return st;
 }
+
+namespace {
 
 #define HIP_TRY(x)                                                                   \
   do {                                                                               \
@@ -111,7 +114,7 @@ struct dgpu_ctx {
     int& cached = batch ? shade_blocks_batch : (all ? shade_blocks_all : shade_blocks_single);
     if (cached == 0) {
       int nb = 0;
-      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, shade_kernel_ptr(batch, all), 256, 0) != hipSuccess ||
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, shade_kernel_ptr(batch, all), SHADE_BLOCK, 0) != hipSuccess ||
           nb <= 0)
         nb = 2;
       cached = nb * num_cus * 4;
@@ -133,6 +136,11 @@ struct dgpu_ctx {
     return g;
   }
 };
+
+std::mutex& dgpu::ctx_mutex(dgpu_ctx* ctx) { return ctx->mu; }
+int dgpu::ctx_device(dgpu_ctx* ctx) { return ctx->device; }
+hipStream_t dgpu::ctx_stream(dgpu_ctx* ctx) { return ctx->stream; }
+
 
 struct dgpu_scene {
   dgpu_ctx* ctx = nullptr;

@@ -1,5 +1,7 @@
 // dgpu_internal.hpp -- launch interface between the C-ABI (api.hip) and the kernels.
 #pragma once
+#include <mutex>
+
 #include "dgpu_device.hpp"
 
 namespace dgpu {
@@ -12,6 +14,14 @@ constexpr uint32_t CNT_FETCH_EXT = 128;  // persistent-kernel work counters
 constexpr uint32_t CNT_FETCH_SHD = 192;
 constexpr uint32_t NUM_COUNTERS = 256;
 constexpr uint32_t MAX_SLOT_LIGHTS = 8;  // sample_all_lights supports up to this many lights
+#ifndef DGPU_SHADE_BLOCK
+#define DGPU_SHADE_BLOCK 256
+#endif
+constexpr uint32_t SHADE_BLOCK = DGPU_SHADE_BLOCK;  // threads per shade workgroup (one queue atomic each)
+#ifndef DGPU_TRACE_BATCH
+#define DGPU_TRACE_BATCH 4
+#endif
+constexpr uint32_t TRACE_BATCH = DGPU_TRACE_BATCH;  // 64-ray batches a trace wave takes per atomic
 
 constexpr uint32_t MAX_BATCH = 255;      // scenes per batch (scene index in the top byte)
 constexpr uint32_t MAX_CHUNK_SEGS = 64;  // scenes overlapping one chunk of a batch
@@ -111,5 +121,10 @@ hipError_t launch_connect(const WaveParams& P, const KernelCfg& cfg, uint32_t bo
 hipError_t launch_finalize(const WaveParams& P, uint32_t capacity, bool batch, hipStream_t st);
 hipError_t launch_advance(uint32_t* cur, hipStream_t st);
 hipError_t launch_adjoint(const WaveParams& P, const AdjArgs& A, uint32_t width, hipStream_t st);
+
+// A context's lock, device and stream, for entry points outside api.hip.
+std::mutex& ctx_mutex(dgpu_ctx* ctx);
+int ctx_device(dgpu_ctx* ctx);
+hipStream_t ctx_stream(dgpu_ctx* ctx);
 
 }  // namespace dgpu

@@ -28,17 +28,27 @@ namespace dgpu {
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
-// Appends `pred` lanes to a queue: one atomic per wave, ranks by ballot + mbcnt.
-__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred) {
+// Appends `pred` lanes of a whole workgroup of SHADE_BLOCK threads with one atomic: the
+// waves' counts go through LDS, thread 0 takes their exclusive prefix and adds the
+// total.  Every thread of the workgroup must make the same sequence of calls.  Queue
+// counters are single addresses that every wave of the chip appends to, and atomics on
+// one address serialise in L2: one atomic per wave capped the shade kernel at ~90M
+// appends/s per counter (measured on MI355X); per workgroup the cap moves out by the
+// number of waves in the workgroup.
+__device__ __forceinline__ uint32_t block_append(uint32_t* counter, bool pred, uint32_t* sh) {
+  constexpr uint32_t WAVES = SHADE_BLOCK / 64;
   unsigned long long mask = __ballot(pred);
-  if (mask == 0ull) return 0u;
-  uint32_t lane = lane_id();
-  uint32_t leader = (uint32_t)__ffsll((long long)mask) - 1u;
-  uint32_t base = 0u;
-  if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
-  base = __shfl(base, (int)leader);
+  uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
+  if (lane == 0) sh[wave] = (uint32_t)__popcll(mask);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t tot = 0;
+    for (uint32_t w = 0; w < WAVES; ++w) { uint32_t c = sh[w]; sh[w] = tot; tot += c; }
+    sh[WAVES] = tot ? atomicAdd(counter, tot) : 0u;
+  }
+  __syncthreads();
   unsigned long long below = lane == 0 ? 0ull : (mask & ((1ull << lane) - 1ull));
-  return base + (uint32_t)__popcll(below);
+  return sh[WAVES] + sh[wave] + (uint32_t)__popcll(below);
 }
 
 // ----------------------------------------------------------------- trace kernels
@@ -56,7 +66,8 @@ __device__ __forceinline__ bool trace_one(BvhRef b, const TraceRay& r, LaneStack
 }
 
 // Shared persistent loop: `io.load(i, ray)` returns the ray's BVH, `io.store(i, hit, found)`.
-// Each wave takes 64 consecutive queue entries per atomic and traces them to the end.
+// Each wave takes TRACE_BATCH x 64 consecutive queue entries per atomic (the fetch counter
+// is one address for the whole chip) and traces them 64 at a time.
 // Consecutive entries are neighbouring pixels (queues are compacted in order), so the
 // lanes of a wave fetch largely the same nodes and triangles.  Refilling finished
 // lanes from the queue as they drain (Aila & Laine 2009, dynamic fetch) was measured
@@ -78,16 +89,18 @@ __device__ __forceinline__ void trace_loop(IO& io, uint32_t n, uint32_t* fetch, 
   st.cap = stack_cap;
   for (;;) {
     uint32_t base = 0;
-    if (lane_id() == 0) base = atomicAdd(fetch, 64u);
+    if (lane_id() == 0) base = atomicAdd(fetch, 64u * TRACE_BATCH);
     base = __shfl(base, 0);
     if (base >= n) break;
-    uint32_t i = base + lane_id();
-    if (i < n) {
-      TraceRay r;
-      BvhRef bvh = io.load(i, r);
-      TraceHit h;
-      bool found = trace_one<ANY, WIDTH, FAST>(bvh, r, st, h);
-      io.store(i, h, found);
+    for (uint32_t j = 0; j < TRACE_BATCH; ++j) {
+      uint32_t i = base + 64u * j + lane_id();
+      if (i < n) {
+        TraceRay r;
+        BvhRef bvh = io.load(i, r);
+        TraceHit h;
+        bool found = trace_one<ANY, WIDTH, FAST>(bvh, r, st, h);
+        io.store(i, h, found);
+      }
     }
   }
 }
@@ -410,10 +423,12 @@ __device__ __forceinline__ void emit_shadow(const WaveParams& P, const NeeOut& s
 // One bounce of PathRenderer::sample for every path in the current queue.
 // Every lane of a wave runs the same number of queue appends (inactive lanes
 // append nothing), so the ballot-based compaction sees the whole wave.
+// sh: the workgroup's LDS words for block_append.
 template <bool BATCH, bool ALL>
 __device__ __forceinline__ void shade_body(const WaveParams& P, const DevScene& S, uint32_t bounce,
                                            bool active, uint32_t i, uint32_t p, uint32_t scene,
-                                           uint32_t pixel, uint32_t sample, uint32_t k0, uint32_t k1) {
+                                           uint32_t pixel, uint32_t sample, uint32_t k0, uint32_t k1,
+                                           uint32_t* sh) {
   const uint32_t cur = bounce & 1u, nxt = cur ^ 1u;
   const uint32_t SL = P.slots;
   uint32_t flags = 0, bounces = 0;
@@ -494,7 +509,7 @@ __device__ __forceinline__ void shade_body(const WaveParams& P, const DevScene& 
       NeeOut nb;
       nb.emit = false;
       if (nee && use_bsdf) nb = nee_from_bsdf(S, g, wo, bsdf, L, scale, thr, use_light, rb0[1], rb0[2], rb0[3]);
-      uint32_t qb = wave_append(&P.counters[CNT_SHADOW + bounce], nb.emit);
+      uint32_t qb = block_append(&P.counters[CNT_SHADOW + bounce], nb.emit, sh);
       if (nb.emit) {
         P.pend[slot_b] = make_float4(nb.contrib.x, nb.contrib.y, nb.contrib.z, 1.f);
         emit_shadow(P, nb, slot_b | scene_bits, qb);
@@ -507,7 +522,7 @@ __device__ __forceinline__ void shade_body(const WaveParams& P, const DevScene& 
         dgpu_rng_block(k0, k1, pixel, sample, bounce, 1, rb1);
         nl = nee_from_light(S, g, wo, bsdf, L, scale, thr, use_bsdf, rb1[0], rb1[1]);
       }
-      uint32_t ql = wave_append(&P.counters[CNT_SHADOW + bounce], nl.emit);
+      uint32_t ql = block_append(&P.counters[CNT_SHADOW + bounce], nl.emit, sh);
       if (nl.emit) {
         P.pend[slot_l] = make_float4(nl.contrib.x, nl.contrib.y, nl.contrib.z, 1.f);
         emit_shadow(P, nl, slot_l | scene_bits, ql);
@@ -547,7 +562,7 @@ __device__ __forceinline__ void shade_body(const WaveParams& P, const DevScene& 
     P.thr[p] = make_float4(thr.x, thr.y, thr.z, __uint_as_float(flags));
     P.rad[p] = make_float4(rad.x, rad.y, rad.z, 0.f);
   }
-  uint32_t qn = wave_append(&P.counters[CNT_QUEUE + bounce + 1], cont);
+  uint32_t qn = block_append(&P.counters[CNT_QUEUE + bounce + 1], cont, sh);
   if (cont) {
     P.qo[nxt][qn] = make_float4(next_o.x, next_o.y, next_o.z, next_eps);
     P.qd[nxt][qn] = make_float4(next_d.x, next_d.y, next_d.z, INFINITY);
@@ -563,8 +578,9 @@ __device__ __forceinline__ void shade_body(const WaveParams& P, const DevScene& 
 #endif
 // ALL: sample_all_lights (one NEE per light and vertex) instead of one light picked by power.
 template <bool BATCH, bool ALL>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DGPU_SHADE_WAVES)))
+__global__ __launch_bounds__(SHADE_BLOCK) __attribute__((amdgpu_waves_per_eu(DGPU_SHADE_WAVES)))
 void k_shade(WaveParams P, uint32_t bounce) {
+  __shared__ uint32_t sh[SHADE_BLOCK / 64 + 1];
   const uint32_t cur = bounce & 1u;
   const uint32_t n = P.counters[CNT_QUEUE + bounce];
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
@@ -579,13 +595,13 @@ void k_shade(WaveParams P, uint32_t bounce) {
         ps = P.pid[p];
       }
       const BatchEntry& E = P.batch[scene];
-      shade_body<true, false>(P, E.S, bounce, active, i, p, scene, ps.x, ps.y, E.k0, E.k1);
+      shade_body<true, false>(P, E.S, bounce, active, i, p, scene, ps.x, ps.y, E.k0, E.k1, sh);
     } else {
       uint32_t p = active ? P.qpath[cur][i] : 0u;
       uint64_t gidx = P.g0 + p;
       uint32_t sample = P.sample_begin + (uint32_t)(gidx / P.npix);
       uint32_t pixel = (uint32_t)(gidx % P.npix);
-      shade_body<false, ALL>(P, P.S, bounce, active, i, p, 0u, pixel, sample, P.k0, P.k1);
+      shade_body<false, ALL>(P, P.S, bounce, active, i, p, 0u, pixel, sample, P.k0, P.k1, sh);
     }
   }
 }
@@ -968,10 +984,10 @@ const void* shade_kernel_ptr(bool batch, bool all) {
 
 hipError_t launch_shade(const WaveParams& P, uint32_t capacity, uint32_t bounce, bool batch, hipStream_t st,
                         uint32_t max_blocks) {
-  uint32_t blocks = std::min((capacity + 255u) / 256u, std::max(1u, max_blocks));
-  if (batch) hipLaunchKernelGGL((k_shade<true, false>), dim3(blocks), dim3(256), 0, st, P, bounce);
-  else if (P.sample_all) hipLaunchKernelGGL((k_shade<false, true>), dim3(blocks), dim3(256), 0, st, P, bounce);
-  else hipLaunchKernelGGL((k_shade<false, false>), dim3(blocks), dim3(256), 0, st, P, bounce);
+  uint32_t blocks = std::min((capacity + SHADE_BLOCK - 1u) / SHADE_BLOCK, std::max(1u, max_blocks));
+  if (batch) hipLaunchKernelGGL((k_shade<true, false>), dim3(blocks), dim3(SHADE_BLOCK), 0, st, P, bounce);
+  else if (P.sample_all) hipLaunchKernelGGL((k_shade<false, true>), dim3(blocks), dim3(SHADE_BLOCK), 0, st, P, bounce);
+  else hipLaunchKernelGGL((k_shade<false, false>), dim3(blocks), dim3(SHADE_BLOCK), 0, st, P, bounce);
   return hipGetLastError();
 }
 

@@ -19,7 +19,9 @@ _lib = None
 EXPORTS = ["dgpu_abi_version", "dgpu_last_error", "dgpu_create", "dgpu_destroy", "dgpu_device_alloc",
            "dgpu_device_free", "dgpu_copy", "dgpu_scene_create", "dgpu_scene_destroy",
            "dgpu_scene_get_info", "dgpu_scene_update_texture", "dgpu_scene_update_material",
-           "dgpu_trace_closest", "dgpu_trace_any", "dgpu_render", "dgpu_render_batch", "dgpu_adjoint"]
+           "dgpu_trace_closest", "dgpu_trace_any", "dgpu_render", "dgpu_render_batch", "dgpu_adjoint",
+           "dgpu_read_ply", "dgpu_free", "dgpu_read_rgbe", "dgpu_write_rgbe", "dgpu_test_convergence",
+           "dgpu_bias_variance"]
 
 
 class DgpuError(RuntimeError):
@@ -57,6 +59,22 @@ def lib():
         L.dgpu_scene_update_material.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(abi.Material)]
         L.dgpu_adjoint.argtypes = [C.c_void_p, C.POINTER(abi.RenderOpts), C.c_void_p, C.c_void_p,
                                    C.c_void_p, C.c_void_p, C.POINTER(abi.RenderStats)]
+        L.dgpu_read_ply.argtypes = [C.c_char_p, C.POINTER(C.POINTER(C.c_float)), C.POINTER(C.c_uint32),
+                                    C.POINTER(C.POINTER(C.c_uint32)), C.POINTER(C.c_uint32)]
+        L.dgpu_free.argtypes = [C.c_void_p]
+        L.dgpu_free.restype = None
+        L.dgpu_read_rgbe.argtypes = [C.c_char_p, C.POINTER(C.POINTER(C.c_float)), C.POINTER(C.c_uint32),
+                                     C.POINTER(C.c_uint32)]
+        L.dgpu_write_rgbe.argtypes = [C.c_char_p, C.c_void_p, C.c_uint32, C.c_uint32]
+        L.dgpu_test_convergence.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_int32,
+                                            C.c_float, C.c_float, C.c_float, C.c_uint32, C.POINTER(C.c_int32),
+                                            C.c_char_p, C.c_size_t]
+        L.dgpu_bias_variance.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32,
+                                         C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(C.c_double),
+                                         C.POINTER(C.c_double)]
+        for n in ("dgpu_read_ply", "dgpu_read_rgbe", "dgpu_write_rgbe", "dgpu_test_convergence",
+                  "dgpu_bias_variance"):
+            getattr(L, n).restype = C.c_int
         for n in ("dgpu_create", "dgpu_device_alloc", "dgpu_device_free", "dgpu_copy",
                   "dgpu_scene_create", "dgpu_scene_get_info", "dgpu_scene_update_texture",
                   "dgpu_scene_update_material", "dgpu_trace_closest", "dgpu_trace_any", "dgpu_render",
@@ -69,6 +87,38 @@ def lib():
 def check(status):
     if status != abi.DGPU_OK:
         raise DgpuError(status, lib().dgpu_last_error().decode())
+
+
+def read_ply(path):
+    """dgpu_read_ply: (positions (V, 3) float32, indices (T, 3) uint32), faces fan-triangulated."""
+    pos, ind = C.POINTER(C.c_float)(), C.POINTER(C.c_uint32)()
+    nv, nt = C.c_uint32(), C.c_uint32()
+    check(lib().dgpu_read_ply(os.fsencode(path), C.byref(pos), C.byref(nv), C.byref(ind), C.byref(nt)))
+    try:
+        p = np.ctypeslib.as_array(pos, (max(1, nv.value) * 3,))[: 3 * nv.value].reshape(-1, 3).copy()
+        t = np.ctypeslib.as_array(ind, (max(1, nt.value) * 3,))[: 3 * nt.value].reshape(-1, 3).copy()
+    finally:
+        lib().dgpu_free(pos)
+        lib().dgpu_free(ind)
+    return p, t
+
+
+def read_rgbe(path):
+    """dgpu_read_rgbe: (H, W, 3) float32."""
+    px = C.POINTER(C.c_float)()
+    w, h = C.c_uint32(), C.c_uint32()
+    check(lib().dgpu_read_rgbe(os.fsencode(path), C.byref(px), C.byref(w), C.byref(h)))
+    try:
+        img = np.ctypeslib.as_array(px, (h.value * w.value * 3,)).reshape(h.value, w.value, 3).copy()
+    finally:
+        lib().dgpu_free(px)
+    return img
+
+
+def write_rgbe(path, img):
+    """dgpu_write_rgbe of an (H, W, 3) image."""
+    a = np.ascontiguousarray(img, dtype=np.float32)
+    check(lib().dgpu_write_rgbe(os.fsencode(path), a.ctypes.data, a.shape[1], a.shape[0]))
 
 
 class Context:
@@ -105,6 +155,36 @@ class Context:
         st = abi.RenderStats()
         check(lib().dgpu_render_batch(self.h, arr, n, flags, C.byref(st)))
         return films, {f: getattr(st, f) for f, _ in abi.RenderStats._fields_}
+
+    def test_convergence(self, reference, tested, min_tile_size=32, variation=8.0, bias=0.0, p_value=0.01):
+        """dgpu_test_convergence on (H, W, 3) float32 images (numpy on the host, or
+        DeviceArrays): "" when the test passes, else dort's description of the first
+        failing tile."""
+        dev = isinstance(reference, DeviceArray)
+        if dev:
+            h, w = tested.shape[0], tested.shape[1]
+            rp, tp = reference.ptr, tested.ptr
+        else:
+            reference = np.ascontiguousarray(reference, dtype=np.float32)
+            tested = np.ascontiguousarray(tested, dtype=np.float32)
+            h, w = reference.shape[:2]
+            rp, tp = reference.ctypes.data, tested.ctypes.data
+        ok = C.c_int32()
+        msg = C.create_string_buffer(512)
+        check(lib().dgpu_test_convergence(self.h, rp, tp, w, h, min_tile_size, variation, bias, p_value,
+                                          abi.BUF_DEVICE if dev else 0, C.byref(ok), msg, len(msg)))
+        return "" if ok.value else msg.value.decode()
+
+    def bias_variance(self, reference, tested, rect=None):
+        """dgpu_bias_variance over rect = (x0, y0, x1, y1) (default: the whole image)."""
+        reference = np.ascontiguousarray(reference, dtype=np.float32)
+        tested = np.ascontiguousarray(tested, dtype=np.float32)
+        h, w = reference.shape[:2]
+        x0, y0, x1, y1 = rect if rect is not None else (0, 0, w, h)
+        b, v = C.c_double(), C.c_double()
+        check(lib().dgpu_bias_variance(self.h, reference.ctypes.data, tested.ctypes.data, w, h, x0, y0, x1, y1,
+                                       0, C.byref(b), C.byref(v)))
+        return b.value, v.value
 
     def device_array(self, host):
         """Copies a numpy array into a new DeviceArray on this context's GPU."""

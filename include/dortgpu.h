@@ -271,6 +271,40 @@ dgpu_status dgpu_adjoint(dgpu_scene* scene, const dgpu_render_opts* opts, const 
                          float* d_albedo_texels, float* d_roughness, float* image_out,
                          dgpu_stats* out_stats);
 
+/* ---- scene and image files (host; dort_amd/csrc/io.cpp) -------------------- */
+
+/* PLY mesh (ascii, binary little or big endian), as dort's read_ply_callbacks /
+ * read_ply_to_mesh read it (src/dort/ply_mesh.cpp:9-103): vertex x, y, z and each face's
+ * vertex_indices list as the fan (i0, i1, i2), (i0, i2, i3), ...  The arrays are
+ * allocated by the library (release them with dgpu_free).  Unlike dort, an index out
+ * of range is an error (DGPU_EINVAL) instead of undefined behaviour. */
+dgpu_status dgpu_read_ply(const char* path, float** positions, uint32_t* num_vertices,
+                          uint32_t** indices, uint32_t* num_triangles);
+void dgpu_free(void* p);
+
+/* Radiance RGBE images of 3-channel fp32 pixels, row-major from the top, as dort's
+ * read_image_f / write_image_rgbe (src/dort/image.cpp:28-67, stb_image) read and write
+ * them: "-Y H +X W" scanlines, flat or run-length encoded. */
+dgpu_status dgpu_read_rgbe(const char* path, float** rgb, uint32_t* width, uint32_t* height);
+dgpu_status dgpu_write_rgbe(const char* path, const float* rgb, uint32_t width, uint32_t height);
+
+/* ---- image comparison (GPU; dort_amd/csrc/compare.hip) --------------------- */
+
+/* dort's test_convergence (src/dort/convergence_test.cpp:10-101) over width*height*3 fp32
+ * images (host memory, or device memory with DGPU_BUF_DEVICE): *passed = 1 when every
+ * tile of every depth passes, else 0 with the first failing tile (smallest depth, then
+ * row-major) described in msg as dort describes it. */
+dgpu_status dgpu_test_convergence(dgpu_ctx* ctx, const float* reference, const float* tested,
+                                  uint32_t width, uint32_t height, int32_t min_tile_size,
+                                  float variation, float bias, float p_value, uint32_t flags,
+                                  int32_t* passed, char* msg, size_t msg_size);
+/* dort's bias_variance (src/dort/lua_image.cpp:179-211) over [x0, x1) x [y0, y1): the sum
+ * of (reference - tested) and the sum of its squares over the pixels' channels. */
+dgpu_status dgpu_bias_variance(dgpu_ctx* ctx, const float* reference, const float* tested,
+                               uint32_t width, uint32_t height, uint32_t x0, uint32_t y0,
+                               uint32_t x1, uint32_t y1, uint32_t flags, double* bias,
+                               double* variance);
+
 #ifdef __cplusplus
 }
 #endif

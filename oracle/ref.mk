@@ -1,6 +1,7 @@
-# Builds oracle/_ref/libdortref.so: dort's own pt-path sources, compiled where
-# they lie under /root/reference, linked with ref_driver.cpp (TEST
-# INFRASTRUCTURE ONLY; the output stays out of git).
+# Builds oracle/_ref/libdortref.so: dort's own pt-path sources, its PLY reader
+# (with the rply C parser it links), image I/O (stb) and convergence test, compiled
+# where they lie under /root/reference, linked with ref_driver.cpp and
+# ref_io_driver.cpp (TEST INFRASTRUCTURE ONLY; the output stays out of git).
 #
 #   make -f oracle/ref.mk            (needs /root/reference; nothing is copied)
 #
@@ -15,7 +16,8 @@ OUT := $(HERE)_ref
 REF_CXX ?= $(firstword $(wildcard /opt/rocm/llvm/bin/clang++) clang++)
 CXXFLAGS := -O3 -std=c++14 -fPIC -DDORT_DISABLE_STAT -DNDEBUG \
   -include functional -include string -include stdexcept \
-  -I$(REF)/include -I$(REF)/extern -I$(REF)/extern/stb -w
+  -I$(REF)/include -I$(REF)/extern -I$(REF)/extern/stb -I$(REF)/extern/rply -w
+REF_CC ?= $(firstword $(wildcard /opt/rocm/llvm/bin/clang) clang)
 
 SRCS := path_renderer renderer film bvh bvh_primitive mesh_bvh_primitive triangle \
   primitive scene bsdf lambert_material microfacet rough_dielectric_material \
@@ -23,8 +25,9 @@ SRCS := path_renderer renderer film bvh bvh_primitive mesh_bvh_primitive triangl
   infinite_light piecewise_distrib_2d discrete_distrib_1d monte_carlo \
   pinhole_camera camera transform random_sampler sampler sampled_filter filter \
   stats thread_pool box geometry math image image_texture texture_maps \
-  list_primitive mat triangle_shape triangle_shape_primitive mesh_triangle_primitive
-OBJS := $(addprefix $(OUT)/obj/,$(addsuffix .o,$(SRCS)))
+  list_primitive mat triangle_shape triangle_shape_primitive mesh_triangle_primitive \
+  ply_mesh convergence_test
+OBJS := $(addprefix $(OUT)/obj/,$(addsuffix .o,$(SRCS))) $(OUT)/obj/rply.o $(OUT)/obj/ref_io_driver.o
 
 all: $(OUT)/libdortref.so
 
@@ -35,6 +38,15 @@ $(OUT)/obj/%.o: $(REF)/src/dort/%.cpp
 $(OUT)/obj/ref_driver.o: $(HERE)ref_driver.cpp $(HERE)../include/dortgpu.h
 	@mkdir -p $(OUT)/obj
 	$(REF_CXX) $(CXXFLAGS) -c $< -o $@
+
+$(OUT)/obj/ref_io_driver.o: $(HERE)ref_io_driver.cpp
+	@mkdir -p $(OUT)/obj
+	$(REF_CXX) $(CXXFLAGS) -c $< -o $@
+
+# rply (extern/rply, the PLY parser dort links) is C
+$(OUT)/obj/rply.o: $(REF)/extern/rply/rply.c
+	@mkdir -p $(OUT)/obj
+	$(REF_CC) -O2 -fPIC -w -I$(REF)/extern/rply -c $< -o $@
 
 $(OUT)/libdortref.so: $(OBJS) $(OUT)/obj/ref_driver.o
 	$(REF_CXX) -shared -o $@ $^ -lpthread

@@ -58,8 +58,59 @@ def lib():
                                 C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.ref_render.restype = C.c_int
         L.ref_render.argtypes = [C.c_void_p, C.POINTER(abi.RenderOpts), C.c_uint32, C.c_void_p, C.c_void_p]
+        L.ref_read_ply.argtypes = [C.c_char_p, C.POINTER(C.POINTER(C.c_float)), C.POINTER(C.c_uint32),
+                                   C.POINTER(C.POINTER(C.c_uint32)), C.POINTER(C.c_uint32)]
+        L.ref_free.argtypes = [C.c_void_p]
+        L.ref_read_hdr.argtypes = [C.c_char_p, C.POINTER(C.POINTER(C.c_float)), C.POINTER(C.c_uint32),
+                                   C.POINTER(C.c_uint32)]
+        L.ref_write_hdr.argtypes = [C.c_char_p, C.c_void_p, C.c_uint32, C.c_uint32]
+        L.ref_test_convergence.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_int32, C.c_float,
+                                           C.c_float, C.c_float, C.c_char_p, C.c_size_t]
         _lib = L
     return _lib
+
+
+def read_ply(path):
+    """dort's read_ply_to_ply_mesh: (positions (V, 3), indices (T, 3))."""
+    pos, ind = C.POINTER(C.c_float)(), C.POINTER(C.c_uint32)()
+    nv, nt = C.c_uint32(), C.c_uint32()
+    rc = lib().ref_read_ply(os.fsencode(path), C.byref(pos), C.byref(nv), C.byref(ind), C.byref(nt))
+    if rc != 0:
+        raise ValueError(f"dort could not read {path} ({rc})")
+    p = np.ctypeslib.as_array(pos, (3 * nv.value + 1,))[: 3 * nv.value].reshape(-1, 3).copy()
+    t = np.ctypeslib.as_array(ind, (3 * nt.value + 1,))[: 3 * nt.value].reshape(-1, 3).copy()
+    lib().ref_free(pos)
+    lib().ref_free(ind)
+    return p, t
+
+
+def read_hdr(path):
+    """dort's read_image_f (stb_image): (H, W, 3) float32."""
+    px = C.POINTER(C.c_float)()
+    w, h = C.c_uint32(), C.c_uint32()
+    if lib().ref_read_hdr(os.fsencode(path), C.byref(px), C.byref(w), C.byref(h)) != 0:
+        raise ValueError(f"dort could not open {path}")
+    img = np.ctypeslib.as_array(px, (3 * w.value * h.value + 1,))[: 3 * w.value * h.value]
+    img = img.reshape(h.value, w.value, 3).copy()
+    lib().ref_free(px)
+    return img
+
+
+def write_hdr(path, img):
+    """dort's write_image_rgbe (stb_image_write)."""
+    a = np.ascontiguousarray(img, dtype=np.float32)
+    if lib().ref_write_hdr(os.fsencode(path), a.ctypes.data, a.shape[1], a.shape[0]) != 0:
+        raise ValueError(f"dort could not write {path}")
+
+
+def test_convergence(reference, tested, min_tile_size=32, variation=8.0, bias=0.0, p_value=0.01):
+    """dort's test_convergence on one thread: "" or the first failing tile."""
+    a = np.ascontiguousarray(reference, dtype=np.float32)
+    b = np.ascontiguousarray(tested, dtype=np.float32)
+    msg = C.create_string_buffer(512)
+    ok = lib().ref_test_convergence(a.ctypes.data, b.ctypes.data, a.shape[1], a.shape[0], min_tile_size,
+                                    variation, bias, p_value, msg, len(msg))
+    return "" if ok else msg.value.decode()
 
 
 class RefScene:
