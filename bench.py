@@ -105,8 +105,8 @@ def intersection_bench(ctx, args):
         ams.append(gsc.trace_any(dev, out=occ, device=True)[1])
     c_ms, a_ms = float(np.median(cms)), float(np.median(ams))
     hit_frac = float((out["prim"].numpy() >= 0).mean())
-    # the scene once: 36 B of positions per triangle + 32 B per child box (width per node)
-    scene_bytes = 36 * sc.num_triangles + 32 * info["width"] * info["nodes"]
+    # the scene once: 36 B of positions per triangle + the BVH nodes as stored
+    scene_bytes = 36 * sc.num_triangles + info["node_bytes"]
     res = {"workload": f"displaced icosphere {sc.num_triangles} tris, {n} rays (4096x2048 pinhole + random)",
            "Mrays_closest": n / c_ms / 1e3, "Mrays_any": n / a_ms / 1e3,
            "closest_ms": c_ms, "any_ms": a_ms, "hit_fraction": hit_frac,
@@ -257,8 +257,8 @@ def main():
     popts.flags |= abi.RENDER_PROFILE
     _, _, ps = gsc.render(popts, film, wfilm, device=True)
     ext_rays, shd_rays = ps["extension_rays"], ps["shadow_rays"]
-    # the scene once: 36 B of positions per triangle + 32 B per child box (width per node)
-    scene_bytes = 36 * sc.num_triangles + 32 * info["width"] * info["nodes"]
+    # the scene once: 36 B of positions per triangle + the BVH nodes as stored
+    scene_bytes = 36 * sc.num_triangles + info["node_bytes"]
     kernels = {
         # closest-hit: 32 B ray in + 16 B hit out per ray, plus the scene once per launch
         "extend": (ps["extend_ms"], ps["extend_launches"], 48 * ext_rays / max(1, ps["extend_launches"]) + scene_bytes),

@@ -90,7 +90,14 @@ class RenderStats(C.Structure):
                 ("kernel_ms", C.c_double), ("extend_ms", C.c_double), ("shade_ms", C.c_double),
                 ("connect_ms", C.c_double), ("other_ms", C.c_double),
                 ("extend_launches", C.c_uint32), ("shade_launches", C.c_uint32),
-                ("connect_launches", C.c_uint32), ("other_launches", C.c_uint32)]
+                ("connect_launches", C.c_uint32), ("other_launches", C.c_uint32),
+                ("bounce_extension_rays", C.c_uint64 * 16), ("bounce_shadow_rays", C.c_uint64 * 16)]
+
+
+def stats_dict(st):
+    """A RenderStats as a dict (arrays as lists)."""
+    return {f: (list(getattr(st, f)) if f.startswith("bounce_") else getattr(st, f))
+            for f, _ in RenderStats._fields_}
 
 
 class SceneInfo(C.Structure):

@@ -683,7 +683,7 @@ dgpu_status dgpu_render(dgpu_scene* s, const dgpu_render_opts* o, float* film_rg
   P.sd = (float4*)ctx->buf("sd", (size_t)C * slots * 16, &st);
   P.smeta = (uint2*)ctx->buf("smeta", (size_t)C * slots * 8, &st);
   P.counters = (uint32_t*)ctx->buf("counters", NUM_COUNTERS * 4, &st);
-  P.stats = (unsigned long long*)ctx->buf("stats", 64, &st);
+  P.stats = (unsigned long long*)ctx->buf("stats", STATS_WORDS * 8, &st);
   P.spill = (uint32_t*)ctx->buf("spill", (size_t)maxgrid * s->cfg.block * s->stack_cap * 4, &st);
   P.stack_cap = s->stack_cap;
   bool dev = o->flags & DGPU_BUF_DEVICE;
@@ -695,7 +695,7 @@ dgpu_status dgpu_render(dgpu_scene* s, const dgpu_render_opts* o, float* film_rg
     P.film_w = (float*)ctx->buf("film_w", (size_t)npix * 4, &st);
   }
   if (st != DGPU_OK) return st;
-  HIP_TRY(hipMemsetAsync(P.stats, 0, 64, stream));
+  HIP_TRY(hipMemsetAsync(P.stats, 0, STATS_WORDS * 8, stream));
   if (!dev) {
     HIP_TRY(hipMemsetAsync(P.film_rgb, 0, (size_t)npix * 12, stream));
     HIP_TRY(hipMemsetAsync(P.film_w, 0, (size_t)npix * 4, stream));
@@ -729,7 +729,7 @@ dgpu_status dgpu_render(dgpu_scene* s, const dgpu_render_opts* o, float* film_rg
     if (!cancelled) HIP_TRY(timed(3, [&] { return launch_finalize(P, P.n_chunk, false, stream); }));
   }
   HIP_TRY(hipEventRecord(ctx->ev1, stream));
-  unsigned long long hstats[4] = {0, 0, 0, 0};
+  unsigned long long hstats[STATS_WORDS] = {};
   HIP_TRY(hipMemcpyAsync(hstats, P.stats, sizeof hstats, hipMemcpyDeviceToHost, stream));
   std::vector<float> hrgb, hw;
   if (!dev && !cancelled) {
@@ -751,6 +751,10 @@ dgpu_status dgpu_render(dgpu_scene* s, const dgpu_render_opts* o, float* film_rg
     out_stats->extension_rays = hstats[1];
     out_stats->shadow_rays = hstats[2];
     out_stats->dropped_samples = hstats[3];
+    for (int b = 0; b < 16; ++b) {
+      out_stats->bounce_extension_rays[b] = hstats[4 + b];
+      out_stats->bounce_shadow_rays[b] = hstats[20 + b];
+    }
     out_stats->kernel_ms = ms;
     double* kms[4] = {&out_stats->extend_ms, &out_stats->shade_ms, &out_stats->connect_ms, &out_stats->other_ms};
     uint32_t* kn[4] = {&out_stats->extend_launches, &out_stats->shade_launches, &out_stats->connect_launches,
@@ -1008,7 +1012,7 @@ dgpu_status dgpu_render_batch(dgpu_ctx* ctx, const dgpu_batch_item* items, uint3
   P.smeta = (uint2*)ctx->buf("smeta", (size_t)cap * 2 * 8, &st);
   P.pid = (uint2*)ctx->buf("pid", (size_t)cap * 8, &st);
   P.counters = (uint32_t*)ctx->buf("counters", NUM_COUNTERS * 4, &st);
-  P.stats = (unsigned long long*)ctx->buf("stats", 64, &st);
+  P.stats = (unsigned long long*)ctx->buf("stats", STATS_WORDS * 8, &st);
   P.spill = (uint32_t*)ctx->buf("spill", (size_t)maxgrid * s0->cfg.block * stack_cap * 4, &st);
   P.stack_cap = stack_cap;
   BatchEntry* dtab = (BatchEntry*)ctx->buf("batch_tab", n * sizeof(BatchEntry), &st);
@@ -1022,7 +1026,7 @@ dgpu_status dgpu_render_batch(dgpu_ctx* ctx, const dgpu_batch_item* items, uint3
   HIP_TRY(hipMemcpyAsync(dtab, tab.data(), n * sizeof(BatchEntry), hipMemcpyHostToDevice, stream));
   HIP_TRY(hipMemcpyAsync(dchunks, chunks.data(), chunks.size() * sizeof(ChunkDesc), hipMemcpyHostToDevice, stream));
   HIP_TRY(hipMemsetAsync(dcur, 0, 4, stream));
-  HIP_TRY(hipMemsetAsync(P.stats, 0, 64, stream));
+  HIP_TRY(hipMemsetAsync(P.stats, 0, STATS_WORDS * 8, stream));
   // tab and chunks must stay alive until the copies above have run
   HIP_TRY(hipStreamSynchronize(stream));
 
@@ -1064,7 +1068,7 @@ dgpu_status dgpu_render_batch(dgpu_ctx* ctx, const dgpu_batch_item* items, uint3
     for (size_t c = 0; c < chunks.size(); ++c) HIP_TRY(chunk_sequence());
   }
   HIP_TRY(hipEventRecord(ctx->ev1, stream));
-  unsigned long long hstats[4] = {0, 0, 0, 0};
+  unsigned long long hstats[STATS_WORDS] = {};
   HIP_TRY(hipMemcpyAsync(hstats, P.stats, sizeof hstats, hipMemcpyDeviceToHost, stream));
   std::vector<float> hfilm;
   if (!dev) {
@@ -1088,6 +1092,10 @@ dgpu_status dgpu_render_batch(dgpu_ctx* ctx, const dgpu_batch_item* items, uint3
     out_stats->extension_rays = hstats[1];
     out_stats->shadow_rays = hstats[2];
     out_stats->dropped_samples = hstats[3];
+    for (int b = 0; b < 16; ++b) {
+      out_stats->bounce_extension_rays[b] = hstats[4 + b];
+      out_stats->bounce_shadow_rays[b] = hstats[20 + b];
+    }
     out_stats->kernel_ms = ms;
   }
   (void)total;

@@ -13,6 +13,9 @@ constexpr uint32_t CNT_SHADOW = 64;      // [b]: shadow rays of bounce b
 constexpr uint32_t CNT_FETCH_EXT = 128;  // persistent-kernel work counters
 constexpr uint32_t CNT_FETCH_SHD = 192;
 constexpr uint32_t NUM_COUNTERS = 256;
+// Render statistics on the device: paths, extension rays, shadow rays, dropped samples,
+// then 16 per-bounce extension and 16 per-bounce shadow ray counts.
+constexpr uint32_t STATS_WORDS = 4 + 2 * 16;
 constexpr uint32_t MAX_SLOT_LIGHTS = 8;  // sample_all_lights supports up to this many lights
 #ifndef DGPU_SHADE_BLOCK
 #define DGPU_SHADE_BLOCK 256

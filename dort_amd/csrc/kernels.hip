@@ -649,6 +649,8 @@ __device__ __forceinline__ void add_chunk_stats(const WaveParams& P, uint32_t n_
     for (uint32_t b = 0; b < MAX_BOUNCES; ++b) {
       ext += P.counters[CNT_QUEUE + b];
       shd += P.counters[CNT_SHADOW + b];
+      P.stats[4 + (b < 15u ? b : 15u)] += P.counters[CNT_QUEUE + b];
+      P.stats[20 + (b < 15u ? b : 15u)] += P.counters[CNT_SHADOW + b];
     }
     P.stats[0] += n_chunk;
     P.stats[1] += ext;

@@ -154,7 +154,7 @@ class Context:
         flags = (abi.BUF_DEVICE if device else 0) | (0 if graph else abi.BATCH_NO_GRAPH)
         st = abi.RenderStats()
         check(lib().dgpu_render_batch(self.h, arr, n, flags, C.byref(st)))
-        return films, {f: getattr(st, f) for f, _ in abi.RenderStats._fields_}
+        return films, abi.stats_dict(st)
 
     def test_convergence(self, reference, tested, min_tile_size=32, variation=8.0, bias=0.0, p_value=0.01):
         """dgpu_test_convergence on (H, W, 3) float32 images (numpy on the host, or
@@ -300,7 +300,7 @@ class Scene:
         if cancel is not None:
             cptr = cancel.ctypes.data
         check(lib().dgpu_render(self.h, C.byref(o), ptr(film_rgb), ptr(film_w), C.byref(st), cptr))
-        stats = {f: getattr(st, f) for f, _ in abi.RenderStats._fields_}
+        stats = abi.stats_dict(st)
         return film_rgb, film_w, stats
 
     def image(self, opts):
@@ -328,5 +328,5 @@ class Scene:
         st = abi.RenderStats()
         check(lib().dgpu_adjoint(self.h, C.byref(opts), dldi.ctypes.data, ptr(g_tex), ptr(g_rough),
                                  ptr(img), C.byref(st)))
-        stats = {f: getattr(st, f) for f, _ in abi.RenderStats._fields_}
+        stats = abi.stats_dict(st)
         return g_tex, g_rough, img, stats

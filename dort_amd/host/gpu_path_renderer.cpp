@@ -110,6 +110,10 @@ GpuPathRenderer::GpuPathRenderer(std::shared_ptr<dort::Scene> scene, std::shared
     : dort::Renderer(std::move(scene), std::move(film), std::move(sampler), std::move(camera)),
       gpu_scene_(std::move(gpu_scene)),
       opts_(opts) {
+  // the structs of dortgpu.h this file was compiled with must be the library's
+  if (dgpu_abi_version() != DGPU_ABI_VERSION)
+    throw std::runtime_error("GpuPathRenderer: libdortgpu ABI version " + std::to_string(dgpu_abi_version()) +
+                             ", compiled against " + std::to_string(DGPU_ABI_VERSION));
   if (!gpu_scene_) throw std::invalid_argument("GpuPathRenderer: no GPU scene description");
   if (opts_.iterations == 0) throw std::invalid_argument("GpuPathRenderer: iterations must be positive");
   // The GPU film is dort's default 1-pixel box filter (lua_render.cpp:108-109):

@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define DGPU_ABI_VERSION 2u
+#define DGPU_ABI_VERSION 3u
 
 typedef enum {
   DGPU_OK = 0,
@@ -174,6 +174,9 @@ typedef struct {
   /* With DGPU_RENDER_PROFILE: GPU time and launch count per kernel kind. */
   double extend_ms, shade_ms, connect_ms, other_ms;
   uint32_t extend_launches, shade_launches, connect_launches, other_launches;
+  /* Queue lengths per bounce (bounces past 15 are added to [15]). */
+  uint64_t bounce_extension_rays[16];
+  uint64_t bounce_shadow_rays[16];
 } dgpu_stats;
 
 /* ---- context ------------------------------------------------------------- */

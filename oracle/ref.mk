@@ -60,7 +60,7 @@ $(OUT)/obj/gpu_path_renderer.o: $(HOST)/gpu_path_renderer.cpp $(HOST)/gpu_path_r
 	@mkdir -p $(OUT)/obj
 	$(REF_CXX) $(CXXFLAGS) -I$(HERE)../include -c $< -o $@
 
-$(OUT)/obj/ref_gpu_glue.o: $(HERE)ref_gpu_glue.cpp $(HOST)/gpu_path_renderer.hpp
+$(OUT)/obj/ref_gpu_glue.o: $(HERE)ref_gpu_glue.cpp $(HOST)/gpu_path_renderer.hpp $(HERE)../include/dortgpu.h
 	@mkdir -p $(OUT)/obj
 	$(REF_CXX) $(CXXFLAGS) -I$(HERE)../include -c $< -o $@
 
