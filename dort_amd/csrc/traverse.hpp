@@ -264,7 +264,7 @@ DG_INLINE void order_pair(float& da, uint32_t& ca, float& db, uint32_t& cb) {
 }
 
 // Visits inner node s.node.  BVH2: both child boxes in the parent node (64 B);
-// BVH4: 128 B nodes.  s.node becomes the nearest child hit (the others pushed far
+// BVH4: compressed 64 B nodes.  s.node becomes the nearest child hit (the others pushed far
 // to near), or the next stacked node, or TRAV_END.  Any-hit rays (ANY) stop at the
 // first hit anywhere, so only the nearest child is picked (a three-comparison
 // tournament) and the other hit children are pushed unsorted.
