@@ -197,7 +197,7 @@ BuiltBvh build_bvh(const float* positions, uint32_t num_vertices, const uint32_t
                    uint32_t num_tris, const BvhParams& prm) {
   (void)num_vertices;
   if (num_tris == 0) throw std::invalid_argument("scene has no triangles");
-  if (num_tris > (1u << 27)) throw std::invalid_argument("more than 2^27 triangles");
+  if (num_tris >= (1u << 27)) throw std::invalid_argument("2^27 triangles or more");
   BuiltBvh out;
   out.width = prm.width == 4 ? 4 : 2;
   uint32_t max_leaf = std::min(std::max(prm.max_leaf_size, 1u), 16u);
@@ -317,6 +317,7 @@ BuiltBvh build_bvh(const float* positions, uint32_t num_vertices, const uint32_t
     //   [2]  lo.z[4] hi.z[4] child[0] child[1]
     //   [3]  child[2] child[3] 0 0
     // A plane is origin + q * 2^e.  The boxes are grown by box_pad before quantizing.
+    // An empty slot has an inverted box (lo 255, hi 0) and empty_child_code.
     out.nodes.assign(16 * wide.size(), 0.f);
     uint32_t leaves = 0;
     for (size_t i = 0; i < wide.size(); ++i) {
@@ -347,10 +348,11 @@ BuiltBvh build_bvh(const float* positions, uint32_t num_vertices, const uint32_t
       }
       w[3] = exps;
       uint32_t q[6] = {0, 0, 0, 0, 0, 0};  // lo.x hi.x lo.y hi.y lo.z hi.z, a byte per child
-      uint32_t code[4] = {EMPTY_CHILD_CODE, EMPTY_CHILD_CODE, EMPTY_CHILD_CODE, EMPTY_CHILD_CODE};
+      const uint32_t empty = empty_child_code(num_tris);
+      uint32_t code[4] = {empty, empty, empty, empty};
       for (size_t c = 0; c < 4; ++c) {
         for (int a = 0; a < 3; ++a) {
-          uint32_t ql = 255, qh = 0;  // empty slot: inverted box (and the child code says empty)
+          uint32_t ql = 255, qh = 0;  // empty slot: inverted box
           if (c < nk) {
             // (child - origin) * 2^-e is exact in double unless the subtraction rounds;
             // then one more quantum outward keeps the quantized box around the child
@@ -378,6 +380,7 @@ BuiltBvh build_bvh(const float* positions, uint32_t num_vertices, const uint32_t
     }
     out.num_nodes = (uint32_t)wide.size();
     out.num_leaves = leaves;
+    out.tris.resize(out.tris.size() + 12, 0.f);  // the empty-slot triangle: degenerate, never hit
     // wide depth
     std::vector<uint32_t> dep(wide.size(), 0);
     uint32_t md = 0;

@@ -15,7 +15,7 @@ struct BvhParams {
 
 struct BuiltBvh {
   std::vector<float> nodes;          // 16 floats (64 B) per node: BVH2, or compressed BVH4
-  std::vector<float> tris;           // 12 floats per triangle, leaf order
+  std::vector<float> tris;           // 12 floats per triangle, leaf order (BVH4: + the empty-slot triangle)
   std::vector<uint32_t> tri_leaf_of; // scene triangle id -> leaf-order slot
   uint32_t width = 2;
   uint32_t depth = 0;
@@ -23,13 +23,17 @@ struct BuiltBvh {
   uint32_t num_leaves = 0;
 };
 
-// Child code of an empty slot of a 4-wide node.
-constexpr uint32_t EMPTY_CHILD_CODE = 0x7FFFFFFFu;
 
 // Leaf child code: bit 31 set, bits 27..30 = count-1, bits 0..26 = first triangle.
 inline uint32_t leaf_code(uint32_t first, uint32_t count) {
   return 0x80000000u | ((count - 1u) << 27) | first;
 }
+
+// Child code of an empty slot of a 4-wide node: a leaf holding only triangle slot
+// num_tris, a degenerate triangle (all vertices at the origin) that the builder
+// appends and that no ray hits.  The slot's box is inverted, so a ray visits it only
+// through rounding, and then harmlessly.
+inline uint32_t empty_child_code(uint32_t num_tris) { return leaf_code(num_tris, 1); }
 
 BuiltBvh build_bvh(const float* positions, uint32_t num_vertices, const uint32_t* indices,
                    uint32_t num_tris, const BvhParams& prm);

@@ -8,7 +8,9 @@
 //      n3 = (child0, child1, -, -)      child >= 0: inner node; child < 0: leaf
 //  * BVH4 nodes: 64 B each, child boxes quantized to 8 bits per plane in the node's frame:
 //      (origin.xyz, exponents), (lo.x[4] hi.x[4] lo.y[4] hi.y[4] bytes),
-//      (lo.z[4] hi.z[4] bytes, child0, child1), (child2, child3, -, -)
+//      (lo.z[4] hi.z[4] bytes, child0, child1), (child2, child3, -, -); an empty slot has
+//      an inverted box (lo 255, hi 0) and the code of a leaf holding one degenerate
+//      triangle, appended after the scene's triangles, that no ray hits
 //  * Triangles in leaf order: 48 B each (3 x float4 vertices; v0.w = scene triangle id).
 //  * Leaf child encoding: bit 31 set, bits 27..30 = count-1, bits 0..26 = first triangle.
 #pragma once
@@ -29,7 +31,6 @@ constexpr float INV_TWO_PI_F = 1.0f / TWO_PI_F;
 constexpr float INV_FOUR_PI_F = 1.0f / (4.0f * PI_F);
 
 constexpr uint32_t LEAF_BIT = 0x80000000u;
-constexpr uint32_t EMPTY_CHILD = 0x7FFFFFFFu;  // child code of an empty BVH4 slot
 constexpr int MAX_STACK = 64;                   // BVH depth bound enforced by the builder
 
 DG_INLINE uint32_t leaf_encode(uint32_t first, uint32_t count) {

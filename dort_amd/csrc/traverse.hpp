@@ -195,18 +195,21 @@ DG_INLINE void trav_begin(TravState& s, const TraceRay& r, LaneStack& st) {
   st.sp = 0;
 }
 
-// Tests the first triangle of leaf `code`; returns true if (ANY) a hit was found.
-// A leaf is walked one triangle per traversal step (see trav_step).
+// Tests the first triangle of leaf `code`, whose vertices are va, vb, vc; returns
+// true if (ANY) a hit was found.  A leaf is walked one triangle per traversal step
+// (see trav_step).
 template <bool ANY>
-DG_INLINE bool leaf_test(const float4* __restrict__ tris, uint32_t code, TravState& s) {
+DG_INLINE bool leaf_tri(float4 va, float4 vb, float4 vc, uint32_t code, TravState& s) {
   uint32_t ti = leaf_first(code);
-  if (ti == s.ignore) return false;
-  float4 va = tris[3 * ti], vb = tris[3 * ti + 1], vc = tris[3 * ti + 2];
   float t, b1, b2;
-  if (!wt_hit(s.w, va, vb, vc, s.tmin, s.tmax, t, b1, b2)) return false;
+  if (ti == s.ignore || !wt_hit(s.w, va, vb, vc, s.tmin, s.tmax, t, b1, b2)) return false;
   s.h.t = t; s.h.b1 = b1; s.h.b2 = b2; s.h.tri = ti;
   if (!ANY) s.tmax = t;
   return ANY;
+}
+
+DG_INLINE uint4 f4_bits(float4 v) {
+  return make_uint4(__float_as_uint(v.x), __float_as_uint(v.y), __float_as_uint(v.z), __float_as_uint(v.w));
 }
 
 constexpr uint32_t TRAV_END = 0xFFFFFFFFu;  // no node left (the builder keeps leaf codes below it)
@@ -214,44 +217,56 @@ constexpr uint32_t TRAV_END = 0xFFFFFFFFu;  // no node left (the builder keeps l
 DG_INLINE void trav_pop(TravState& s, LaneStack& st) { s.node = st.sp ? st.pop() : TRAV_END; }
 
 DG_INLINE float byte_f(uint32_t w, int k) { return (float)((w >> (8 * k)) & 0xFFu); }
-DG_INLINE float pow2f(uint32_t e8) {  // 2^e for the int8 e in the low byte of e8, |e| <= 100
-  return __uint_as_float((uint32_t)((int32_t)(int8_t)(e8 & 0xFFu) + 127) << 23);
-}
+// The int8 exponent in byte k of w (|e| <= 100).
+DG_INLINE int exp_i8(uint32_t w, int k) { return (int)(int8_t)((w >> (8 * k)) & 0xFFu); }
 
 // Entry distances of the four children of a compressed node (A: origin + exponents,
-// Q: lo.x hi.x lo.y hi.y bytes, R: lo.z hi.z bytes).  A plane is origin + q 2^e:
+// Q: lo.x hi.x lo.y hi.y bytes, R: lo.z hi.z bytes), +inf for a child the ray misses.
+// A plane is origin + q 2^e:
 //  FAST:  t = fma(q, 2^e inv, fma(origin, inv, -o inv)), two instructions per plane;
+//         the box padding covers its rounding (see box_enter), so the far distance is
+//         not widened;
 //  exact: plane = fma(q, 2^e, origin) (one rounding, covered by the box padding), then
-//         (plane - o) * inv as in box_enter.
+//         (plane - o) * inv, far distance scaled by 1 + 2 gamma(3) as in box_enter.
+// Along an axis the entry plane is lo where the direction is positive and hi where it
+// is negative (t is monotonic in the plane, also after rounding), so each node picks
+// its near and far plane words by the direction's sign instead of a min and a max per
+// child and axis.  An empty slot's inverted box (lo 255 > hi 0) then has its entry
+// beyond its exit on every axis and is missed without a test of its child code.
 template <bool FAST>
 DG_INLINE void decode_boxes(uint4 A, uint4 Q, uint4 R, const TravState& s, float& d0, float& d1, float& d2,
                             float& d3) {
   V3 org = v3(__uint_as_float(A.x), __uint_as_float(A.y), __uint_as_float(A.z));
-  V3 sc = v3(pow2f(A.w), pow2f(A.w >> 8), pow2f(A.w >> 16));
-  float t[4][6];
+  bool nx = s.inv.x < 0.f, ny = s.inv.y < 0.f, nz = s.inv.z < 0.f;
+  uint32_t near_x = nx ? Q.y : Q.x, far_x = nx ? Q.x : Q.y;
+  uint32_t near_y = ny ? Q.w : Q.z, far_y = ny ? Q.z : Q.w;
+  uint32_t near_z = nz ? R.y : R.x, far_z = nz ? R.x : R.y;
+  float d[4];
   if (FAST) {
-    V3 si = v3(sc.x * s.inv.x, sc.y * s.inv.y, sc.z * s.inv.z);
+    // 2^e inv, exact (ldexp); inv is at least 1 in magnitude and 2^e at least 2^-100
+    V3 si = v3(ldexpf(s.inv.x, exp_i8(A.w, 0)), ldexpf(s.inv.y, exp_i8(A.w, 1)), ldexpf(s.inv.z, exp_i8(A.w, 2)));
     V3 of = v3(fmaf(org.x, s.inv.x, s.o.x), fmaf(org.y, s.inv.y, s.o.y), fmaf(org.z, s.inv.z, s.o.z));
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-      t[c][0] = fmaf(byte_f(Q.x, c), si.x, of.x); t[c][1] = fmaf(byte_f(Q.y, c), si.x, of.x);
-      t[c][2] = fmaf(byte_f(Q.z, c), si.y, of.y); t[c][3] = fmaf(byte_f(Q.w, c), si.y, of.y);
-      t[c][4] = fmaf(byte_f(R.x, c), si.z, of.z); t[c][5] = fmaf(byte_f(R.y, c), si.z, of.z);
+      float tn = fmaxf(fmaxf(fmaf(byte_f(near_x, c), si.x, of.x), fmaf(byte_f(near_y, c), si.y, of.y)),
+                       fmaxf(fmaf(byte_f(near_z, c), si.z, of.z), s.tmin));
+      float tf = fminf(fminf(fmaf(byte_f(far_x, c), si.x, of.x), fmaf(byte_f(far_y, c), si.y, of.y)),
+                       fminf(fmaf(byte_f(far_z, c), si.z, of.z), s.tmax));
+      d[c] = tn <= tf ? tn : INFINITY;
     }
   } else {
+    V3 sc = v3(ldexpf(1.f, exp_i8(A.w, 0)), ldexpf(1.f, exp_i8(A.w, 1)), ldexpf(1.f, exp_i8(A.w, 2)));
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-      t[c][0] = (fmaf(byte_f(Q.x, c), sc.x, org.x) - s.o.x) * s.inv.x;
-      t[c][1] = (fmaf(byte_f(Q.y, c), sc.x, org.x) - s.o.x) * s.inv.x;
-      t[c][2] = (fmaf(byte_f(Q.z, c), sc.y, org.y) - s.o.y) * s.inv.y;
-      t[c][3] = (fmaf(byte_f(Q.w, c), sc.y, org.y) - s.o.y) * s.inv.y;
-      t[c][4] = (fmaf(byte_f(R.x, c), sc.z, org.z) - s.o.z) * s.inv.z;
-      t[c][5] = (fmaf(byte_f(R.y, c), sc.z, org.z) - s.o.z) * s.inv.z;
+      float tn = fmaxf(fmaxf((fmaf(byte_f(near_x, c), sc.x, org.x) - s.o.x) * s.inv.x,
+                             (fmaf(byte_f(near_y, c), sc.y, org.y) - s.o.y) * s.inv.y),
+                       fmaxf((fmaf(byte_f(near_z, c), sc.z, org.z) - s.o.z) * s.inv.z, s.tmin));
+      float tf = fminf(fminf((fmaf(byte_f(far_x, c), sc.x, org.x) - s.o.x) * s.inv.x,
+                             (fmaf(byte_f(far_y, c), sc.y, org.y) - s.o.y) * s.inv.y),
+                       fminf((fmaf(byte_f(far_z, c), sc.z, org.z) - s.o.z) * s.inv.z, s.tmax));
+      d[c] = tn <= tf * ROBUST_FAR ? tn : INFINITY;
     }
   }
-  float d[4];
-#pragma unroll
-  for (int c = 0; c < 4; ++c) d[c] = slab_enter(t[c][0], t[c][1], t[c][2], t[c][3], t[c][4], t[c][5], s.tmin, s.tmax);
   d0 = d[0]; d1 = d[1]; d2 = d[2]; d3 = d[3];
 }
 
@@ -264,23 +279,19 @@ DG_INLINE void order_pair(float& da, uint32_t& ca, float& db, uint32_t& cb) {
 }
 
 // Visits inner node s.node.  BVH2: both child boxes in the parent node (64 B);
-// BVH4: compressed 64 B nodes.  s.node becomes the nearest child hit (the others pushed far
-// to near), or the next stacked node, or TRAV_END.  Any-hit rays (ANY) stop at the
-// first hit anywhere, so only the nearest child is picked (a three-comparison
-// tournament) and the other hit children are pushed unsorted.
+// BVH4: compressed 64 B nodes (A, Q, R, K), loaded by the caller.
+// s.node becomes the nearest child hit (the others pushed far to near), or the next
+// stacked node, or TRAV_END.  Any-hit rays (ANY) stop at the first hit anywhere, so
+// only the nearest child is picked (a three-comparison tournament) and the other hit
+// children are pushed unsorted.
 template <int WIDTH, bool ANY, bool FAST>
-DG_INLINE void trav_inner(const float4* __restrict__ nodes, TravState& s, LaneStack& st) {
+DG_INLINE void trav_inner(const float4* __restrict__ nodes, TravState& s, LaneStack& st, uint4 A, uint4 Q,
+                          uint4 R, uint4 K) {
   if (WIDTH == 4) {
     // compressed node (64 B): origin, per-axis scale 2^e, 8-bit child planes
-    const uint4* n = reinterpret_cast<const uint4*>(nodes) + 4 * (size_t)s.node;
-    uint4 A = n[0], Q = n[1], R = n[2], K = n[3];
     float d0, d1, d2, d3;
     decode_boxes<FAST>(A, Q, R, s, d0, d1, d2, d3);
     uint32_t c0 = R.z, c1 = R.w, c2 = K.x, c3 = K.y;
-    d0 = c0 == EMPTY_CHILD ? INFINITY : d0;
-    d1 = c1 == EMPTY_CHILD ? INFINITY : d1;
-    d2 = c2 == EMPTY_CHILD ? INFINITY : d2;
-    d3 = c3 == EMPTY_CHILD ? INFINITY : d3;
     // (0,1) (2,3) then the two winners: the nearest is in slot 0
     order_pair(d0, c0, d1, c1);
     order_pair(d2, c2, d3, c3);
@@ -326,20 +337,45 @@ DG_INLINE void trav_inner(const float4* __restrict__ nodes, TravState& s, LaneSt
 // an inner-node visit plus the longest leaf of the wave.
 // s.node becomes TRAV_END when the ray is finished (stack empty, or any-hit found),
 // or when it exceeds MAX_VISITS steps.
+// BVH4: the first 48 B of a node and a whole triangle are loaded through the same
+// three 16 B loads (each lane from its own address, node or triangle), so a wave
+// whose lanes hold both kinds issues 4 loads per step instead of 7.  The node's last
+// 8 B (child codes 2, 3) are loaded with them, the leaf lanes reading the root's (a
+// valid address every wave shares): loaded in the node branch, they are needed right
+// after the box test and their latency shows.  The addresses are computed with
+// arithmetic: written as a select of two pointers, the compiler makes it a divergent
+// branch.
 template <bool ANY, int WIDTH, bool FAST>
 DG_INLINE void trav_step(const float4* __restrict__ nodes, const float4* __restrict__ tris, TravState& s,
                          LaneStack& st) {
   if (++s.visits > MAX_VISITS) {
     s.node = TRAV_END;
-  } else if (!(s.node & LEAF_BIT)) {
-    trav_inner<WIDTH, ANY, FAST>(nodes, s, st);
-  } else if (leaf_test<ANY>(tris, s.node, s)) {
-    s.node = TRAV_END;
-  } else if (leaf_count(s.node) > 1u) {
-    s.node += 1u - (1u << 27);  // next triangle: first + 1, count - 1
-  } else {
-    trav_pop(s, st);
+    return;
   }
+  bool leaf = (s.node & LEAF_BIT) != 0u;
+  float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = a, c = a, k = a;
+  if (WIDTH == 4) {
+    // node: 4 x 16 B at 4 * node; triangle: 3 x 16 B at 3 * first (both indices < 2^27)
+    size_t idx = s.node & 0x07FFFFFFu;
+    const float4* p = (leaf ? tris : nodes) + ((idx << 2) - (leaf ? idx : 0u));
+    a = p[0]; b = p[1]; c = p[2];
+    k = nodes[(leaf ? 0u : (idx << 2)) + 3u];
+    // The triangle test reads only xyz and no k; without this the compiler moves the
+    // node-only words into the node branch as separate loads.  Takes values, no memory
+    // access.
+    asm volatile("" : : "v"(a.w), "v"(b.w), "v"(c.w), "v"(k.x), "v"(k.y));
+  }
+  if (!leaf) {
+    trav_inner<WIDTH, ANY, FAST>(nodes, s, st, f4_bits(a), f4_bits(b), f4_bits(c), f4_bits(k));
+    return;
+  }
+  if (WIDTH != 4) {
+    uint32_t ti = leaf_first(s.node);
+    a = tris[3 * ti]; b = tris[3 * ti + 1]; c = tris[3 * ti + 2];
+  }
+  if (leaf_tri<ANY>(a, b, c, s.node, s)) s.node = TRAV_END;
+  else if (leaf_count(s.node) > 1u) s.node += 1u - (1u << 27);  // next triangle: first + 1, count - 1
+  else trav_pop(s, st);
 }
 
 // Whole traversal of one ray; returns true on a hit (closest) / any hit (ANY).
