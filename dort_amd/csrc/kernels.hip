@@ -231,7 +231,6 @@ __global__ void k_advance(uint32_t* cur) {
 // bits 24..31 the scene of a batched path.
 constexpr uint32_t F_BOUNCES = 0xFFu;
 constexpr uint32_t F_DELTA = 1u << 8;
-constexpr uint32_t F_NOFILM = 1u << 9;
 
 // Chunk segment holding batched slot i (binary search over the segment starts).
 __device__ __forceinline__ const ChunkSeg& find_seg_slot(const ChunkDesc& cd, uint32_t i) {
@@ -276,15 +275,22 @@ __global__ void k_raygen(WaveParams P) {
     dgpu_rng_block(k0, k1, pixel, sample, 0, 2, r);
     jx = r[2]; jy = r[3];
   }
-  uint32_t flags = ((jx == 0.f || jy == 0.f) ? F_NOFILM : 0u) | (scene << SCENE_SHIFT);
   V3 o, d;
   camera_ray(*S, float(x) + jx, float(y) + jy, xr, yr, o, d);
+  // only the queue: bounce-0 shade starts the path state (throughput 1, radiance 0,
+  // pending NEE) without reading it, and finalize recomputes the jitter's edge test
   P.qo[0][i] = make_float4(o.x, o.y, o.z, 0.f);
   P.qd[0][i] = make_float4(d.x, d.y, d.z, INFINITY);
   P.qpath[0][i] = i | (scene << SCENE_SHIFT);
-  P.thr[i] = make_float4(1.f, 1.f, 1.f, __uint_as_float(flags));
-  P.rad[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-  for (uint32_t k = 0; k < P.slots; ++k) P.pend[(size_t)i * P.slots + k] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// A jitter of exactly 0 puts iteration `sample` (> 0) on its pixel's lower edge; dort's
+// film drops such samples (the jitter is k_raygen's: block 2 of bounce 0, words 2, 3).
+__device__ __forceinline__ bool jitter_on_edge(uint32_t k0, uint32_t k1, uint32_t pixel, uint32_t sample) {
+  if (sample == 0u) return false;
+  float r[4];
+  dgpu_rng_block(k0, k1, pixel, sample, 0, 2, r);
+  return r[2] == 0.f || r[3] == 0.f;
 }
 
 struct NeeOut {
@@ -446,15 +452,23 @@ __device__ __forceinline__ void shade_body(const WaveParams& P, const DevScene& 
   // the path, bounce and block), so fewer of them are live across the NEE code
   float rb0[4] = {0.f, 0.f, 0.f, 0.f}, rb1[4] = {0.f, 0.f, 0.f, 0.f}, rb2[4] = {0.f, 0.f, 0.f, 0.f};
   if (active) {
-    float4 t4 = P.thr[p], r4 = P.rad[p];
-    thr = xyz(t4);
-    rad = xyz(r4);
-    flags = __float_as_uint(t4.w);
-    // NEE of the previous vertex, now that its visibility is known
-    for (uint32_t k = 0; k < SL; ++k) {
-      size_t s = (size_t)p * SL + k;
-      float4 pe = P.pend[s];
-      if (pe.w != 0.f) { rad = rad + xyz(pe); P.pend[s].w = 0.f; }
+    if (bounce == 0u) {
+      // the path starts here (k_raygen writes only the queue)
+      thr = v3(1.f, 1.f, 1.f);
+      flags = scene << SCENE_SHIFT;
+      for (uint32_t k = 2u * (ALL ? P.S.num_lights : 1u); k < SL; ++k)  // slots no light writes
+        P.pend[(size_t)p * SL + k] = make_float4(0.f, 0.f, 0.f, 0.f);
+    } else {
+      float4 t4 = P.thr[p], r4 = P.rad[p];
+      thr = xyz(t4);
+      rad = xyz(r4);
+      flags = __float_as_uint(t4.w);
+      // NEE of the previous vertex, now that its visibility is known
+      for (uint32_t k = 0; k < SL; ++k) {
+        size_t s = (size_t)p * SL + k;
+        float4 pe = P.pend[s];
+        if (pe.w != 0.f) { rad = rad + xyz(pe); P.pend[s].w = 0.f; }
+      }
     }
     bounces = flags & F_BOUNCES;
     bool last_delta = (flags & F_DELTA) != 0u;
@@ -518,6 +532,8 @@ __device__ __forceinline__ void shade_body(const WaveParams& P, const DevScene& 
       if (nb.emit) {
         P.pend[slot_b] = make_float4(nb.contrib.x, nb.contrib.y, nb.contrib.z, 1.f);
         emit_shadow(P, nb, slot_b | scene_bits, qb);
+      } else if (active && bounce == 0u) {
+        P.pend[slot_b] = make_float4(0.f, 0.f, 0.f, 0.f);
       }
     }
     {
@@ -531,6 +547,8 @@ __device__ __forceinline__ void shade_body(const WaveParams& P, const DevScene& 
       if (nl.emit) {
         P.pend[slot_l] = make_float4(nl.contrib.x, nl.contrib.y, nl.contrib.z, 1.f);
         emit_shadow(P, nl, slot_l | scene_bits, ql);
+      } else if (active && bounce == 0u) {
+        P.pend[slot_l] = make_float4(0.f, 0.f, 0.f, 0.f);
       }
     }
   }
@@ -595,8 +613,9 @@ void k_shade(WaveParams P, uint32_t bounce) {
       uint32_t p = 0, scene = 0;
       uint2 ps = make_uint2(0u, 0u);
       if (active) {
-        p = P.qpath[cur][i] & LOW24;
-        scene = __float_as_uint(P.thr[p].w) >> SCENE_SHIFT;
+        uint32_t q = P.qpath[cur][i];
+        p = q & LOW24;
+        scene = q >> SCENE_SHIFT;
         ps = P.pid[p];
       }
       const BatchEntry& E = P.batch[scene];
@@ -617,7 +636,8 @@ void k_shade(WaveParams P, uint32_t bounce) {
 // depend on how paths were split into chunks or batched with other scenes.
 __device__ __forceinline__ void film_accumulate(const WaveParams& P, float* film_rgb, float* film_w,
                                                 uint32_t px, uint32_t slot, uint32_t npix,
-                                                uint64_t local, uint64_t local_end) {
+                                                uint64_t local, uint64_t local_end, uint32_t sample_begin,
+                                                uint32_t k0, uint32_t k1) {
   float fr = film_rgb[3 * (size_t)px], fg = film_rgb[3 * (size_t)px + 1], fb = film_rgb[3 * (size_t)px + 2];
   float fw = film_w[px];
   uint32_t dropped = 0;
@@ -633,8 +653,7 @@ __device__ __forceinline__ void film_accumulate(const WaveParams& P, float* film
     bool ok = isfinite(rad.x) && isfinite(rad.y) && isfinite(rad.z) && rad.x >= 0.f &&
               rad.y >= 0.f && rad.z >= 0.f;
     if (!ok) { ++dropped; continue; }
-    uint32_t flags = __float_as_uint(P.thr[slot].w);
-    if (flags & F_NOFILM) continue;
+    if (jitter_on_edge(k0, k1, px, sample_begin + (uint32_t)(local / npix))) continue;
     fr += rad.x; fg += rad.y; fb += rad.z;
     fw += 1.f;
     touched = true;
@@ -681,7 +700,8 @@ __global__ __launch_bounds__(256) void k_finalize(WaveParams P) {
     uint32_t q = t - sg.pair_begin;
     uint64_t local = sg.local_begin + q;
     uint32_t px = (uint32_t)(local % E.npix);
-    film_accumulate(P, E.film_rgb, E.film_w, px, sg.slot_begin + q, E.npix, local, sg.local_end);
+    film_accumulate(P, E.film_rgb, E.film_w, px, sg.slot_begin + q, E.npix, local, sg.local_end, E.sample_begin,
+                    E.k0, E.k1);
   } else {
     add_chunk_stats(P, P.n_chunk);
     uint32_t px = t;
@@ -689,7 +709,8 @@ __global__ __launch_bounds__(256) void k_finalize(WaveParams P) {
     uint64_t g_end = P.g0 + P.n_chunk;
     uint64_t g = P.g0 + ((px + P.npix - (uint32_t)(P.g0 % P.npix)) % P.npix);
     if (g >= g_end) return;
-    film_accumulate(P, P.film_rgb, P.film_w, px, (uint32_t)(g - P.g0), P.npix, g, g_end);
+    film_accumulate(P, P.film_rgb, P.film_w, px, (uint32_t)(g - P.g0), P.npix, g, g_end, P.sample_begin, P.k0,
+                    P.k1);
   }
 }
 
