@@ -95,7 +95,8 @@ DG_INLINE float row_dot(V3 r, V3 a) { return fmaf(r.z, a.z, fmaf(r.y, a.y, r.x *
 // Contraction into FMA is off here: an edge shared by two triangles must give edge
 // functions of exactly opposite sign, which needs both products rounded the same way.
 // The range test runs first on T against det * [tmin, tmax] (widened by 2^-20), so
-// the reciprocal is taken only for candidates; the final test is on t itself.
+// the reciprocal is taken only for candidates; the final test is on t itself.  The
+// reciprocal is the hardware's (1 ulp) rather than a correctly rounded division.
 DG_INLINE bool wt_hit(const WtRay& w, float4 va, float4 vb, float4 vc, float tmin, float tmax,
                       float& t_out, float& b1_out, float& b2_out) {
 #pragma clang fp contract(off)
@@ -111,7 +112,7 @@ DG_INLINE bool wt_hit(const WtRay& w, float4 va, float4 vb, float4 vc, float tmi
     V = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
     W = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
   }
-  if ((U < 0.f || V < 0.f || W < 0.f) && (U > 0.f || V > 0.f || W > 0.f)) return false;
+  if (fminf(fminf(U, V), W) < 0.f && fmaxf(fmaxf(U, V), W) > 0.f) return false;  // mixed signs
   float det = U + V + W;
   if (det == 0.f) return false;
   float Az = row_dot(w.rz, A), Bz = row_dot(w.rz, B), Cz = row_dot(w.rz, C);
@@ -119,7 +120,7 @@ DG_INLINE bool wt_hit(const WtRay& w, float4 va, float4 vb, float4 vc, float tmi
   uint32_t sgn = __float_as_uint(det) & 0x80000000u;
   float Tn = __uint_as_float(__float_as_uint(T) ^ sgn), dn = fabsf(det);
   if (Tn > tmax * dn * 1.000001f || Tn < tmin * dn * (tmin > 0.f ? 0.999999f : 1.000001f)) return false;
-  float inv = 1.0f / det;
+  float inv = __builtin_amdgcn_rcpf(det);  // 1 ulp (v_rcp_f32); t is compared to 1e-5 relative
   float t = T * inv;
   if (!(t >= tmin && t <= tmax)) return false;
   t_out = t;
