@@ -1,4 +1,4 @@
-// io.cpp -- scene and image files for the drop-in: PLY meshes and Radiance RGBE images.
+// io.cpp -- scene and image files for the drop-in: PLY meshes, Radiance RGBE and PNG images.
 //
 // Replaces dort's readers/writers at the two ends of the pt path:
 //  * read_ply_callbacks / read_ply_to_mesh (src/dort/ply_mesh.cpp:9-103, over rply):
@@ -11,8 +11,11 @@
 //    rule (mantissa of the largest channel scaled to 256).  Like stb, a file whose first
 //    scanline is not run-length encoded is read as flat pixels; unlike stb, a run that
 //    would overflow its scanline is an error, and "#?RGBE" headers are accepted too.
-// Written from the PLY format description (Turk, 1994) and the Radiance picture format
-// (Ward, "Real pixels", Graphics Gems II, 1991).
+//  * read_image_8 / write_image_png (src/dort/image.cpp:14-26, 42-51, over stb_image):
+//    PNG of any colour type and bit depth read into 8-bit RGB; 8-bit RGB written.
+// Written from the PLY format description (Turk, 1994), the Radiance picture format
+// (Ward, "Real pixels", Graphics Gems II, 1991), the PNG specification (W3C, 2003) and
+// RFC 1950 / 1951 (zlib, deflate).
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -396,6 +399,506 @@ dgpu_status dgpu_write_rgbe(const char* path, const float* rgb, uint32_t width, 
     }
     if (std::fwrite(enc.data(), 1, enc.size(), f) != enc.size()) return fail(DGPU_EINVAL, "write failed");
   }
+  return DGPU_OK;
+}
+
+}  // extern "C"
+
+// ----------------------------------------------------------------- PNG
+
+namespace {
+
+// Deflate (RFC 1951) decoding over a byte range: stored, fixed-Huffman and
+// dynamic-Huffman blocks.  Huffman codes are canonical; a symbol is decoded bit by bit
+// against the count of codes of each length (codes of one length are consecutive
+// integers, RFC 1951 3.2.2).
+struct Inflater {
+  const uint8_t* in;
+  size_t n, pos = 0;
+  uint32_t bitbuf = 0;
+  int bitcnt = 0;
+  std::vector<uint8_t>& out;
+  bool bad = false;
+
+  Inflater(const uint8_t* d, size_t len, std::vector<uint8_t>& o) : in(d), n(len), out(o) {}
+
+  uint32_t bits(int k) {  // k <= 16, LSB first
+    while (bitcnt < k) {
+      if (pos >= n) { bad = true; return 0; }
+      bitbuf |= (uint32_t)in[pos++] << bitcnt;
+      bitcnt += 8;
+    }
+    uint32_t v = bitbuf & ((1u << k) - 1u);
+    bitbuf >>= k;
+    bitcnt -= k;
+    return v;
+  }
+
+  struct Huff {
+    uint16_t count[16];
+    uint16_t symbol[320];
+  };
+
+  // Returns false for an over-subscribed code; incomplete codes are allowed (a lone
+  // distance code is legal) and fail only if an unused code turns up.
+  static bool build(Huff& h, const uint8_t* len, int nsym) {
+    std::memset(h.count, 0, sizeof h.count);
+    for (int s = 0; s < nsym; ++s) h.count[len[s]]++;
+    h.count[0] = 0;
+    int left = 1;
+    for (int l = 1; l < 16; ++l) {
+      left = 2 * left - h.count[l];
+      if (left < 0) return false;
+    }
+    uint16_t offs[16];
+    offs[1] = 0;
+    for (int l = 1; l < 15; ++l) offs[l + 1] = (uint16_t)(offs[l] + h.count[l]);
+    for (int s = 0; s < nsym; ++s)
+      if (len[s]) h.symbol[offs[len[s]]++] = (uint16_t)s;
+    return true;
+  }
+
+  int decode(const Huff& h) {
+    int code = 0, first = 0, index = 0;
+    for (int l = 1; l < 16; ++l) {
+      code |= (int)bits(1);
+      if (bad) return -1;
+      int c = h.count[l];
+      if (code - first < c) return h.symbol[index + code - first];
+      index += c;
+      first = (first + c) << 1;
+      code <<= 1;
+    }
+    bad = true;
+    return -1;
+  }
+
+  bool codes(const Huff& lit, const Huff& dist) {
+    // RFC 1951 3.2.5: base lengths / distances and extra bits of symbols 257.. / 0..
+    static const uint16_t lbase[29] = {3, 4, 5, 6, 7, 8, 9, 10, 11, 13, 15, 17, 19, 23, 27, 31,
+                                       35, 43, 51, 59, 67, 83, 99, 115, 131, 163, 195, 227, 258};
+    static const uint8_t lext[29] = {0, 0, 0, 0, 0, 0, 0, 0, 1, 1, 1, 1, 2, 2, 2, 2,
+                                     3, 3, 3, 3, 4, 4, 4, 4, 5, 5, 5, 5, 0};
+    static const uint16_t dbase[30] = {1, 2, 3, 4, 5, 7, 9, 13, 17, 25, 33, 49, 65, 97, 129,
+                                       193, 257, 385, 513, 769, 1025, 1537, 2049, 3073, 4097,
+                                       6145, 8193, 12289, 16385, 24577};
+    static const uint8_t dext[30] = {0, 0, 0, 0, 1, 1, 2, 2, 3, 3, 4, 4, 5, 5, 6,
+                                     6, 7, 7, 8, 8, 9, 9, 10, 10, 11, 11, 12, 12, 13, 13};
+    for (;;) {
+      int sym = decode(lit);
+      if (sym < 0) return false;
+      if (sym < 256) { out.push_back((uint8_t)sym); continue; }
+      if (sym == 256) return true;
+      sym -= 257;
+      if (sym >= 29) return false;
+      size_t len = lbase[sym] + bits(lext[sym]);
+      int ds = decode(dist);
+      if (ds < 0 || ds >= 30) return false;
+      size_t d = dbase[ds] + bits(dext[ds]);
+      if (bad || d > out.size()) return false;
+      size_t from = out.size() - d;
+      for (size_t k = 0; k < len; ++k) out.push_back(out[from + k]);
+    }
+  }
+
+  bool run() {
+    int last;
+    do {
+      last = (int)bits(1);
+      int type = (int)bits(2);
+      if (bad) return false;
+      if (type == 0) {
+        bitbuf = 0;  // stored: skip to a byte boundary
+        bitcnt = 0;
+        if (pos + 4 > n) return false;
+        uint32_t len = in[pos] | in[pos + 1] << 8, nlen = in[pos + 2] | in[pos + 3] << 8;
+        pos += 4;
+        if (len != (~nlen & 0xFFFFu) || pos + len > n) return false;
+        out.insert(out.end(), in + pos, in + pos + len);
+        pos += len;
+      } else if (type == 1) {
+        struct Fixed {  // RFC 1951 3.2.6
+          Huff lit, dist;
+          Fixed() {
+            uint8_t l[288];
+            for (int s = 0; s < 288; ++s) l[s] = s < 144 ? 8 : s < 256 ? 9 : s < 280 ? 7 : 8;
+            build(lit, l, 288);
+            for (int s = 0; s < 30; ++s) l[s] = 5;
+            build(dist, l, 30);
+          }
+        };
+        static const Fixed fixed;  // initialised once, thread-safe
+        if (!codes(fixed.lit, fixed.dist)) return false;
+      } else if (type == 2) {
+        int hlit = (int)bits(5) + 257, hdist = (int)bits(5) + 1, hclen = (int)bits(4) + 4;
+        if (bad || hlit > 286 || hdist > 30) return false;
+        static const uint8_t order[19] = {16, 17, 18, 0, 8, 7, 9, 6, 10, 5, 11, 4, 12, 3, 13, 2, 14, 1, 15};
+        uint8_t cl[19] = {0};
+        for (int k = 0; k < hclen; ++k) cl[order[k]] = (uint8_t)bits(3);
+        Huff hc;
+        if (bad || !build(hc, cl, 19)) return false;
+        uint8_t l[320] = {0};
+        int k = 0;
+        while (k < hlit + hdist) {
+          int sym = decode(hc);
+          if (sym < 0) return false;
+          if (sym < 16) { l[k++] = (uint8_t)sym; continue; }
+          int rep = 0;
+          uint8_t v = 0;
+          if (sym == 16) {
+            if (k == 0) return false;
+            v = l[k - 1];
+            rep = 3 + (int)bits(2);
+          } else if (sym == 17) {
+            rep = 3 + (int)bits(3);
+          } else {
+            rep = 11 + (int)bits(7);
+          }
+          if (bad || k + rep > hlit + hdist) return false;
+          while (rep--) l[k++] = v;
+        }
+        if (l[256] == 0) return false;  // no end-of-block code
+        Huff lit, dist;
+        if (!build(lit, l, hlit) || !build(dist, l + hlit, hdist)) return false;
+        if (!codes(lit, dist)) return false;
+      } else {
+        return false;
+      }
+    } while (!last);
+    return true;
+  }
+};
+
+uint32_t be32(const uint8_t* p) { return (uint32_t)p[0] << 24 | (uint32_t)p[1] << 16 | (uint32_t)p[2] << 8 | p[3]; }
+
+const uint32_t* crc_table() {  // CRC-32, reflected polynomial 0xEDB88320 (PNG spec annex D)
+  struct Table {
+    uint32_t t[256];
+    Table() {
+      for (uint32_t n = 0; n < 256; ++n) {
+        uint32_t c = n;
+        for (int k = 0; k < 8; ++k) c = (c & 1u) ? 0xEDB88320u ^ (c >> 1) : c >> 1;
+        t[n] = c;
+      }
+    }
+  };
+  static const Table table;  // initialised once, thread-safe
+  return table.t;
+}
+
+uint32_t crc32(uint32_t crc, const uint8_t* p, size_t n) {
+  const uint32_t* t = crc_table();
+  crc = ~crc;
+  for (size_t k = 0; k < n; ++k) crc = t[(crc ^ p[k]) & 0xFFu] ^ (crc >> 8);
+  return ~crc;
+}
+
+uint8_t paeth(int a, int b, int c) {
+  int p = a + b - c, pa = std::abs(p - a), pb = std::abs(p - b), pc = std::abs(p - c);
+  return (uint8_t)(pa <= pb && pa <= pc ? a : pb <= pc ? b : c);
+}
+
+// Undoes the filter of one scanline in place (PNG spec 9.2); prior is the previous
+// scanline of the same pass, unfiltered (zeros for the first).
+bool unfilter(int type, uint8_t* row, const uint8_t* prior, size_t len, size_t bpp) {
+  switch (type) {
+    case 0: return true;
+    case 1: for (size_t i = bpp; i < len; ++i) row[i] = (uint8_t)(row[i] + row[i - bpp]); return true;
+    case 2: for (size_t i = 0; i < len; ++i) row[i] = (uint8_t)(row[i] + prior[i]); return true;
+    case 3:
+      for (size_t i = 0; i < len; ++i)
+        row[i] = (uint8_t)(row[i] + (((i >= bpp ? row[i - bpp] : 0) + prior[i]) >> 1));
+      return true;
+    case 4:
+      for (size_t i = 0; i < len; ++i)
+        row[i] = (uint8_t)(row[i] + paeth(i >= bpp ? row[i - bpp] : 0, prior[i], i >= bpp ? prior[i - bpp] : 0));
+      return true;
+    default: return false;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+// Reads a PNG into 8-bit RGB as stb_image's stbi_load(..., 3) does for dort's
+// read_image_8 (src/dort/image.cpp:14-26): every colour type and bit depth, Adam7
+// interlacing; 16-bit samples keep their high byte, grey of 1, 2 or 4 bits is scaled
+// to 0..255 (x255, x85, x17), palette indices are looked up, grey is replicated and
+// alpha (and tRNS) dropped.  Ancillary chunks are skipped, an unknown critical chunk
+// is an error; CRCs and the zlib checksum are not checked (stb does not either).
+dgpu_status dgpu_read_png(const char* path, uint8_t** rgb, uint32_t* width, uint32_t* height) {
+  if (!path || !rgb || !width || !height) return fail(DGPU_EINVAL, "null argument");
+  *rgb = nullptr;
+  *width = 0;
+  *height = 0;
+  std::vector<uint8_t> file;
+  {
+    File f(path, "rb");
+    if (!f.f) return fail(DGPU_EINVAL, "cannot open %s", path);
+    uint8_t buf[65536];
+    size_t got;
+    while ((got = std::fread(buf, 1, sizeof buf, f.f)) > 0) file.insert(file.end(), buf, buf + got);
+  }
+  static const uint8_t sig[8] = {137, 80, 78, 71, 13, 10, 26, 10};
+  if (file.size() < 8 || std::memcmp(file.data(), sig, 8) != 0) return fail(DGPU_EINVAL, "%s: not a PNG file", path);
+  uint32_t w = 0, h = 0;
+  int depth = 0, ctype = -1, interlace = 0;
+  std::vector<uint8_t> idat, pal;
+  bool seen_ihdr = false, seen_iend = false;
+  size_t p = 8;
+  while (p + 8 <= file.size()) {
+    uint32_t len = be32(&file[p]);
+    const uint8_t* type = &file[p + 4];
+    if (len > file.size() - p - 8 || file.size() - p - 8 - len < 4) return fail(DGPU_EINVAL, "%s: truncated chunk", path);
+    const uint8_t* d = &file[p + 8];
+    if (!seen_ihdr && std::memcmp(type, "IHDR", 4) != 0) return fail(DGPU_EINVAL, "%s: IHDR is not first", path);
+    if (std::memcmp(type, "IHDR", 4) == 0) {
+      if (len != 13 || seen_ihdr) return fail(DGPU_EINVAL, "%s: bad IHDR", path);
+      w = be32(d);
+      h = be32(d + 4);
+      depth = d[8];
+      ctype = d[9];
+      interlace = d[12];
+      bool ok_depth = (ctype == 0 && (depth == 1 || depth == 2 || depth == 4 || depth == 8 || depth == 16)) ||
+                      (ctype == 3 && (depth == 1 || depth == 2 || depth == 4 || depth == 8)) ||
+                      ((ctype == 2 || ctype == 4 || ctype == 6) && (depth == 8 || depth == 16));
+      if (!ok_depth || d[10] != 0 || d[11] != 0 || interlace > 1)
+        return fail(DGPU_EINVAL, "%s: bad IHDR fields", path);
+      if (w == 0 || h == 0 || w > (1u << 24) || h > (1u << 24) || (uint64_t)w * h > (1ull << 31))
+        return fail(DGPU_EINVAL, "%s: bad image size", path);
+      seen_ihdr = true;
+    } else if (std::memcmp(type, "PLTE", 4) == 0) {
+      if (len % 3 != 0 || len > 768 || len == 0) return fail(DGPU_EINVAL, "%s: bad PLTE", path);
+      pal.assign(d, d + len);
+    } else if (std::memcmp(type, "IDAT", 4) == 0) {
+      idat.insert(idat.end(), d, d + len);
+    } else if (std::memcmp(type, "IEND", 4) == 0) {
+      seen_iend = true;
+      break;
+    } else if (!(type[0] & 0x20)) {
+      return fail(DGPU_EUNSUPPORTED, "%s: unknown critical chunk %.4s", path, (const char*)type);
+    }
+    p += 12 + (size_t)len;
+  }
+  if (!seen_ihdr || idat.empty()) return fail(DGPU_EINVAL, "%s: no image data", path);
+  (void)seen_iend;  // stb reads images whose IEND is missing
+  if (ctype == 3 && pal.empty()) return fail(DGPU_EINVAL, "%s: palette image without PLTE", path);
+  // zlib stream (RFC 1950): CM 8, no preset dictionary
+  if (idat.size() < 2 || (idat[0] & 0x0F) != 8 || ((idat[0] << 8) | idat[1]) % 31 != 0 || (idat[1] & 0x20))
+    return fail(DGPU_EINVAL, "%s: bad zlib header", path);
+  std::vector<uint8_t> raw;
+  Inflater inf(idat.data() + 2, idat.size() - 2, raw);
+  if (!inf.run()) return fail(DGPU_EINVAL, "%s: corrupt image data", path);
+
+  const int channels = ctype == 0 ? 1 : ctype == 2 ? 3 : ctype == 3 ? 1 : ctype == 4 ? 2 : 4;
+  const size_t bits_pp = (size_t)channels * depth, bpp = std::max<size_t>(1, bits_pp / 8);
+  std::vector<uint8_t> out((size_t)w * h * 3);
+  auto put = [&](uint32_t x, uint32_t y, const uint8_t* row, uint32_t k) {  // pixel k of an unfiltered row
+    uint8_t* o = &out[3 * ((size_t)y * w + x)];
+    auto sample = [&](int c) -> uint32_t {  // 8-bit value of channel c (or the index for palettes)
+      if (depth == 16) return row[2 * ((size_t)k * channels + c)];
+      if (depth == 8) return row[(size_t)k * channels + c];
+      size_t bit = (size_t)k * depth;
+      uint32_t v = (row[bit >> 3] >> (8 - depth - (bit & 7))) & ((1u << depth) - 1u);
+      if (ctype == 3) return v;
+      return v * (depth == 1 ? 255u : depth == 2 ? 85u : 17u);
+    };
+    if (ctype == 3) {
+      uint32_t i = sample(0);
+      if (3 * i + 2 >= pal.size()) { o[0] = o[1] = o[2] = 0; return; }
+      o[0] = pal[3 * i]; o[1] = pal[3 * i + 1]; o[2] = pal[3 * i + 2];
+    } else if (ctype == 0 || ctype == 4) {
+      o[0] = o[1] = o[2] = (uint8_t)sample(0);
+    } else {
+      o[0] = (uint8_t)sample(0); o[1] = (uint8_t)sample(1); o[2] = (uint8_t)sample(2);
+    }
+  };
+  // Adam7 passes (PNG spec 8.2), or one pass for a non-interlaced image
+  static const uint32_t ax0[7] = {0, 4, 0, 2, 0, 1, 0}, ay0[7] = {0, 0, 4, 0, 2, 0, 1};
+  static const uint32_t adx[7] = {8, 8, 4, 4, 2, 2, 1}, ady[7] = {8, 8, 8, 4, 4, 2, 2};
+  size_t at = 0;
+  for (int pass = 0; pass < (interlace ? 7 : 1); ++pass) {
+    uint32_t x0 = interlace ? ax0[pass] : 0, y0 = interlace ? ay0[pass] : 0;
+    uint32_t dx = interlace ? adx[pass] : 1, dy = interlace ? ady[pass] : 1;
+    if (x0 >= w || y0 >= h) continue;
+    uint32_t pw = (w - x0 + dx - 1) / dx, ph = (h - y0 + dy - 1) / dy;
+    size_t rowlen = ((size_t)pw * bits_pp + 7) / 8;
+    std::vector<uint8_t> prior(rowlen, 0), row(rowlen);
+    for (uint32_t r = 0; r < ph; ++r) {
+      if (at + 1 + rowlen > raw.size()) return fail(DGPU_EINVAL, "%s: image data too short", path);
+      int ft = raw[at];
+      std::memcpy(row.data(), &raw[at + 1], rowlen);
+      at += 1 + rowlen;
+      if (!unfilter(ft, row.data(), prior.data(), rowlen, bpp)) return fail(DGPU_EINVAL, "%s: bad filter type", path);
+      for (uint32_t k = 0; k < pw; ++k) put(x0 + k * dx, y0 + r * dy, row.data(), k);
+      std::swap(prior, row);
+    }
+  }
+  uint8_t* o = (uint8_t*)std::malloc(out.size());
+  if (!o) return fail(DGPU_ENOMEM, "out of host memory");
+  std::memcpy(o, out.data(), out.size());
+  *rgb = o;
+  *width = w;
+  *height = h;
+  return DGPU_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Deflate with the fixed Huffman code (RFC 1951 3.2.6) and greedy LZ77 matching over a
+// 32 KiB window (the most recent earlier position with the same 3 bytes, then up to
+// 15 older ones along a hash chain).
+struct Deflater {
+  std::vector<uint8_t>& out;
+  uint32_t bitbuf = 0;
+  int bitcnt = 0;
+  explicit Deflater(std::vector<uint8_t>& o) : out(o) {}
+
+  void put(uint32_t v, int k) {  // LSB first
+    bitbuf |= v << bitcnt;
+    bitcnt += k;
+    while (bitcnt >= 8) {
+      out.push_back((uint8_t)bitbuf);
+      bitbuf >>= 8;
+      bitcnt -= 8;
+    }
+  }
+  void put_code(uint32_t code, int len) {  // Huffman codes go MSB first
+    uint32_t r = 0;
+    for (int k = 0; k < len; ++k) r |= ((code >> k) & 1u) << (len - 1 - k);
+    put(r, len);
+  }
+  void literal(int s) {
+    if (s < 144) put_code(0x30 + s, 8);
+    else if (s < 256) put_code(0x190 + s - 144, 9);
+    else if (s < 280) put_code(s - 256, 7);
+    else put_code(0xC0 + s - 280, 8);
+  }
+  void match(int len, int dist) {
+    static const uint16_t lbase[29] = {3, 4, 5, 6, 7, 8, 9, 10, 11, 13, 15, 17, 19, 23, 27, 31,
+                                       35, 43, 51, 59, 67, 83, 99, 115, 131, 163, 195, 227, 258};
+    static const uint8_t lext[29] = {0, 0, 0, 0, 0, 0, 0, 0, 1, 1, 1, 1, 2, 2, 2, 2,
+                                     3, 3, 3, 3, 4, 4, 4, 4, 5, 5, 5, 5, 0};
+    static const uint16_t dbase[30] = {1, 2, 3, 4, 5, 7, 9, 13, 17, 25, 33, 49, 65, 97, 129,
+                                       193, 257, 385, 513, 769, 1025, 1537, 2049, 3073, 4097,
+                                       6145, 8193, 12289, 16385, 24577};
+    static const uint8_t dext[30] = {0, 0, 0, 0, 1, 1, 2, 2, 3, 3, 4, 4, 5, 5, 6,
+                                     6, 7, 7, 8, 8, 9, 9, 10, 10, 11, 11, 12, 12, 13, 13};
+    int ls = 28;
+    while (lbase[ls] > len) --ls;
+    literal(257 + ls);
+    put((uint32_t)(len - lbase[ls]), lext[ls]);
+    int ds = 29;
+    while (dbase[ds] > dist) --ds;
+    put_code((uint32_t)ds, 5);
+    put((uint32_t)(dist - dbase[ds]), dext[ds]);
+  }
+  void compress(const uint8_t* d, size_t n) {
+    put(1, 1);  // one final block
+    put(1, 2);  // fixed Huffman
+    const int HBITS = 15, WIN = 32768, CHAIN = 16;
+    std::vector<int32_t> head(1u << HBITS, -1), prev(n, -1);
+    auto hash = [&](size_t i) { return ((uint32_t)d[i] << 10 ^ (uint32_t)d[i + 1] << 5 ^ d[i + 2]) & ((1u << HBITS) - 1u); };
+    size_t i = 0;
+    while (i < n) {
+      int best = 0, bdist = 0;
+      if (i + 2 < n) {
+        uint32_t hv = hash(i);
+        int32_t c = head[hv];
+        for (int k = 0; k < CHAIN && c >= 0 && (int64_t)i - c <= WIN; ++k, c = prev[c]) {
+          int l = 0, maxl = (int)std::min<size_t>(258, n - i);
+          while (l < maxl && d[c + l] == d[i + l]) ++l;
+          if (l > best) { best = l; bdist = (int)(i - c); if (l == maxl) break; }
+        }
+        prev[i] = head[hv];
+        head[hv] = (int32_t)i;
+      }
+      if (best >= 3) {
+        match(best, bdist);
+        for (size_t k = i + 1; k < i + (size_t)best && k + 2 < n; ++k) {
+          uint32_t hv = hash(k);
+          prev[k] = head[hv];
+          head[hv] = (int32_t)k;
+        }
+        i += best;
+      } else {
+        literal(d[i]);
+        ++i;
+      }
+    }
+    literal(256);
+    if (bitcnt) put(0, 8 - bitcnt);
+  }
+};
+
+void chunk(std::vector<uint8_t>& f, const char* type, const uint8_t* d, size_t n) {
+  uint8_t hdr[8] = {(uint8_t)(n >> 24), (uint8_t)(n >> 16), (uint8_t)(n >> 8), (uint8_t)n,
+                    (uint8_t)type[0], (uint8_t)type[1], (uint8_t)type[2], (uint8_t)type[3]};
+  f.insert(f.end(), hdr, hdr + 8);
+  f.insert(f.end(), d, d + n);
+  uint32_t c = crc32(crc32(0, hdr + 4, 4), d, n);
+  uint8_t t[4] = {(uint8_t)(c >> 24), (uint8_t)(c >> 16), (uint8_t)(c >> 8), (uint8_t)c};
+  f.insert(f.end(), t, t + 4);
+}
+
+}  // namespace
+
+extern "C" {
+
+// Writes 8-bit RGB as a PNG (as dort's write_image_png, src/dort/image.cpp:42-51, over
+// stb_image_write): colour type 2, not interlaced; each row takes the filter with the
+// smallest sum of absolute filtered bytes (the PNG spec's suggested heuristic), the
+// filtered rows are deflated with the fixed Huffman code, zlib-wrapped with Adler-32.
+dgpu_status dgpu_write_png(const char* path, const uint8_t* rgb, uint32_t width, uint32_t height) {
+  if (!path || !rgb) return fail(DGPU_EINVAL, "null argument");
+  if (width == 0 || height == 0 || width > (1u << 24) || height > (1u << 24) || (uint64_t)width * height > (1ull << 31))
+    return fail(DGPU_EINVAL, "bad image size");
+  const size_t rowlen = (size_t)width * 3;
+  std::vector<uint8_t> filtered;
+  filtered.reserve((rowlen + 1) * height);
+  std::vector<uint8_t> zero(rowlen, 0), cand(rowlen), best(rowlen);
+  for (uint32_t y = 0; y < height; ++y) {
+    const uint8_t* row = rgb + (size_t)y * rowlen;
+    const uint8_t* prior = y ? row - rowlen : zero.data();
+    uint64_t best_sum = ~0ull;
+    int best_type = 0;
+    for (int t = 0; t < 5; ++t) {
+      uint64_t sum = 0;
+      for (size_t i = 0; i < rowlen; ++i) {
+        int a = i >= 3 ? row[i - 3] : 0, b = prior[i], c = i >= 3 ? prior[i - 3] : 0;
+        int pred = t == 0 ? 0 : t == 1 ? a : t == 2 ? b : t == 3 ? (a + b) >> 1 : paeth(a, b, c);
+        cand[i] = (uint8_t)(row[i] - pred);
+        sum += (uint64_t)std::abs((int)(int8_t)cand[i]);
+      }
+      if (sum < best_sum) { best_sum = sum; best_type = t; best.swap(cand); }
+    }
+    filtered.push_back((uint8_t)best_type);
+    filtered.insert(filtered.end(), best.begin(), best.end());
+  }
+  std::vector<uint8_t> z = {0x78, 0x01};  // CM 8, 32 KiB window, FCHECK: 0x7801 % 31 == 0
+  Deflater(z).compress(filtered.data(), filtered.size());
+  uint32_t s1 = 1, s2 = 0;  // Adler-32 (RFC 1950 8.2)
+  for (size_t i = 0; i < filtered.size(); ++i) {
+    s1 = (s1 + filtered[i]) % 65521u;
+    s2 = (s2 + s1) % 65521u;
+  }
+  uint32_t ad = s2 << 16 | s1;
+  uint8_t adb[4] = {(uint8_t)(ad >> 24), (uint8_t)(ad >> 16), (uint8_t)(ad >> 8), (uint8_t)ad};
+  z.insert(z.end(), adb, adb + 4);
+  std::vector<uint8_t> f = {137, 80, 78, 71, 13, 10, 26, 10};
+  uint8_t ihdr[13] = {(uint8_t)(width >> 24), (uint8_t)(width >> 16), (uint8_t)(width >> 8), (uint8_t)width,
+                      (uint8_t)(height >> 24), (uint8_t)(height >> 16), (uint8_t)(height >> 8), (uint8_t)height,
+                      8, 2, 0, 0, 0};
+  chunk(f, "IHDR", ihdr, 13);
+  for (size_t off = 0; off < z.size(); off += (1u << 20))  // IDAT chunks of at most 1 MiB
+    chunk(f, "IDAT", z.data() + off, std::min<size_t>(1u << 20, z.size() - off));
+  chunk(f, "IEND", nullptr, 0);
+  File file(path, "wb");
+  if (!file.f) return fail(DGPU_EINVAL, "cannot open %s for writing", path);
+  if (std::fwrite(f.data(), 1, f.size(), file.f) != f.size()) return fail(DGPU_EINVAL, "write failed");
   return DGPU_OK;
 }
 

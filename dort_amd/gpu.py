@@ -20,8 +20,8 @@ EXPORTS = ["dgpu_abi_version", "dgpu_last_error", "dgpu_create", "dgpu_destroy",
            "dgpu_device_free", "dgpu_copy", "dgpu_scene_create", "dgpu_scene_destroy",
            "dgpu_scene_get_info", "dgpu_scene_update_texture", "dgpu_scene_update_material",
            "dgpu_trace_closest", "dgpu_trace_any", "dgpu_render", "dgpu_render_batch", "dgpu_adjoint",
-           "dgpu_read_ply", "dgpu_free", "dgpu_read_rgbe", "dgpu_write_rgbe", "dgpu_test_convergence",
-           "dgpu_bias_variance"]
+           "dgpu_read_ply", "dgpu_free", "dgpu_read_rgbe", "dgpu_write_rgbe", "dgpu_read_png", "dgpu_write_png",
+           "dgpu_test_convergence", "dgpu_bias_variance"]
 
 
 class DgpuError(RuntimeError):
@@ -66,14 +66,17 @@ def lib():
         L.dgpu_read_rgbe.argtypes = [C.c_char_p, C.POINTER(C.POINTER(C.c_float)), C.POINTER(C.c_uint32),
                                      C.POINTER(C.c_uint32)]
         L.dgpu_write_rgbe.argtypes = [C.c_char_p, C.c_void_p, C.c_uint32, C.c_uint32]
+        L.dgpu_read_png.argtypes = [C.c_char_p, C.POINTER(C.POINTER(C.c_uint8)), C.POINTER(C.c_uint32),
+                                    C.POINTER(C.c_uint32)]
+        L.dgpu_write_png.argtypes = [C.c_char_p, C.c_void_p, C.c_uint32, C.c_uint32]
         L.dgpu_test_convergence.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_int32,
                                             C.c_float, C.c_float, C.c_float, C.c_uint32, C.POINTER(C.c_int32),
                                             C.c_char_p, C.c_size_t]
         L.dgpu_bias_variance.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32,
                                          C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(C.c_double),
                                          C.POINTER(C.c_double)]
-        for n in ("dgpu_read_ply", "dgpu_read_rgbe", "dgpu_write_rgbe", "dgpu_test_convergence",
-                  "dgpu_bias_variance"):
+        for n in ("dgpu_read_ply", "dgpu_read_rgbe", "dgpu_write_rgbe", "dgpu_read_png", "dgpu_write_png",
+                  "dgpu_test_convergence", "dgpu_bias_variance"):
             getattr(L, n).restype = C.c_int
         for n in ("dgpu_create", "dgpu_device_alloc", "dgpu_device_free", "dgpu_copy",
                   "dgpu_scene_create", "dgpu_scene_get_info", "dgpu_scene_update_texture",
@@ -119,6 +122,26 @@ def write_rgbe(path, img):
     """dgpu_write_rgbe of an (H, W, 3) image."""
     a = np.ascontiguousarray(img, dtype=np.float32)
     check(lib().dgpu_write_rgbe(os.fsencode(path), a.ctypes.data, a.shape[1], a.shape[0]))
+
+
+def read_png(path):
+    """dgpu_read_png: (H, W, 3) uint8."""
+    px = C.POINTER(C.c_uint8)()
+    w, h = C.c_uint32(), C.c_uint32()
+    check(lib().dgpu_read_png(os.fsencode(path), C.byref(px), C.byref(w), C.byref(h)))
+    try:
+        img = np.ctypeslib.as_array(px, (h.value * w.value * 3,)).reshape(h.value, w.value, 3).copy()
+    finally:
+        lib().dgpu_free(px)
+    return img
+
+
+def write_png(path, img):
+    """dgpu_write_png of an (H, W, 3) uint8 image."""
+    a = np.ascontiguousarray(img, dtype=np.uint8)
+    if a.ndim != 3 or a.shape[2] != 3:
+        raise ValueError("write_png takes an (H, W, 3) image")
+    check(lib().dgpu_write_png(os.fsencode(path), a.ctypes.data, a.shape[1], a.shape[0]))
 
 
 class Context:

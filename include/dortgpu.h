@@ -291,6 +291,15 @@ void dgpu_free(void* p);
 dgpu_status dgpu_read_rgbe(const char* path, float** rgb, uint32_t* width, uint32_t* height);
 dgpu_status dgpu_write_rgbe(const char* path, const float* rgb, uint32_t width, uint32_t height);
 
+/* PNG images as dort's read_image_8 / write_image_png (src/dort/image.cpp:14-26, 42-51,
+ * stb_image) read and write them: any colour type and bit depth (and Adam7) read into
+ * 3-byte RGB pixels, row-major from the top, converted as stb converts (16-bit: high
+ * byte; grey of 1/2/4 bits scaled to 0..255; palette looked up; grey replicated; alpha
+ * dropped); 3-byte RGB written as an 8-bit colour-type-2 PNG.  The pixels read are
+ * allocated by the library (release them with dgpu_free). */
+dgpu_status dgpu_read_png(const char* path, uint8_t** rgb, uint32_t* width, uint32_t* height);
+dgpu_status dgpu_write_png(const char* path, const uint8_t* rgb, uint32_t width, uint32_t height);
+
 /* ---- image comparison (GPU; dort_amd/csrc/compare.hip) --------------------- */
 
 /* dort's test_convergence (src/dort/convergence_test.cpp:10-101) over width*height*3 fp32

@@ -64,6 +64,9 @@ def lib():
         L.ref_read_hdr.argtypes = [C.c_char_p, C.POINTER(C.POINTER(C.c_float)), C.POINTER(C.c_uint32),
                                    C.POINTER(C.c_uint32)]
         L.ref_write_hdr.argtypes = [C.c_char_p, C.c_void_p, C.c_uint32, C.c_uint32]
+        L.ref_read_png.argtypes = [C.c_char_p, C.POINTER(C.POINTER(C.c_uint8)), C.POINTER(C.c_uint32),
+                                   C.POINTER(C.c_uint32)]
+        L.ref_write_png.argtypes = [C.c_char_p, C.c_void_p, C.c_uint32, C.c_uint32]
         L.ref_test_convergence.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_int32, C.c_float,
                                            C.c_float, C.c_float, C.c_char_p, C.c_size_t]
         _lib = L
@@ -100,6 +103,25 @@ def write_hdr(path, img):
     """dort's write_image_rgbe (stb_image_write)."""
     a = np.ascontiguousarray(img, dtype=np.float32)
     if lib().ref_write_hdr(os.fsencode(path), a.ctypes.data, a.shape[1], a.shape[0]) != 0:
+        raise ValueError(f"dort could not write {path}")
+
+
+def read_png(path):
+    """dort's read_image_8 (stb_image, 3 channels): (H, W, 3) uint8."""
+    px = C.POINTER(C.c_uint8)()
+    w, h = C.c_uint32(), C.c_uint32()
+    if lib().ref_read_png(os.fsencode(path), C.byref(px), C.byref(w), C.byref(h)) != 0:
+        raise ValueError(f"dort could not read {path}")
+    img = np.ctypeslib.as_array(px, (3 * w.value * h.value + 1,))[: 3 * w.value * h.value]
+    img = img.reshape(h.value, w.value, 3).copy()
+    lib().ref_free(px)
+    return img
+
+
+def write_png(path, img):
+    """dort's write_image_png (stb_image_write) of an (H, W, 3) uint8 image."""
+    a = np.ascontiguousarray(img, dtype=np.uint8)
+    if lib().ref_write_png(os.fsencode(path), a.ctypes.data, a.shape[1], a.shape[0]) != 0:
         raise ValueError(f"dort could not write {path}")
 
 

@@ -7,6 +7,8 @@
 //   ref_read_ply          read_ply_to_ply_mesh (src/dort/ply_mesh.cpp, over rply)
 //   ref_read_hdr          read_image_f (src/dort/image.cpp, stb_image)
 //   ref_write_hdr         write_image_rgbe (src/dort/image.cpp, stb_image_write)
+//   ref_read_png          read_image_8 (src/dort/image.cpp, stb_image; also JPEG / BMP)
+//   ref_write_png         write_image_png (src/dort/image.cpp, stb_image_write)
 //   ref_test_convergence  test_convergence (src/dort/convergence_test.cpp), one thread
 #include <cstdint>
 #include <cstdio>
@@ -14,6 +16,8 @@
 #include <cstring>
 #include <memory>
 #include <string>
+
+#include <stb_image.h>
 
 #include "dort/convergence_test.hpp"
 #include "dort/ctx.hpp"
@@ -64,6 +68,32 @@ int ref_write_hdr(const char* path, const float* rgb, uint32_t w, uint32_t h) {
   FILE* f = std::fopen(path, "wb");
   if (!f) return 1;
   write_image_rgbe(f, img);
+  std::fclose(f);
+  return 0;
+}
+
+int ref_read_png(const char* path, uint8_t** rgb, uint32_t* w, uint32_t* h) {
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return 1;
+  // read_image_8 asserts on a file stb cannot decode; probe it first
+  int x = 0, y = 0, n = 0;
+  bool ok = stbi_info_from_file(f, &x, &y, &n) != 0;
+  if (!ok) { std::fclose(f); return 2; }
+  Image<PixelRgb8> img = read_image_8(f);
+  std::fclose(f);
+  *w = (uint32_t)img.res.x;
+  *h = (uint32_t)img.res.y;
+  *rgb = (uint8_t*)std::malloc(img.storage.size() + 4);
+  std::memcpy(*rgb, img.storage.data(), img.storage.size());
+  return 0;
+}
+
+int ref_write_png(const char* path, const uint8_t* rgb, uint32_t w, uint32_t h) {
+  Image<PixelRgb8> img(w, h);
+  std::memcpy(img.storage.data(), rgb, img.storage.size());
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return 1;
+  write_image_png(f, img);
   std::fclose(f);
   return 0;
 }
