@@ -528,6 +528,28 @@ dgpu_status dgpu_scene_create(dgpu_ctx* ctx, const dgpu_scene_desc* d, const dgp
   }
 #undef UP
   for (int i = 0; i < 12; ++i) S.c2w[i] = d->camera.camera_to_world[i];
+  {
+    // world-to-camera: the inverse of the affine camera_to_world (in double), as dort's
+    // Transform keeps the inverse matrix for apply_inv
+    const float* m = d->camera.camera_to_world;
+    double a[3][3] = {{m[0], m[1], m[2]}, {m[4], m[5], m[6]}, {m[8], m[9], m[10]}};
+    double det = a[0][0] * (a[1][1] * a[2][2] - a[1][2] * a[2][1]) - a[0][1] * (a[1][0] * a[2][2] - a[1][2] * a[2][0]) +
+                 a[0][2] * (a[1][0] * a[2][1] - a[1][1] * a[2][0]);
+    if (!(det != 0.0) || !std::isfinite(det)) { delete s; return fail(DGPU_EINVAL, "camera transform is singular"); }
+    double inv[3][3];
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) {
+        int r1 = (c + 1) % 3, r2 = (c + 2) % 3, c1 = (r + 1) % 3, c2 = (r + 2) % 3;
+        inv[r][c] = (a[r1][c1] * a[r2][c2] - a[r1][c2] * a[r2][c1]) / det;
+      }
+    double t[3] = {m[3], m[7], m[11]};
+    for (int r = 0; r < 3; ++r) {
+      for (int c = 0; c < 3; ++c) S.w2c[4 * r + c] = (float)inv[r][c];
+      S.w2c[4 * r + 3] = (float)-(inv[r][0] * t[0] + inv[r][1] * t[1] + inv[r][2] * t[2]);
+    }
+  }
+  S.centroid = v3(0.5f * (lo[0] + hi[0]), 0.5f * (lo[1] + hi[1]), 0.5f * (lo[2] + hi[2]));
+  S.radius = radius;
   S.project_dim = 2.f * std::tan(0.5f * d->camera.fov);
   auto t2 = std::chrono::steady_clock::now();
 
@@ -744,6 +766,135 @@ dgpu_status dgpu_render(dgpu_scene* s, const dgpu_render_opts* o, float* film_rg
     for (size_t i = 0; i < (size_t)npix * 3; ++i) film_rgb_sum[i] += hrgb[i];
     for (size_t i = 0; i < npix; ++i) film_weight[i] += hw[i];
   }
+  if (out_stats) {
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    out_stats->paths = hstats[0];
+    out_stats->extension_rays = hstats[1];
+    out_stats->shadow_rays = hstats[2];
+    out_stats->dropped_samples = hstats[3];
+    for (int b = 0; b < 16; ++b) {
+      out_stats->bounce_extension_rays[b] = hstats[4 + b];
+      out_stats->bounce_shadow_rays[b] = hstats[20 + b];
+    }
+    out_stats->kernel_ms = ms;
+    double* kms[4] = {&out_stats->extend_ms, &out_stats->shade_ms, &out_stats->connect_ms, &out_stats->other_ms};
+    uint32_t* kn[4] = {&out_stats->extend_launches, &out_stats->shade_launches, &out_stats->connect_launches,
+                       &out_stats->other_launches};
+    for (auto& m : marks) {
+      float e = 0.f;
+      HIP_TRY(hipEventElapsedTime(&e, ctx->ev_pool[m.second], ctx->ev_pool[m.second + 1]));
+      *kms[m.first] += e;
+      *kn[m.first] += 1;
+    }
+  }
+  return DGPU_OK;
+}
+
+// LightRenderer::render (light_renderer.cpp:13-45): x_res * y_res light paths per
+// iteration, paths of min_depth + 2 .. max_depth + 2 vertices (lua_render.cpp:146-150).
+dgpu_status dgpu_render_light(dgpu_scene* s, const dgpu_render_opts* o, float* film_splat_rgb,
+                              dgpu_stats* out_stats, const volatile int* cancel) {
+  if (!s || !o) return fail(DGPU_EINVAL, "null scene or options");
+  if (!film_splat_rgb) return fail(DGPU_EINVAL, "null film buffer");
+  if (o->x_res == 0 || o->y_res == 0) return fail(DGPU_EINVAL, "empty film");
+  if (o->sample_end < o->sample_begin) return fail(DGPU_EINVAL, "sample_end < sample_begin");
+  if (o->max_depth + 2 >= MAX_BOUNCES) return fail(DGPU_EINVAL, "max_depth must be < %u", MAX_BOUNCES - 2);
+  if (o->min_depth > o->max_depth) return fail(DGPU_EINVAL, "min_depth > max_depth");
+  if ((uint64_t)o->x_res * o->y_res > 0xFFFFFFFFull) return fail(DGPU_EINVAL, "film too large");
+  if (out_stats) memset(out_stats, 0, sizeof *out_stats);
+  dgpu_ctx* ctx = s->ctx;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HIP_TRY(hipSetDevice(ctx->device));
+  hipStream_t stream = ctx->stream;
+  dgpu_status st = DGPU_OK;
+  const uint32_t npix = o->x_res * o->y_res;
+  const uint64_t total = (uint64_t)npix * (o->sample_end - o->sample_begin);
+  uint64_t tile = s->queue_tile ? s->queue_tile : ctx->default_tile(2);
+  uint32_t C = (uint32_t)std::min<uint64_t>(std::min<uint64_t>(total ? total : 1, tile), 0x7FFFFFFFull);
+  const uint32_t grid = ctx->shade_blocks(false, false);
+
+  KernelCfg ce = s->cfg, cc = s->cfg;
+  ce.grid = ctx->grid_for(2, ce);
+  cc.grid = ctx->grid_for(3, cc);
+  uint32_t maxgrid = std::max(ce.grid, cc.grid);
+  WaveParams P;
+  memset(&P, 0, sizeof P);
+  P.S = s->S;
+  P.xr = o->x_res; P.yr = o->y_res; P.npix = npix;
+  P.min_len = o->min_depth + 2; P.max_len = o->max_depth + 2;
+  P.slots = 1;
+  dgpu_rng_key(o->seed, o->scene_id, &P.k0, &P.k1);
+  P.sample_begin = o->sample_begin;
+  // one camera connection per path and stage: the shadow queue and its pending
+  // records need C entries
+  P.thr = (float4*)ctx->buf("thr", (size_t)C * 16, &st);
+  P.rad = (float4*)ctx->buf("rad", (size_t)C * 16, &st);
+  P.pend = (float4*)ctx->buf("pend", (size_t)C * 16, &st);
+  P.qo[0] = (float4*)ctx->buf("qo0", (size_t)C * 16, &st);
+  P.qo[1] = (float4*)ctx->buf("qo1", (size_t)C * 16, &st);
+  P.qd[0] = (float4*)ctx->buf("qd0", (size_t)C * 16, &st);
+  P.qd[1] = (float4*)ctx->buf("qd1", (size_t)C * 16, &st);
+  P.qpath[0] = (uint32_t*)ctx->buf("qp0", (size_t)C * 4, &st);
+  P.qpath[1] = (uint32_t*)ctx->buf("qp1", (size_t)C * 4, &st);
+  P.hit = (float4*)ctx->buf("hit", (size_t)C * 16, &st);
+  P.so = (float4*)ctx->buf("so", (size_t)C * 16, &st);
+  P.sd = (float4*)ctx->buf("sd", (size_t)C * 16, &st);
+  P.smeta = (uint2*)ctx->buf("smeta", (size_t)C * 8, &st);
+  P.fpos = (float2*)ctx->buf("fpos", (size_t)C * 8, &st);
+  P.counters = (uint32_t*)ctx->buf("counters", NUM_COUNTERS * 4, &st);
+  P.stats = (unsigned long long*)ctx->buf("stats", STATS_WORDS * 8, &st);
+  P.spill = (uint32_t*)ctx->buf("spill", (size_t)maxgrid * s->cfg.block * s->stack_cap * 4, &st);
+  P.stack_cap = s->stack_cap;
+  bool dev = o->flags & DGPU_BUF_DEVICE;
+  P.film_rgb = dev ? film_splat_rgb : (float*)ctx->buf("film_rgb", (size_t)npix * 12, &st);
+  if (st != DGPU_OK) return st;
+  HIP_TRY(hipMemsetAsync(P.stats, 0, STATS_WORDS * 8, stream));
+  if (!dev) HIP_TRY(hipMemsetAsync(P.film_rgb, 0, (size_t)npix * 12, stream));
+  const bool profile = (o->flags & DGPU_RENDER_PROFILE) != 0;
+  std::vector<std::pair<int, size_t>> marks;
+  size_t next_ev = 0;
+  auto timed = [&](int kind, auto&& launch) -> hipError_t {
+    if (!profile) return launch();
+    hipEvent_t a = ctx->pool_event(next_ev), b = ctx->pool_event(next_ev + 1);
+    if (!a || !b) return hipErrorOutOfMemory;
+    hipError_t e = hipEventRecord(a, stream);
+    if (e == hipSuccess) e = launch();
+    if (e == hipSuccess) e = hipEventRecord(b, stream);
+    marks.push_back({kind, next_ev});
+    next_ev += 2;
+    return e;
+  };
+  HIP_TRY(hipEventRecord(ctx->ev0, stream));
+  bool cancelled = false;
+  for (uint64_t g0 = 0; g0 < total && !cancelled; g0 += C) {
+    P.g0 = g0;
+    P.n_chunk = (uint32_t)std::min<uint64_t>(C, total - g0);
+    HIP_TRY(launch_zero(P.counters, NUM_COUNTERS, stream));
+    HIP_TRY(timed(1, [&] { return launch_lt_start(P, stream, grid); }));
+    HIP_TRY(timed(2, [&] { return launch_connect(P, cc, 0, false, stream); }));
+    HIP_TRY(timed(3, [&] { return launch_lt_splat(P, 0, stream, grid); }));
+    for (uint32_t b = 0; b + 3u <= P.max_len; ++b) {
+      if (cancel && *cancel) { cancelled = true; break; }
+      HIP_TRY(timed(0, [&] { return launch_extend(P, ce, b, false, stream); }));
+      HIP_TRY(timed(1, [&] { return launch_lt_shade(P, b, stream, grid); }));
+      HIP_TRY(timed(2, [&] { return launch_connect(P, cc, b + 1, false, stream); }));
+      HIP_TRY(timed(3, [&] { return launch_lt_splat(P, b + 1, stream, grid); }));
+    }
+    HIP_TRY(launch_lt_stats(P, stream));
+  }
+  HIP_TRY(hipEventRecord(ctx->ev1, stream));
+  unsigned long long hstats[STATS_WORDS] = {};
+  HIP_TRY(hipMemcpyAsync(hstats, P.stats, sizeof hstats, hipMemcpyDeviceToHost, stream));
+  std::vector<float> hrgb;
+  if (!dev && !cancelled) {
+    hrgb.resize((size_t)npix * 3);
+    HIP_TRY(hipMemcpyAsync(hrgb.data(), P.film_rgb, (size_t)npix * 12, hipMemcpyDeviceToHost, stream));
+  }
+  HIP_TRY(hipStreamSynchronize(stream));
+  if (cancelled) return fail(DGPU_ECANCELLED, "render cancelled");
+  if (!dev)
+    for (size_t i = 0; i < (size_t)npix * 3; ++i) film_splat_rgb[i] += hrgb[i];
   if (out_stats) {
     float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));

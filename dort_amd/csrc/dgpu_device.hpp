@@ -148,7 +148,11 @@ struct DevScene {
   uint32_t num_tex;
   // camera (pinhole_camera.cpp:6-11)
   float c2w[12];
+  float w2c[12];             // inverse of c2w (Transform::apply_inv)
   float project_dim;
+  // bounding sphere of the scene (Scene::centroid, radius; lua_builder.cpp:124-128)
+  V3 centroid;
+  float radius;
 };
 
 // ----------------------------------------------------------------- RNG

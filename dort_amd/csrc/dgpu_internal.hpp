@@ -90,6 +90,9 @@ struct WaveParams {
   const ChunkDesc* chunks;
   uint32_t* chunk_cur;     // index of the chunk being rendered (advanced on the device)
   uint2* pid;              // per path slot: (pixel, sample)
+  // light tracing (dgpu_render_light)
+  uint32_t min_len, max_len;  // path lengths in vertices (LightRenderer min_length / max_length)
+  float2* fpos;            // film position of each camera-connection shadow ray
 };
 
 struct AdjArgs {
@@ -124,6 +127,11 @@ hipError_t launch_connect(const WaveParams& P, const KernelCfg& cfg, uint32_t bo
 hipError_t launch_finalize(const WaveParams& P, uint32_t capacity, bool batch, hipStream_t st);
 hipError_t launch_advance(uint32_t* cur, hipStream_t st);
 hipError_t launch_adjoint(const WaveParams& P, const AdjArgs& A, uint32_t width, hipStream_t st);
+hipError_t launch_zero(uint32_t* p, uint32_t n, hipStream_t st);
+hipError_t launch_lt_start(const WaveParams& P, hipStream_t st, uint32_t grid);
+hipError_t launch_lt_shade(const WaveParams& P, uint32_t bounce, hipStream_t st, uint32_t grid);
+hipError_t launch_lt_splat(const WaveParams& P, uint32_t stage, hipStream_t st, uint32_t grid);
+hipError_t launch_lt_stats(const WaveParams& P, hipStream_t st);
 
 // A context's lock, device and stream, for entry points outside api.hip.
 std::mutex& ctx_mutex(dgpu_ctx* ctx);

@@ -926,6 +926,301 @@ const void* trace_kernel_ptr(int which, uint32_t width, uint32_t lds, uint32_t b
   return nullptr;
 }
 
+// ----------------------------------------------------------------- light tracing
+//
+// dort's LightRenderer (src/dort/light_renderer.cpp:13-181) as a wavefront.  k_lt_start
+// picks a light per path, connects a point on it to the camera (sample_immediate, a
+// path of 2 vertices) and samples the first ray (Light::sample_ray_radiance);
+// k_lt_shade connects each hit to the camera (connect_to_camera) and scatters
+// (Bsdf::sample_camera_f, Russian roulette); k_connect tests the connections and
+// k_lt_splat adds the visible ones to the film (Film::add_splat) with float atomics.
+// Stage s of the shadow queue holds the connections of paths of s + 2 vertices; a
+// connection's pending record (contribution, visible flag, film position) sits at its
+// own queue slot.  Path state: thr = (throughput, bounces), rad = (normal of the last
+// vertex, pdf of the direction sampled there).
+
+// Four uniforms of block `blk` of stage `stage` of light path (pixel, sample): Philox
+// with a counter word of its own, independent of the path tracer's streams.
+__device__ __forceinline__ void lt_rng(const WaveParams& P, uint32_t pixel, uint32_t sample, uint32_t stage,
+                                       uint32_t blk, float out[4]) {
+  dgpu_u32x4 r = dgpu_philox4x32_10(pixel, sample, (stage << 2) | blk, 0x6C747261u, P.k0, P.k1);
+  for (int k = 0; k < 4; ++k) out[k] = dgpu_u01(r.v[k]);
+}
+
+// DiscreteDistrib1d::sample / pdf over the lights' power (discrete_distrib_1d.cpp:22-42,
+// scene.cpp:20-28): the light for u and its probability.
+__device__ __forceinline__ uint32_t lt_pick_light(const DevScene& S, float u, float& prob) {
+  uint32_t b = 0, e = S.num_lights;
+  while (b + 1u < e) {
+    uint32_t m = b + (e - b) / 2u;
+    if (S.light_cdf[m] < u) b = m; else e = m;
+  }
+  prob = S.light_cdf[b + 1] - S.light_cdf[b];
+  return b;
+}
+
+// Triangle::sample_point (triangle.cpp:62-76) on an area light's triangle.
+__device__ __forceinline__ V3 lt_area_point(const DevScene& S, const DevLight& L, float u, float v, V3& n) {
+  float4 a = S.tris[3 * L.tri], b = S.tris[3 * L.tri + 1], c = S.tris[3 * L.tri + 2];
+  V3 p0 = xyz(a), e1 = xyz(b) - p0, e2 = xyz(c) - p0;
+  float s = sqrtf(u);
+  n = normalize(cross(e2, e1));
+  return p0 + e1 * (1.f - s) + e2 * (v * s);
+}
+
+// World point and vector in the camera frame (Transform::apply_inv).
+__device__ __forceinline__ V3 lt_cam_vector(const DevScene& S, V3 w) {
+  const float* m = S.w2c;
+  return v3(m[0] * w.x + m[1] * w.y + m[2] * w.z, m[4] * w.x + m[5] * w.y + m[6] * w.z,
+            m[8] * w.x + m[9] * w.y + m[10] * w.z);
+}
+__device__ __forceinline__ V3 lt_cam_point(const DevScene& S, V3 p) {
+  const float* m = S.w2c;
+  return lt_cam_vector(S, p) + v3(m[3], m[7], m[11]);
+}
+
+// PinholeCamera::eval_importance / sample_pivot_importance with get_film_pos
+// (pinhole_camera.cpp:33-82, 104-126): film position of the camera-frame direction c
+// and the importance 1 / (A cos^3 theta); false when c is behind or outside the image.
+__device__ __forceinline__ bool lt_importance(const WaveParams& P, V3 c, float2& fp, float& imp) {
+  if (!(c.z > 0.f)) return false;
+  float nx = c.x / c.z / P.S.project_dim, ny = c.y / c.z / P.S.project_dim;
+  if (fabsf(nx) > 0.5f || fabsf(ny) > 0.5f) return false;
+  float rx = float(P.xr), ry = float(P.yr);
+  float cx = nx, cy = ny;
+  if (rx > ry) cy = ny * rx / ry; else cx = nx * ry / rx;
+  fp = make_float2((cx + 0.5f) * rx, (cy + 0.5f) * ry);
+  float ct = c.z / length(c);
+  float area = fminf(rx, ry) / fmaxf(rx, ry) * sqf(P.S.project_dim);
+  imp = 1.f / (area * (ct * ct * ct));
+  return true;
+}
+
+struct LtRay {
+  V3 o, d, nn, rad;
+  float tmin, pos_pdf, dir_pdf;
+};
+
+// Light::sample_ray_radiance (diffuse_light.cpp:15-38, environment_light.cpp:34-52,
+// infinite_light.cpp:5-23): r[0..1] the position sample, r[2..3] the direction sample.
+__device__ __forceinline__ LtRay lt_light_ray(const DevScene& S, const DevLight& L, const float r[4]) {
+  LtRay out;
+  if (L.kind == LK_AREA) {
+    V3 n;
+    V3 p = lt_area_point(S, L, r[0], r[1], n);
+    float z = r[2], rr = sqrtf(fmaxf(0.f, 1.f - z * z)), sp, cp;  // uniform_hemisphere_sample
+    sincos_small(TWO_PI_F * r[3], sp, cp);
+    V3 w = v3(cp * rr, sp * rr, z);
+    if (dot(w, n) < 0.f) w = -w;
+    out.o = p; out.d = w; out.nn = n; out.tmin = 1e-3f;
+    out.pos_pdf = 1.f / L.area;
+    out.dir_pdf = 1.f / TWO_PI_F;
+    out.rad = xyz(L.radiance);
+    return out;
+  }
+  V3 dir;
+  if (L.kind == LK_ENV) {
+    dir = env_sample(S, r[2], r[3], out.dir_pdf, out.rad);
+  } else {
+    float z = 1.f - 2.f * r[2], rr = sqrtf(fmaxf(0.f, 1.f - z * z)), sp, cp;  // uniform_sphere_sample
+    sincos_small(TWO_PI_F * r[3], sp, cp);
+    dir = v3(cp * rr, sp * rr, z);
+    out.dir_pdf = INV_FOUR_PI_F;
+    out.rad = S.inf_rad;
+  }
+  float dr = sqrtf(r[0]), ds, dc;  // uniform_disk_sample
+  sincos_small(TWO_PI_F * r[1], ds, dc);
+  V3 su, sv;
+  coordinate_system(dir, su, sv);
+  V3 off = su * (dc * dr) + sv * (ds * dr);
+  if (L.kind == LK_ENV) {  // rays come in from the direction sampled
+    out.o = S.centroid + (dir + off) * S.radius;
+    out.d = -dir;
+  } else {
+    out.o = S.centroid + (-dir + off) * S.radius;
+    out.d = dir;
+  }
+  out.nn = out.d;
+  out.tmin = 0.f;
+  out.pos_pdf = INV_PI_F / sqf(S.radius);
+  return out;
+}
+
+__device__ __forceinline__ void lt_emit_connection(const WaveParams& P, bool emit, V3 from, float eps, V3 to,
+                                                   V3 contrib, float2 fp, uint32_t stage, uint32_t* sh) {
+  uint32_t q = block_append(&P.counters[CNT_SHADOW + stage], emit, sh);
+  if (emit) {
+    // ShadowTest::init_point_point(from, eps, to, 0) (light.cpp:40-49)
+    V3 d = to - from;
+    float dist = length(d);
+    d = d / dist;
+    P.so[q] = make_float4(from.x, from.y, from.z, eps);
+    P.sd[q] = make_float4(d.x, d.y, d.z, dist);
+    P.smeta[q] = make_uint2(q, 0xFFFFFFFFu);
+    P.pend[q] = make_float4(contrib.x, contrib.y, contrib.z, 1.f);
+    P.fpos[q] = fp;
+  }
+}
+
+__global__ __launch_bounds__(SHADE_BLOCK) void k_lt_start(WaveParams P) {
+  __shared__ uint32_t sh[SHADE_BLOCK / 64 + 1];
+  const DevScene& S = P.S;
+  const V3 cam = v3(S.c2w[3], S.c2w[7], S.c2w[11]);
+  for (uint32_t base = blockIdx.x * blockDim.x; base < P.n_chunk; base += gridDim.x * blockDim.x) {
+    uint32_t i = base + threadIdx.x;
+    bool imm = false, go = false;
+    V3 lp = v3(0.f, 0.f, 0.f), contrib = lp, thr = lp;
+    float2 fp = make_float2(0.f, 0.f);
+    LtRay lr;
+    lr.o = lr.d = lr.nn = lr.rad = lp;
+    lr.tmin = lr.pos_pdf = lr.dir_pdf = 0.f;
+    if (i < P.n_chunk && S.num_lights > 0u) {
+      uint64_t g = P.g0 + i;
+      uint32_t sample = P.sample_begin + (uint32_t)(g / P.npix), pixel = (uint32_t)(g % P.npix);
+      float r0[4], r1[4];
+      lt_rng(P, pixel, sample, 0, 0, r0);
+      float pick;
+      uint32_t li = lt_pick_light(S, r0[0], pick);
+      if (pick != 0.f) {
+        const DevLight L = S.lights[li];
+        if (P.min_len <= 2u && 2u <= P.max_len && L.kind == LK_AREA) {
+          // sample_immediate (light_renderer.cpp:107-144); only area lights have points
+          V3 ln;
+          lp = lt_area_point(S, L, r0[1], r0[2], ln);
+          float pos_pdf = 1.f / L.area;
+          float imp;
+          if (pos_pdf != 0.f && !(lp.x == cam.x && lp.y == cam.y && lp.z == cam.z) &&
+              lt_importance(P, lt_cam_vector(S, normalize(lp - cam)), fp, imp) && dot(ln, cam - lp) >= 0.f) {
+            V3 rad = xyz(L.radiance);
+            contrib = rad * imp / (length_sq(lp - cam) * (pick * pos_pdf));
+            contrib = contrib * fabsf(dot(ln, normalize(lp - cam)));
+            imm = !is_black(contrib);
+          }
+        }
+        if (P.max_len >= 3u) {
+          lt_rng(P, pixel, sample, 0, 1, r1);
+          lr = lt_light_ray(S, L, r1);
+          float light_pdf = pick * lr.pos_pdf * lr.dir_pdf;
+          if (!is_black(lr.rad) && light_pdf != 0.f) {
+            thr = lr.rad / (lr.pos_pdf * pick);
+            go = true;
+          }
+        }
+      }
+    }
+    lt_emit_connection(P, imm, lp, 1e-3f, cam, contrib, fp, 0, sh);
+    uint32_t qn = block_append(&P.counters[CNT_QUEUE + 0], go, sh);
+    if (go) {
+      P.qo[0][qn] = make_float4(lr.o.x, lr.o.y, lr.o.z, lr.tmin);
+      P.qd[0][qn] = make_float4(lr.d.x, lr.d.y, lr.d.z, INFINITY);
+      P.qpath[0][qn] = i;
+      P.thr[i] = make_float4(thr.x, thr.y, thr.z, __uint_as_float(0u));
+      P.rad[i] = make_float4(lr.nn.x, lr.nn.y, lr.nn.z, lr.dir_pdf);
+    }
+  }
+}
+
+// One vertex of LightRenderer::sample_path (light_renderer.cpp:66-104) for every hit of
+// extension stage `bounce`.  Bsdf::sample_camera_f calls each lobe's sample function
+// with the same arguments as sample_light_f (bsdf.cpp:95-110; dielectric_material.cpp:10-22,
+// rough_dielectric_material.cpp:39-52 pass fix_is_light = false to both), and every
+// material here is one lobe, so it is bsdf_sample with the fixed direction wi.
+__global__ __launch_bounds__(SHADE_BLOCK) void k_lt_shade(WaveParams P, uint32_t bounce) {
+  __shared__ uint32_t sh[SHADE_BLOCK / 64 + 1];
+  const DevScene& S = P.S;
+  const V3 cam = v3(S.c2w[3], S.c2w[7], S.c2w[11]);
+  const uint32_t cur = bounce & 1u, nxt = cur ^ 1u;
+  const uint32_t n = P.counters[CNT_QUEUE + bounce];
+  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+    uint32_t i = base + threadIdx.x;
+    bool conn = false, go = false;
+    uint32_t p = 0, bounces = 0;
+    V3 gp = v3(0.f, 0.f, 0.f), contrib = gp, thr = gp, gn = gp, wo = gp;
+    float eps = 0.f, pdf = 0.f;
+    float2 fp = make_float2(0.f, 0.f);
+    if (i < n) {
+      p = P.qpath[cur][i];
+      float4 h4 = P.hit[i];
+      uint32_t leaf = __float_as_uint(h4.w);
+      if (leaf != 0xFFFFFFFFu) {
+        V3 wi = normalize(-xyz(P.qd[cur][i]));
+        Surface g = surface_at(S, leaf, h4.x, h4.y, h4.z);
+        Bsdf bsdf = make_bsdf(S, g);
+        float4 t4 = P.thr[p], n4 = P.rad[p];
+        thr = xyz(t4);
+        bounces = __float_as_uint(t4.w);
+        float geom = fabsf(dot(xyz(n4), wi)) / n4.w;
+        uint64_t gi = P.g0 + p;
+        uint32_t sample = P.sample_begin + (uint32_t)(gi / P.npix), pixel = (uint32_t)(gi % P.npix);
+        float r[4];
+        lt_rng(P, pixel, sample, bounces + 1u, 0, r);
+        gp = g.p; gn = g.nn; eps = g.eps;
+        if (bounces + 3u >= P.min_len && bsdf.matches(BSDF_ALL & ~BSDF_DELTA)) {
+          // connect_to_camera (light_renderer.cpp:146-180)
+          float imp;
+          if (lt_importance(P, lt_cam_point(S, g.p), fp, imp)) {
+            V3 cw = normalize(cam - g.p);
+            V3 f = bsdf_eval(bsdf, wi, cw, BSDF_ALL);
+            if (!is_black(f)) {
+              contrib = thr * geom * imp * f * (fabsf(dot(g.nn, cw)) / length_sq(cam - g.p));
+              conn = !is_black(contrib);
+            }
+          }
+        }
+        uint32_t fl;
+        V3 f = bsdf_sample(bsdf, wi, BSDF_ALL, r[0], r[1], r[2], wo, pdf, fl);
+        V3 bc = f * geom;
+        thr = thr * bc;
+        go = !is_black(thr);
+        if (go && bounces + 3u > P.min_len && bounces >= 2u) {
+          float q = clampf(average(bc), 0.1f, 0.9f);
+          if (r[3] > q) go = false;
+          else thr = thr / q;
+        }
+        go = go && bounces + 4u <= P.max_len && pdf != 0.f;
+      }
+    }
+    lt_emit_connection(P, conn, gp, eps, cam, contrib, fp, bounce + 1u, sh);
+    uint32_t qn = block_append(&P.counters[CNT_QUEUE + bounce + 1u], go, sh);
+    if (go) {
+      P.qo[nxt][qn] = make_float4(gp.x, gp.y, gp.z, eps);
+      P.qd[nxt][qn] = make_float4(wo.x, wo.y, wo.z, INFINITY);
+      P.qpath[nxt][qn] = p;
+      P.thr[p] = make_float4(thr.x, thr.y, thr.z, __uint_as_float(bounces + 1u));
+      P.rad[p] = make_float4(gn.x, gn.y, gn.z, pdf);
+    }
+  }
+}
+
+// Film::add_splat (film.cpp:32-67) with the 1 px box filter: the pixels whose centres
+// lie within 0.5 of the position (one, or two or four on pixel edges) share the
+// contribution equally.  Non-finite contributions are dropped and counted.
+__global__ __launch_bounds__(256) void k_lt_splat(WaveParams P, uint32_t stage) {
+  const uint32_t n = P.counters[CNT_SHADOW + stage];
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    float4 pe = P.pend[i];
+    if (pe.w == 0.f) continue;
+    if (!(isfinite(pe.x) && isfinite(pe.y) && isfinite(pe.z))) {
+      atomicAdd((unsigned long long*)&P.stats[3], 1ull);
+      continue;
+    }
+    float2 fp = P.fpos[i];
+    int x0 = max(0, (int)ceilf(fp.x - 1.f)), x1 = min((int)P.xr - 1, (int)floorf(fp.x));
+    int y0 = max(0, (int)ceilf(fp.y - 1.f)), y1 = min((int)P.yr - 1, (int)floorf(fp.y));
+    if (x0 > x1 || y0 > y1) continue;
+    float w = 1.f / float((x1 - x0 + 1) * (y1 - y0 + 1));
+    for (int y = y0; y <= y1; ++y)
+      for (int x = x0; x <= x1; ++x) {
+        float* f = P.film_rgb + 3 * ((size_t)y * P.xr + x);
+        atomicAdd(f, pe.x * w);
+        atomicAdd(f + 1, pe.y * w);
+        atomicAdd(f + 2, pe.z * w);
+      }
+  }
+}
+
+__global__ void k_lt_stats(WaveParams P) { add_chunk_stats(P, P.n_chunk); }
+
 hipError_t launch_trace_api(const DevScene& S, const KernelCfg& cfg, const dgpu_rays& rays,
                             const dgpu_hits* hits, uint32_t* occluded, uint32_t n, uint32_t* fetch,
                             uint32_t* spill, bool bounded, hipStream_t st) {
@@ -1039,6 +1334,31 @@ hipError_t launch_adjoint(const WaveParams& P, const AdjArgs& A, uint32_t width,
   uint32_t blocks = (P.n_chunk + ADJ_BLOCK - 1u) / ADJ_BLOCK;
   if (width == 4) hipLaunchKernelGGL(k_adjoint<4>, dim3(blocks), dim3(ADJ_BLOCK), 0, st, P, A);
   else hipLaunchKernelGGL(k_adjoint<2>, dim3(blocks), dim3(ADJ_BLOCK), 0, st, P, A);
+  return hipGetLastError();
+}
+
+hipError_t launch_zero(uint32_t* p, uint32_t n, hipStream_t st) {
+  hipLaunchKernelGGL(k_zero_u32, dim3((n + 63u) / 64u), dim3(64), 0, st, p, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_lt_start(const WaveParams& P, hipStream_t st, uint32_t grid) {
+  hipLaunchKernelGGL(k_lt_start, dim3(grid), dim3(SHADE_BLOCK), 0, st, P);
+  return hipGetLastError();
+}
+
+hipError_t launch_lt_shade(const WaveParams& P, uint32_t bounce, hipStream_t st, uint32_t grid) {
+  hipLaunchKernelGGL(k_lt_shade, dim3(grid), dim3(SHADE_BLOCK), 0, st, P, bounce);
+  return hipGetLastError();
+}
+
+hipError_t launch_lt_splat(const WaveParams& P, uint32_t stage, hipStream_t st, uint32_t grid) {
+  hipLaunchKernelGGL(k_lt_splat, dim3(grid), dim3(256), 0, st, P, stage);
+  return hipGetLastError();
+}
+
+hipError_t launch_lt_stats(const WaveParams& P, hipStream_t st) {
+  hipLaunchKernelGGL(k_lt_stats, dim3(1), dim3(64), 0, st, P);
   return hipGetLastError();
 }
 

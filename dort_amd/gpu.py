@@ -19,7 +19,8 @@ _lib = None
 EXPORTS = ["dgpu_abi_version", "dgpu_last_error", "dgpu_create", "dgpu_destroy", "dgpu_device_alloc",
            "dgpu_device_free", "dgpu_copy", "dgpu_scene_create", "dgpu_scene_destroy",
            "dgpu_scene_get_info", "dgpu_scene_update_texture", "dgpu_scene_update_material",
-           "dgpu_trace_closest", "dgpu_trace_any", "dgpu_render", "dgpu_render_batch", "dgpu_adjoint",
+           "dgpu_trace_closest", "dgpu_trace_any", "dgpu_render", "dgpu_render_light", "dgpu_render_batch",
+           "dgpu_adjoint",
            "dgpu_read_ply", "dgpu_free", "dgpu_read_rgbe", "dgpu_write_rgbe", "dgpu_read_png", "dgpu_write_png",
            "dgpu_test_convergence", "dgpu_bias_variance"]
 
@@ -53,6 +54,8 @@ def lib():
                                      C.c_uint32, C.POINTER(C.c_double)]
         L.dgpu_render.argtypes = [C.c_void_p, C.POINTER(abi.RenderOpts), C.c_void_p, C.c_void_p,
                                   C.POINTER(abi.RenderStats), C.c_void_p]
+        L.dgpu_render_light.argtypes = [C.c_void_p, C.POINTER(abi.RenderOpts), C.c_void_p,
+                                        C.POINTER(abi.RenderStats), C.c_void_p]
         L.dgpu_render_batch.argtypes = [C.c_void_p, C.POINTER(abi.BatchItem), C.c_uint32, C.c_uint32,
                                         C.POINTER(abi.RenderStats)]
         L.dgpu_scene_update_texture.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p]
@@ -81,6 +84,7 @@ def lib():
         for n in ("dgpu_create", "dgpu_device_alloc", "dgpu_device_free", "dgpu_copy",
                   "dgpu_scene_create", "dgpu_scene_get_info", "dgpu_scene_update_texture",
                   "dgpu_scene_update_material", "dgpu_trace_closest", "dgpu_trace_any", "dgpu_render",
+                  "dgpu_render_light",
                   "dgpu_render_batch", "dgpu_adjoint"):
             getattr(L, n).restype = C.c_int
         _lib = L
@@ -329,6 +333,24 @@ class Scene:
     def image(self, opts):
         rgb, w, _ = self.render(opts)
         return rgb / np.maximum(w, 1e-30)[..., None]
+
+    def render_light(self, opts, film_rgb=None, device=False, cancel=None):
+        """dort's light tracer: adds splats to film_rgb (H,W,3); returns (film_rgb, stats)."""
+        if film_rgb is None:
+            film_rgb = np.zeros((opts.y_res, opts.x_res, 3), np.float32)
+        ptr = lambda a: a.data_ptr() if hasattr(a, "data_ptr") else a.ctypes.data
+        o = abi.RenderOpts.from_buffer_copy(opts)
+        if device:
+            o.flags |= abi.BUF_DEVICE
+        st = abi.RenderStats()
+        cptr = cancel.ctypes.data if cancel is not None else None
+        check(lib().dgpu_render_light(self.h, C.byref(o), ptr(film_rgb), C.byref(st), cptr))
+        return film_rgb, abi.stats_dict(st)
+
+    def light_image(self, opts):
+        """The light tracer's image: splats / iterations (Film::to_image's splat_scale)."""
+        rgb, _ = self.render_light(opts)
+        return rgb / float(opts.sample_end - opts.sample_begin)
 
     def update_texture(self, tex, rgb):
         rgb = np.ascontiguousarray(rgb, dtype=np.float32)

@@ -241,6 +241,19 @@ dgpu_status dgpu_render(dgpu_scene* scene, const dgpu_render_opts* opts,
                         float* film_rgb_sum, float* film_weight,
                         dgpu_stats* out_stats, const volatile int* cancel);
 
+/* dort's light tracer, LightRenderer::render (src/dort/light_renderer.cpp:13-181, the
+ * "lt" renderer of lua_render.cpp:146-150): per iteration x_res*y_res paths start on a
+ * light picked by power; every vertex is connected to the camera and splatted into
+ * the film with the 1 px box filter.  Uses x_res, y_res, sample_begin/sample_end
+ * (iterations), seed, scene_id, flags and min_depth/max_depth (paths of min_depth + 2
+ * .. max_depth + 2 vertices, as dort maps them); Russian roulette is dort's (from the
+ * third bounce, survival clamped to [0.1, 0.9]).  film_splat_rgb (x_res*y_res*3, host
+ * or device memory with DGPU_BUF_DEVICE) receives += the splatted contributions;
+ * dort's image is the sum times 1 / (total iterations) (Film::to_image's splat_scale).
+ * Splats are float atomics, so sums vary in the last bits from run to run. */
+dgpu_status dgpu_render_light(dgpu_scene* scene, const dgpu_render_opts* opts, float* film_splat_rgb,
+                              dgpu_stats* out_stats, const volatile int* cancel);
+
 /* ---- batched rendering (no dort counterpart) -------------------------------
  * Renders many scenes in one wavefront: every scene's paths go into shared queues,
  * the per-chunk bounce loop (raygen, extend/shade/connect per bounce, film) is

@@ -58,6 +58,8 @@ def lib():
                                 C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.ref_render.restype = C.c_int
         L.ref_render.argtypes = [C.c_void_p, C.POINTER(abi.RenderOpts), C.c_uint32, C.c_void_p, C.c_void_p]
+        L.ref_render_light.restype = C.c_int
+        L.ref_render_light.argtypes = [C.c_void_p, C.POINTER(abi.RenderOpts), C.c_uint32, C.c_void_p]
         L.ref_read_ply.argtypes = [C.c_char_p, C.POINTER(C.POINTER(C.c_float)), C.POINTER(C.c_uint32),
                                    C.POINTER(C.POINTER(C.c_uint32)), C.POINTER(C.c_uint32)]
         L.ref_free.argtypes = [C.c_void_p]
@@ -175,6 +177,14 @@ class RefScene:
         if lib().ref_render(self.h, C.byref(opts), threads, rgb.ctypes.data, w.ctypes.data) != 0:
             raise RuntimeError("ref_render failed")
         return rgb, w
+
+    def render_light(self, opts, threads=1):
+        """LightRenderer::render (dort's `lt`) with RandomSampler(Rng(seed)): the film's
+        splat sums (H, W, 3); the image is the sum / iterations."""
+        rgb = np.zeros((opts.y_res, opts.x_res, 3), np.float32)
+        if lib().ref_render_light(self.h, C.byref(opts), threads, rgb.ctypes.data) != 0:
+            raise RuntimeError("ref_render_light failed")
+        return rgb
 
 
 def gpu_lib():

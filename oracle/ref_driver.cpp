@@ -8,6 +8,7 @@
 //   ref_trace          Scene::intersect / intersect_p (scene.cpp)
 //   ref_render         PathRenderer::render (path_renderer.cpp:11-21) into a
 //                      Film with the default 1 px box filter (lua_render.cpp:108-115)
+//   ref_render_light   LightRenderer::render (light_renderer.cpp:13-45), the splat sums
 //
 // Scenes with a GGX conductor or an albedo texture are refused (dort has no rough
 // conductor; its ImageTexture is 8-bit).
@@ -32,6 +33,7 @@
 #include "dort/bvh_primitive.hpp"
 #include "dort/mat.hpp"
 #include "dort/mesh.hpp"
+#include "dort/light_renderer.hpp"
 #include "dort/path_renderer.hpp"
 #include "dort/pinhole_camera.hpp"
 #include "dort/primitive.hpp"
@@ -258,6 +260,29 @@ void ref_trace(void* p, uint32_t n, const float* const* rays, int any, float* t,
 // RandomSampler seeded by opts->seed.  Outputs the film's per-pixel filtered
 // colour sums and weights.  The pool runs `threads` threads; with 1 thread
 // the result is deterministic.
+// LightRenderer::render (light_renderer.cpp:13-45) with opts->sample_end iterations and
+// lengths min_depth + 2 .. max_depth + 2 (lua_render.cpp:146-150).  Outputs the film's
+// per-pixel splat sums (Film::Pixel::splat; the image is the sum / iterations).
+int ref_render_light(void* p, const dgpu_render_opts* o, uint32_t threads, float* rgb) {
+  const RefScene& rs = *(const RefScene*)p;
+  if (o->sample_begin != 0 || o->sample_end == 0) return 1;
+  CtxG ctx;
+  ctx.pool = make_pool(threads);
+  auto film = std::make_shared<Film>(o->x_res, o->y_res, std::make_shared<BoxFilter>(Vec2(0.5f, 0.5f)));
+  auto sampler = std::make_shared<RandomSampler>(1, Rng((uint32_t)o->seed));
+  LightRenderer r(rs.scene, film, sampler, rs.camera, o->sample_end, o->min_depth + 2, o->max_depth + 2);
+  NoProgress progress;
+  r.render(ctx, progress);
+  ctx.pool->stop();
+  for (size_t i = 0; i < film->pixels.size(); ++i) {
+    Spectrum sp = film->pixels[i].splat.load_relaxed();
+    rgb[3 * i] = sp.red();
+    rgb[3 * i + 1] = sp.green();
+    rgb[3 * i + 2] = sp.blue();
+  }
+  return 0;
+}
+
 int ref_render(void* p, const dgpu_render_opts* o, uint32_t threads, float* rgb, float* w) {
   const RefScene& rs = *(const RefScene*)p;
   if (o->sample_begin != 0 || o->sample_end == 0) return 1;
