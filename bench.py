@@ -50,6 +50,7 @@ def parse():
     ap.add_argument("--no-intersect", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-light", action="store_true")
     ap.add_argument("--calib", action="store_true",
                     help="also run a 256 MiB device-to-device copy (counter calibration for rocprofv3 --pmc)")
     ap.add_argument("--width", type=int, default=4)
@@ -115,6 +116,27 @@ def intersection_bench(ctx, args):
     for v in list(dev.values()) + [out[k] for k in ("t", "prim", "b1", "b2")] + [occ]:
         v.free()
     return res
+
+
+def light_bench(gsc, film, args):
+    """dort's light tracer (dgpu_render_light) on the headline terrain scene: light paths
+    per second at 1024x1024 x 16 iterations (16.8M paths), depth 8; the splat film is
+    resident in HBM."""
+    from dort_amd import abi
+    its = 16
+    opts = abi.render_opts(args.res, args.res, spp=its, seed=99, max_depth=8)
+    gsc.render_light(opts, film, device=True)
+    ts, st = [], None
+    for _ in range(3):
+        t0 = time.perf_counter()
+        _, st = gsc.render_light(opts, film, device=True)
+        ts.append(time.perf_counter() - t0)
+    t = float(np.median(ts))
+    paths = args.res * args.res * its
+    return {"workload": f"terrain, light tracing, {args.res}x{args.res} x {its} iterations, max depth 8",
+            "Mpaths_s": paths / t / 1e6, "ms": t * 1e3,
+            "extension_rays_per_path": st["extension_rays"] / st["paths"],
+            "shadow_rays_per_path": st["shadow_rays"] / st["paths"]}
 
 
 def adjoint_bench(ctx, args):
@@ -293,6 +315,11 @@ def main():
         log("calibration copy: 2 x 256 MiB device-to-device")
 
     secondary = {}
+    if not args.no_light:
+        secondary["light_tracing"] = light_bench(gsc, film, args)
+        lt = secondary["light_tracing"]
+        log(f"light tracing: {lt['Mpaths_s']:.0f} Mpaths/s ({lt['ms']:.1f} ms), "
+            f"{lt['extension_rays_per_path']:.2f} extension + {lt['shadow_rays_per_path']:.2f} shadow rays per path")
     if not args.no_intersect:
         film.free()
         wfilm.free()

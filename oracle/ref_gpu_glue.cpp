@@ -1,4 +1,5 @@
-// ref_gpu_glue.cpp -- drives dort_amd::GpuPathRenderer inside dort (TEST INFRASTRUCTURE ONLY).
+// ref_gpu_glue.cpp -- drives dort_amd::GpuPathRenderer and GpuLightRenderer inside dort
+// (TEST INFRASTRUCTURE ONLY).
 //
 // Linked with dort's own objects and ref_driver.cpp into oracle/_ref/libdortref_gpu.so
 // (oracle/ref.mk).  It renders a scene assembled by ref_scene_create through the
@@ -104,6 +105,43 @@ int refgpu_render(void* ref_scene, const dgpu_scene_desc* d, const dgpu_render_o
     *out_seed = r.last_seed();
     *out_paths = r.last_stats().paths;
     *out_percent = progress.last;
+    return 0;
+  } catch (const std::exception& e) {
+    std::snprintf(err, err_len, "%s", e.what());
+    return 1;
+  }
+}
+
+// GpuLightRenderer (dort's "lt") in the same seam: the film's splat sums and splat_scale.
+int refgpu_render_light(void* ref_scene, const dgpu_scene_desc* d, const dgpu_render_opts* o,
+                        uint32_t sampler_seed, uint32_t chunk, float* rgb, float* out_scale, uint64_t* out_seed,
+                        uint64_t* out_paths, char* err, size_t err_len) {
+  try {
+    auto film = std::make_shared<dort::Film>(o->x_res, o->y_res,
+                                             std::make_shared<dort::BoxFilter>(dort::Vec2(0.5f, 0.5f)));
+    auto sampler = std::make_shared<dort::RandomSampler>(1, dort::Rng(sampler_seed));
+    dort_amd::GpuLightOptions opts;
+    opts.iterations = o->sample_end;
+    opts.min_depth = o->min_depth;
+    opts.max_depth = o->max_depth;
+    opts.iterations_per_chunk = chunk;
+    dort_amd::GpuLightRenderer r(ref_scene_scene(ref_scene), film, sampler, ref_scene_camera(ref_scene),
+                                 typed_scene(d), opts);
+    dort::CtxG ctx;
+    ctx.pool = std::make_shared<dort::ThreadPool>();
+    ctx.pool->start(1);
+    CountingProgress progress;
+    r.render(ctx, progress);
+    ctx.pool->stop();
+    for (size_t i = 0; i < film->pixels.size(); ++i) {
+      dort::Spectrum sp = film->pixels[i].splat.load_relaxed();
+      rgb[3 * i] = sp.red();
+      rgb[3 * i + 1] = sp.green();
+      rgb[3 * i + 2] = sp.blue();
+    }
+    *out_scale = film->splat_scale;
+    *out_seed = r.last_seed();
+    *out_paths = r.last_stats().paths;
     return 0;
   } catch (const std::exception& e) {
     std::snprintf(err, err_len, "%s", e.what());

@@ -1,4 +1,5 @@
-"""dort_amd::GpuPathRenderer in dort's own Renderer seam (renderer.hpp:12-34).
+"""dort_amd::GpuPathRenderer and GpuLightRenderer in dort's own Renderer seam
+(renderer.hpp:12-34).
 
 oracle/_ref/libdortref_gpu.so is dort (its pt-path sources, built in place) with
 the GPU renderer of dort_amd/host linked in.  The tests build a dort::Scene, a Film
@@ -9,7 +10,9 @@ Renderer::render on the GPU renderer, as dort's Lua front-end does for "pt":
   nothing and drops nothing);
 * the image agrees with dort's own PathRenderer (CPU) on the same scene;
 * Progress cancellation stops between chunks, and an unsupported film filter is
-  rejected with an exception, dort's only error channel.
+  rejected with an exception, dort's only error channel;
+* GpuLightRenderer ("lt") fills the film's splat sums and splat_scale as
+  LightRenderer::render does, with what the C-ABI light tracer gives for the seed.
 Skipped where the reference checkout was not there to build the library.
 """
 import numpy as np
@@ -72,3 +75,16 @@ def test_renderer_seam_progress_cancellation():
 def test_renderer_seam_rejects_other_filters():
     with pytest.raises(RuntimeError, match="box filter"):
         ref.GpuRendererInDort(scenes.cornell()).render(_opts(4), filter_kind=1)
+
+
+def test_light_renderer_seam_film_equals_c_abi_render(gpu_ctx):
+    sc = scenes.dort_test_box("area")
+    opts = abi.render_opts(64, 48, spp=12, seed=0, max_depth=3)
+    rgb, info = ref.GpuRendererInDort(sc).render_light(opts, sampler_seed=5, chunk=4)
+    assert info["paths"] == 64 * 48 * 12 and info["splat_scale"] == pytest.approx(1 / 12)
+    direct = abi.RenderOpts.from_buffer_copy(opts)
+    direct.seed = info["seed"]
+    rgb2, _ = gpu_ctx.scene(sc).render_light(direct)
+    # the same light paths; splat sums differ only in the order of float atomics
+    np.testing.assert_allclose(rgb, rgb2, rtol=1e-4, atol=1e-5)
+    assert rgb.sum() > 0
