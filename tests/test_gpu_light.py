@@ -91,3 +91,17 @@ def test_lt_agrees_with_pt(gpu_ctx, kind):
     assert np.allclose(a, b, rtol=0.02), (a, b)
     msg = convergence.test_convergence(pt, lt)
     assert msg == "", msg
+
+
+def test_lt_errors(gpu_ctx):
+    from dort_amd import gpu
+    s = gpu_ctx.scene(scenes.dort_test_box("area"))
+    with pytest.raises(gpu.DgpuError):
+        s.render_light(abi.render_opts(16, 16, spp=1, min_depth=3, max_depth=2))
+    with pytest.raises(gpu.DgpuError):
+        s.render_light(abi.render_opts(16, 16, spp=1, max_depth=100))
+    with pytest.raises(gpu.DgpuError):
+        s.render_light(abi.render_opts(0, 16, spp=1))
+    # an empty sample range adds nothing
+    film, st = s.render_light(abi.render_opts(16, 16, spp=0))
+    assert st["paths"] == 0 and not film.any()
