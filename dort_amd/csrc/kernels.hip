@@ -138,9 +138,9 @@ __global__ __launch_bounds__(BLOCK) DG_TRACE_WAVES void k_trace_api(ApiIO io, ui
   trace_loop<ANY, WIDTH, LDS, BLOCK, FAST>(io, n, fetch, spill, stack_cap);
 }
 
-// Batched queues carry the scene index in the top byte of the path / slot word.
-constexpr uint32_t SCENE_SHIFT = 24;
-constexpr uint32_t LOW24 = 0x00FFFFFFu;
+// Batched queues carry the scene index in the top bits of the path / slot word, from
+// bit WaveParams::scene_shift up (as few bits as the batch needs, so the chunk of
+// paths, whose slot words fill the bits below, can be as large as possible).
 
 template <bool BATCH>
 struct ExtendIO {
@@ -150,12 +150,13 @@ struct ExtendIO {
   float4* hit;
   BvhRef bvh;            // single scene
   const BvhRef* table;   // batch: per-scene BVHs (in LDS)
+  uint32_t shift;        // batch: first bit of the scene index
   __device__ __forceinline__ BvhRef load(uint32_t i, TraceRay& r) const {
     float4 o = qo[i], d = qd[i];
     r.o = xyz(o); r.tmin = o.w;
     r.d = xyz(d); r.tmax = d.w;
     r.ignore = 0xFFFFFFFFu;
-    if (BATCH) return table[qpath[i] >> SCENE_SHIFT];
+    if (BATCH) return table[qpath[i] >> shift];
     return bvh;
   }
   __device__ __forceinline__ void store(uint32_t i, const TraceHit& h, bool found) const {
@@ -167,21 +168,22 @@ template <bool BATCH>
 struct ConnectIO {
   const float4* so;
   const float4* sd;
-  const uint2* meta;   // (pending slot [| scene << 24], ignored triangle)
+  const uint2* meta;   // (pending slot [| scene << shift], ignored triangle)
   float* pend_w;        // .w of the pending NEE record: 1 = visible so far
   BvhRef bvh;
   const BvhRef* table;
+  uint32_t shift;
   __device__ __forceinline__ BvhRef load(uint32_t i, TraceRay& r) const {
     float4 o = so[i], d = sd[i];
     r.o = xyz(o); r.tmin = o.w;
     r.d = xyz(d); r.tmax = d.w;
     uint2 m = meta[i];
     r.ignore = m.y;
-    if (BATCH) return table[m.x >> SCENE_SHIFT];
+    if (BATCH) return table[m.x >> shift];
     return bvh;
   }
   __device__ __forceinline__ void store(uint32_t i, const TraceHit&, bool found) const {
-    if (found) pend_w[4 * (size_t)(BATCH ? (meta[i].x & LOW24) : meta[i].x)] = 0.f;
+    if (found) pend_w[4 * (size_t)(BATCH ? (meta[i].x & ((1u << shift) - 1u)) : meta[i].x)] = 0.f;
   }
 };
 
@@ -281,7 +283,7 @@ __global__ void k_raygen(WaveParams P) {
   // pending NEE) without reading it, and finalize recomputes the jitter's edge test
   P.qo[0][i] = make_float4(o.x, o.y, o.z, 0.f);
   P.qd[0][i] = make_float4(d.x, d.y, d.z, INFINITY);
-  P.qpath[0][i] = i | (scene << SCENE_SHIFT);
+  P.qpath[0][i] = i | (scene << P.scene_shift);
 }
 
 // A jitter of exactly 0 puts iteration `sample` (> 0) on its pixel's lower edge; dort's
@@ -455,7 +457,7 @@ __device__ __forceinline__ void shade_body(const WaveParams& P, const DevScene& 
     if (bounce == 0u) {
       // the path starts here (k_raygen writes only the queue)
       thr = v3(1.f, 1.f, 1.f);
-      flags = scene << SCENE_SHIFT;
+      flags = scene << P.scene_shift;
       for (uint32_t k = 2u * (ALL ? P.S.num_lights : 1u); k < SL; ++k)  // slots no light writes
         P.pend[(size_t)p * SL + k] = make_float4(0.f, 0.f, 0.f, 0.f);
     } else {
@@ -498,7 +500,7 @@ __device__ __forceinline__ void shade_body(const WaveParams& P, const DevScene& 
   bool do_nee = shading && bounces >= P.min_depth && bsdf.matches(BSDF_ALL & ~BSDF_DELTA) &&
                 S.num_lights > 0u;
   uint32_t lights_loop = ALL ? P.S.num_lights : 1u;
-  const uint32_t scene_bits = scene << SCENE_SHIFT;
+  const uint32_t scene_bits = scene << P.scene_shift;
   const bool use_bsdf = P.strategy != DGPU_DIRECT_LIGHT, use_light = P.strategy != DGPU_DIRECT_BSDF;
   for (uint32_t li = 0; li < lights_loop; ++li) {
     // one light: all of them in turn (sample_all_lights), or one picked by power with
@@ -614,8 +616,8 @@ void k_shade(WaveParams P, uint32_t bounce) {
       uint2 ps = make_uint2(0u, 0u);
       if (active) {
         uint32_t q = P.qpath[cur][i];
-        p = q & LOW24;
-        scene = q >> SCENE_SHIFT;
+        p = q & ((1u << P.scene_shift) - 1u);
+        scene = q >> P.scene_shift;
         ps = P.pid[p];
       }
       const BatchEntry& E = P.batch[scene];
@@ -1259,7 +1261,7 @@ hipError_t launch_extend(const WaveParams& P, const KernelCfg& cfg, uint32_t bou
   ExtendIO<false> io{P.qo[bounce & 1u], P.qd[bounce & 1u], P.qpath[bounce & 1u], P.hit,
                      BvhRef{P.S.nodes, P.S.tris}, nullptr};
   ExtendIO<true> iob{P.qo[bounce & 1u], P.qd[bounce & 1u], P.qpath[bounce & 1u], P.hit,
-                     BvhRef{nullptr, nullptr}, nullptr};
+                     BvhRef{nullptr, nullptr}, nullptr, P.scene_shift};
 #define X(W, L, B)                                                                               \
   if (cfg.width == W && cfg.lds == L && cfg.block == B) {                                        \
     if (batch) hipLaunchKernelGGL((k_extend<W, L, B, true>), dim3(cfg.grid), dim3(B), 0, st, iob, \
@@ -1279,7 +1281,7 @@ hipError_t launch_connect(const WaveParams& P, const KernelCfg& cfg, uint32_t bo
   uint32_t* fetch = P.counters + CNT_FETCH_SHD + bounce;
   float* pend_w = (float*)P.pend + 3;
   ConnectIO<false> io{P.so, P.sd, P.smeta, pend_w, BvhRef{P.S.nodes, P.S.tris}, nullptr};
-  ConnectIO<true> iob{P.so, P.sd, P.smeta, pend_w, BvhRef{nullptr, nullptr}, nullptr};
+  ConnectIO<true> iob{P.so, P.sd, P.smeta, pend_w, BvhRef{nullptr, nullptr}, nullptr, P.scene_shift};
 #define X(W, L, B)                                                                                \
   if (cfg.width == W && cfg.lds == L && cfg.block == B) {                                         \
     if (batch) hipLaunchKernelGGL((k_connect<W, L, B, true>), dim3(cfg.grid), dim3(B), 0, st, iob, \

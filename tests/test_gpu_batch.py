@@ -49,3 +49,25 @@ def test_batch_rejects_mismatched_options(gpu_ctx):
     o2.max_depth = 3
     with pytest.raises(gpu.DgpuError):
         gpu_ctx.render_batch([(gpu_ctx.scene(s1), o1), (gpu_ctx.scene(s2), o2)])
+
+
+@pytest.mark.parametrize("count", [1, 2, 130, 255])
+def test_batch_scene_index_widths(gpu_ctx, count):
+    """The scene index takes ceil(log2(count)) bits of the path and slot words (at least
+    one), and the chunk size the bits below: from 1 scene to the limit of 255, and with
+    chunks that end inside scenes, every film equals the scene's own render."""
+    base = scenes.cornell()
+    uniq = [gpu_ctx.scene(base, queue_tile=500), gpu_ctx.scene(scenes.dort_test_box("area"), queue_tile=500)]
+    items = []
+    for k in range(count):
+        o = abi.render_opts(6 + k % 5, 4 + k % 3, spp=1 + k % 2, seed=1000 + k, max_depth=4, rr_depth=2,
+                            scene_id=k)
+        items.append((uniq[k % 2], o))
+    films, st = gpu_ctx.render_batch(items)
+    assert st["paths"] == sum(o.x_res * o.y_res * (o.sample_end - o.sample_begin) for _, o in items)
+    for k in (0, count // 2, count - 1):
+        gsc, o = items[k]
+        ref_rgb, ref_w, _ = gsc.render(o)
+        rgb, w = films[k]
+        assert np.array_equal(w, ref_w)
+        np.testing.assert_allclose(rgb, ref_rgb, rtol=1e-5, atol=1e-6)
