@@ -1100,15 +1100,15 @@ dgpu_status dgpu_render_batch(dgpu_ctx* ctx, const dgpu_batch_item* items, uint3
   }
 
   // chunks over the scene-major concatenation of every scene's paths.  The scene index
-  // takes the top scene_bits of the path and NEE-slot words; the slot index below it is
-  // 2 * path + k, so a chunk holds at most 2^(scene_shift - 1) paths.  Larger chunks
-  // mean fewer bounce launches and shorter tails (64 scenes: 2^25-path chunks rendered
-  // the batch workload in 225 ms, 2^23 in 319 ms).
+  // takes the top scene_bits of the path and NEE-slot words, the path the bits below,
+  // so a chunk holds at most 2^scene_shift paths.  Larger chunks mean fewer bounce
+  // launches and shorter tails (64 scenes: 2^25-path chunks rendered the batch workload
+  // in 227 ms, 2^23 in 319 ms).
   uint32_t scene_bits = 1;
   while ((1u << scene_bits) < n) ++scene_bits;
   const uint32_t scene_shift = 32u - scene_bits;
   uint32_t C = (uint32_t)std::min<uint64_t>(s0->queue_tile ? s0->queue_tile : ctx->default_tile(2),
-                                            1ull << (scene_shift - 1u));
+                                            1ull << scene_shift);
   std::vector<ChunkDesc> chunks;
   {
     uint32_t si = 0;

@@ -168,7 +168,7 @@ template <bool BATCH>
 struct ConnectIO {
   const float4* so;
   const float4* sd;
-  const uint2* meta;   // (pending slot [| scene << shift], ignored triangle)
+  const uint2* meta;   // (pending slot, ignored triangle); batch: see load()
   float* pend_w;        // .w of the pending NEE record: 1 = visible so far
   BvhRef bvh;
   const BvhRef* table;
@@ -178,12 +178,20 @@ struct ConnectIO {
     r.o = xyz(o); r.tmin = o.w;
     r.d = xyz(d); r.tmax = d.w;
     uint2 m = meta[i];
+    if (BATCH) {
+      // batch: (path | scene << shift, ignored triangle | NEE slot k << 31); a leaf-order
+      // index is below 2^27, so "none" (0x7FFFFFFF after the mask) matches no triangle
+      r.ignore = m.y & 0x7FFFFFFFu;
+      return table[m.x >> shift];
+    }
     r.ignore = m.y;
-    if (BATCH) return table[m.x >> shift];
     return bvh;
   }
   __device__ __forceinline__ void store(uint32_t i, const TraceHit&, bool found) const {
-    if (found) pend_w[4 * (size_t)(BATCH ? (meta[i].x & ((1u << shift) - 1u)) : meta[i].x)] = 0.f;
+    if (!found) return;
+    uint2 m = meta[i];
+    size_t slot = BATCH ? 2 * (size_t)(m.x & ((1u << shift) - 1u)) + (m.y >> 31) : m.x;
+    pend_w[4 * slot] = 0.f;
   }
 };
 
@@ -433,6 +441,16 @@ __device__ __forceinline__ void emit_shadow(const WaveParams& P, const NeeOut& s
   P.smeta[qi] = make_uint2(slot_word, s.ignore);
 }
 
+// Batched queues: the slot word holds the path and the scene (path | scene << shift),
+// the NEE slot k (0: BSDF strategy, 1: light strategy) rides in bit 31 of the ignored-
+// triangle word, so a chunk may hold 2^shift paths.
+__device__ __forceinline__ void emit_shadow_batch(const WaveParams& P, const NeeOut& s, uint32_t path_word,
+                                                  uint32_t k, uint32_t qi) {
+  P.so[qi] = make_float4(s.o.x, s.o.y, s.o.z, s.tmin);
+  P.sd[qi] = make_float4(s.d.x, s.d.y, s.d.z, s.tmax);
+  P.smeta[qi] = make_uint2(path_word, (s.ignore & 0x7FFFFFFFu) | (k << 31));
+}
+
 // One bounce of PathRenderer::sample for every path in the current queue.
 // Every lane of a wave runs the same number of queue appends (inactive lanes
 // append nothing), so the ballot-based compaction sees the whole wave.
@@ -533,7 +551,8 @@ __device__ __forceinline__ void shade_body(const WaveParams& P, const DevScene& 
       uint32_t qb = block_append(&P.counters[CNT_SHADOW + bounce], nb.emit, sh);
       if (nb.emit) {
         P.pend[slot_b] = make_float4(nb.contrib.x, nb.contrib.y, nb.contrib.z, 1.f);
-        emit_shadow(P, nb, slot_b | scene_bits, qb);
+        if (BATCH) emit_shadow_batch(P, nb, p | scene_bits, 0u, qb);
+        else emit_shadow(P, nb, slot_b, qb);
       } else if (active && bounce == 0u) {
         P.pend[slot_b] = make_float4(0.f, 0.f, 0.f, 0.f);
       }
@@ -548,7 +567,8 @@ __device__ __forceinline__ void shade_body(const WaveParams& P, const DevScene& 
       uint32_t ql = block_append(&P.counters[CNT_SHADOW + bounce], nl.emit, sh);
       if (nl.emit) {
         P.pend[slot_l] = make_float4(nl.contrib.x, nl.contrib.y, nl.contrib.z, 1.f);
-        emit_shadow(P, nl, slot_l | scene_bits, ql);
+        if (BATCH) emit_shadow_batch(P, nl, p | scene_bits, 1u, ql);
+        else emit_shadow(P, nl, slot_l, ql);
       } else if (active && bounce == 0u) {
         P.pend[slot_l] = make_float4(0.f, 0.f, 0.f, 0.f);
       }
