@@ -109,6 +109,16 @@ struct dgpu_ctx {
     return std::max<uint64_t>(1ull << 20, budget / per_path);
   }
 
+  // Paths per batch chunk: up to a quarter of the free device memory, at most 32 GiB
+  // (143M paths; chunks larger than about 2^27 paths measured no faster).
+  uint64_t batch_tile(uint32_t slots) {
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
+    const uint64_t per_path = 128ull + 56ull * slots;  // default_tile's, plus (pixel, sample)
+    uint64_t budget = std::min<uint64_t>(32ull << 30, free_b / 4);
+    return std::max<uint64_t>(1ull << 20, budget / per_path);
+  }
+
   // Workgroups of the strided shade kernel: enough to fill every CU.
   uint32_t shade_blocks(bool batch, bool all) {
     int& cached = batch ? shade_blocks_batch : (all ? shade_blocks_all : shade_blocks_single);
@@ -1099,16 +1109,20 @@ dgpu_status dgpu_render_batch(dgpu_ctx* ctx, const dgpu_batch_item* items, uint3
     }
   }
 
-  // chunks over the scene-major concatenation of every scene's paths.  The scene index
-  // takes the top scene_bits of the path and NEE-slot words, the path the bits below,
-  // so a chunk holds at most 2^scene_shift paths.  Larger chunks mean fewer bounce
-  // launches and shorter tails (64 scenes: 2^25-path chunks rendered the batch workload
-  // in 227 ms, 2^23 in 319 ms).
-  uint32_t scene_bits = 1;
-  while ((1u << scene_bits) < n) ++scene_bits;
-  const uint32_t scene_shift = 32u - scene_bits;
-  uint32_t C = (uint32_t)std::min<uint64_t>(s0->queue_tile ? s0->queue_tile : ctx->default_tile(2),
-                                            1ull << scene_shift);
+  // chunks over the scene-major concatenation of every scene's paths.  Queue words hold
+  // chunk path indices (NEE slot 2 * path + k), and the kernels find a path's scene by a
+  // search over the chunk's segment starts, so a chunk can hold the whole batch when
+  // memory allows.  Larger chunks mean fewer bounce launches and shorter tails (64
+  // scenes: 2^23-path chunks rendered the batch workload in 319 ms, 2^25 in 227 ms,
+  // 2^26 in 212 ms, and chunks of a third of the batch, 110M paths, in 202 ms; 2^28 and
+  // one 331M-path chunk took 207 ms).  Chunks are cut to equal sizes.
+  uint64_t total = 0;
+  for (uint64_t p : paths) total += p;
+  uint64_t C = std::min<uint64_t>(s0->queue_tile ? s0->queue_tile : ctx->batch_tile(2), 0x7FFFFFFFull);
+  if (total > C) {
+    uint64_t nch = (total + C - 1) / C;
+    C = (total + nch - 1) / nch;
+  }
   std::vector<ChunkDesc> chunks;
   {
     uint32_t si = 0;
@@ -1138,8 +1152,6 @@ dgpu_status dgpu_render_batch(dgpu_ctx* ctx, const dgpu_batch_item* items, uint3
       chunks.push_back(cd);
     }
   }
-  uint64_t total = 0;
-  for (uint64_t p : paths) total += p;
   if (chunks.empty()) return DGPU_OK;
   uint32_t cap = 0;
   for (const ChunkDesc& cd : chunks) cap = std::max(cap, cd.n_chunk);
@@ -1179,7 +1191,6 @@ dgpu_status dgpu_render_batch(dgpu_ctx* ctx, const dgpu_batch_item* items, uint3
   if (st != DGPU_OK) return st;
   P.batch = dtab;
   P.num_scenes = n;
-  P.scene_shift = scene_shift;
   P.chunks = dchunks;
   P.chunk_cur = dcur;
   HIP_TRY(hipMemcpyAsync(dtab, tab.data(), n * sizeof(BatchEntry), hipMemcpyHostToDevice, stream));

@@ -26,7 +26,7 @@ constexpr uint32_t SHADE_BLOCK = DGPU_SHADE_BLOCK;  // threads per shade workgro
 #endif
 constexpr uint32_t TRACE_BATCH = DGPU_TRACE_BATCH;  // 64-ray batches a trace wave takes per atomic
 
-constexpr uint32_t MAX_BATCH = 255;      // scenes per batch (scene index in the top bits)
+constexpr uint32_t MAX_BATCH = 255;      // scenes per batch
 constexpr uint32_t MAX_CHUNK_SEGS = 64;  // scenes overlapping one chunk of a batch
 
 struct KernelCfg {
@@ -90,7 +90,6 @@ struct WaveParams {
   const ChunkDesc* chunks;
   uint32_t* chunk_cur;     // index of the chunk being rendered (advanced on the device)
   uint2* pid;              // per path slot: (pixel, sample)
-  uint32_t scene_shift;    // batch: first bit of the scene index in path and slot words
   // light tracing (dgpu_render_light)
   uint32_t min_len, max_len;  // path lengths in vertices (LightRenderer min_length / max_length)
   float2* fpos;            // film position of each camera-connection shadow ray

@@ -138,9 +138,28 @@ __global__ __launch_bounds__(BLOCK) DG_TRACE_WAVES void k_trace_api(ApiIO io, ui
   trace_loop<ANY, WIDTH, LDS, BLOCK, FAST>(io, n, fetch, spill, stack_cap);
 }
 
-// Batched queues carry the scene index in the top bits of the path / slot word, from
-// bit WaveParams::scene_shift up (as few bits as the batch needs, so the chunk of
-// paths, whose slot words fill the bits below, can be as large as possible).
+// Batched queues carry chunk path indices, like a single render's: a path's scene is the
+// chunk segment holding it.  The segments of the current chunk (their first path and
+// scene) sit in LDS, and a binary search over the starts finds a path's segment.
+struct SegTable {
+  const uint32_t* begin;  // first path of each segment (ascending; begin[0] = 0)
+  const BvhRef* bvh;      // trace kernels: the segment's scene BVH
+  uint32_t n;
+};
+
+// The segment holding chunk path p: the last one whose first path is <= p.
+__device__ __forceinline__ uint32_t seg_of(const uint32_t* begin, uint32_t n, uint32_t p) {
+  uint32_t b = 0;
+#pragma unroll
+  for (uint32_t step = MAX_CHUNK_SEGS / 2u; step > 0u; step >>= 1)
+    if (b + step < n && begin[b + step] <= p) b += step;
+  return b;
+}
+
+// The BVH of the segment holding chunk path p.
+__device__ __forceinline__ BvhRef seg_bvh_of(const SegTable& t, uint32_t p) {
+  return t.bvh[seg_of(t.begin, t.n, p)];
+}
 
 template <bool BATCH>
 struct ExtendIO {
@@ -149,14 +168,13 @@ struct ExtendIO {
   const uint32_t* qpath;
   float4* hit;
   BvhRef bvh;            // single scene
-  const BvhRef* table;   // batch: per-scene BVHs (in LDS)
-  uint32_t shift;        // batch: first bit of the scene index
+  SegTable segs;         // batch: the chunk's segments (in LDS)
   __device__ __forceinline__ BvhRef load(uint32_t i, TraceRay& r) const {
     float4 o = qo[i], d = qd[i];
     r.o = xyz(o); r.tmin = o.w;
     r.d = xyz(d); r.tmax = d.w;
     r.ignore = 0xFFFFFFFFu;
-    if (BATCH) return table[qpath[i] >> shift];
+    if (BATCH) return seg_bvh_of(segs, qpath[i]);
     return bvh;
   }
   __device__ __forceinline__ void store(uint32_t i, const TraceHit& h, bool found) const {
@@ -171,18 +189,17 @@ struct ConnectIO {
   const uint2* meta;   // (pending slot, ignored triangle); batch: see load()
   float* pend_w;        // .w of the pending NEE record: 1 = visible so far
   BvhRef bvh;
-  const BvhRef* table;
-  uint32_t shift;
+  SegTable segs;        // batch: the chunk's segments (in LDS)
   __device__ __forceinline__ BvhRef load(uint32_t i, TraceRay& r) const {
     float4 o = so[i], d = sd[i];
     r.o = xyz(o); r.tmin = o.w;
     r.d = xyz(d); r.tmax = d.w;
     uint2 m = meta[i];
     if (BATCH) {
-      // batch: (path | scene << shift, ignored triangle | NEE slot k << 31); a leaf-order
-      // index is below 2^27, so "none" (0x7FFFFFFF after the mask) matches no triangle
+      // batch: (path, ignored triangle | NEE slot k << 31); a leaf-order index is below
+      // 2^27, so "none" (0x7FFFFFFF after the mask) matches no triangle
       r.ignore = m.y & 0x7FFFFFFFu;
-      return table[m.x >> shift];
+      return seg_bvh_of(segs, m.x);
     }
     r.ignore = m.y;
     return bvh;
@@ -190,38 +207,50 @@ struct ConnectIO {
   __device__ __forceinline__ void store(uint32_t i, const TraceHit&, bool found) const {
     if (!found) return;
     uint2 m = meta[i];
-    size_t slot = BATCH ? 2 * (size_t)(m.x & ((1u << shift) - 1u)) + (m.y >> 31) : m.x;
+    size_t slot = BATCH ? 2 * (size_t)m.x + (m.y >> 31) : m.x;
     pend_w[4 * slot] = 0.f;
   }
 };
 
+// The current chunk's segments into the workgroup's LDS: first paths and scene BVHs.
 template <bool BATCH, int BLOCK>
-__device__ __forceinline__ const BvhRef* load_bvh_table(const BatchEntry* batch, uint32_t nscenes,
-                                                        BvhRef* lds) {
-  if (!BATCH) return nullptr;
-  for (uint32_t k = threadIdx.x; k < nscenes; k += BLOCK) {
-    lds[k].nodes = batch[k].S.nodes;
-    lds[k].tris = batch[k].S.tris;
+__device__ __forceinline__ SegTable load_seg_table(const BatchEntry* batch, const ChunkDesc* chunks,
+                                                   const uint32_t* chunk_cur, uint32_t* begin, BvhRef* bvh) {
+  SegTable t{nullptr, nullptr, 0u};
+  if (!BATCH) return t;
+  const ChunkDesc& cd = chunks[*chunk_cur];
+  t.n = cd.n_segs;
+  for (uint32_t k = threadIdx.x; k < t.n; k += BLOCK) {
+    const ChunkSeg& sg = cd.segs[k];
+    begin[k] = sg.slot_begin;
+    bvh[k].nodes = batch[sg.scene].S.nodes;
+    bvh[k].tris = batch[sg.scene].S.tris;
   }
   __syncthreads();
-  return lds;
+  t.begin = begin;
+  t.bvh = bvh;
+  return t;
 }
 
 template <int WIDTH, int LDS, int BLOCK, bool BATCH>
 __global__ __launch_bounds__(BLOCK) DG_TRACE_WAVES void k_extend(ExtendIO<BATCH> io, const uint32_t* count,
                                                   uint32_t* fetch, uint32_t* spill, uint32_t stack_cap,
-                                                  const BatchEntry* batch, uint32_t nscenes) {
-  __shared__ BvhRef tab[BATCH ? MAX_BATCH : 1];
-  io.table = load_bvh_table<BATCH, BLOCK>(batch, nscenes, tab);
+                                                  const BatchEntry* batch, const ChunkDesc* chunks,
+                                                  const uint32_t* chunk_cur) {
+  __shared__ uint32_t seg_begin[BATCH ? MAX_CHUNK_SEGS : 1];
+  __shared__ BvhRef seg_bvh[BATCH ? MAX_CHUNK_SEGS : 1];
+  io.segs = load_seg_table<BATCH, BLOCK>(batch, chunks, chunk_cur, seg_begin, seg_bvh);
   trace_loop<false, WIDTH, LDS, BLOCK, true>(io, *count, fetch, spill, stack_cap);
 }
 
 template <int WIDTH, int LDS, int BLOCK, bool BATCH>
 __global__ __launch_bounds__(BLOCK) DG_TRACE_WAVES void k_connect(ConnectIO<BATCH> io, const uint32_t* count,
                                                    uint32_t* fetch, uint32_t* spill, uint32_t stack_cap,
-                                                   const BatchEntry* batch, uint32_t nscenes) {
-  __shared__ BvhRef tab[BATCH ? MAX_BATCH : 1];
-  io.table = load_bvh_table<BATCH, BLOCK>(batch, nscenes, tab);
+                                                   const BatchEntry* batch, const ChunkDesc* chunks,
+                                                   const uint32_t* chunk_cur) {
+  __shared__ uint32_t seg_begin[BATCH ? MAX_CHUNK_SEGS : 1];
+  __shared__ BvhRef seg_bvh[BATCH ? MAX_CHUNK_SEGS : 1];
+  io.segs = load_seg_table<BATCH, BLOCK>(batch, chunks, chunk_cur, seg_begin, seg_bvh);
   trace_loop<true, WIDTH, LDS, BLOCK, true>(io, *count, fetch, spill, stack_cap);
 }
 
@@ -238,7 +267,7 @@ __global__ void k_advance(uint32_t* cur) {
 // ----------------------------------------------------------------- path state
 
 // thr.w holds flags: bits 0..7 bounces, bit 8 last bounce delta, bit 9 no film weight,
-// bits 24..31 the scene of a batched path.
+// the rest zero.
 constexpr uint32_t F_BOUNCES = 0xFFu;
 constexpr uint32_t F_DELTA = 1u << 8;
 
@@ -291,7 +320,7 @@ __global__ void k_raygen(WaveParams P) {
   // pending NEE) without reading it, and finalize recomputes the jitter's edge test
   P.qo[0][i] = make_float4(o.x, o.y, o.z, 0.f);
   P.qd[0][i] = make_float4(d.x, d.y, d.z, INFINITY);
-  P.qpath[0][i] = i | (scene << P.scene_shift);
+  P.qpath[0][i] = i;
 }
 
 // A jitter of exactly 0 puts iteration `sample` (> 0) on its pixel's lower edge; dort's
@@ -441,9 +470,8 @@ __device__ __forceinline__ void emit_shadow(const WaveParams& P, const NeeOut& s
   P.smeta[qi] = make_uint2(slot_word, s.ignore);
 }
 
-// Batched queues: the slot word holds the path and the scene (path | scene << shift),
-// the NEE slot k (0: BSDF strategy, 1: light strategy) rides in bit 31 of the ignored-
-// triangle word, so a chunk may hold 2^shift paths.
+// Batched queues: the slot word holds the chunk path, and the NEE slot k (0: BSDF
+// strategy, 1: light strategy) rides in bit 31 of the ignored-triangle word.
 __device__ __forceinline__ void emit_shadow_batch(const WaveParams& P, const NeeOut& s, uint32_t path_word,
                                                   uint32_t k, uint32_t qi) {
   P.so[qi] = make_float4(s.o.x, s.o.y, s.o.z, s.tmin);
@@ -475,7 +503,7 @@ __device__ __forceinline__ void shade_body(const WaveParams& P, const DevScene& 
     if (bounce == 0u) {
       // the path starts here (k_raygen writes only the queue)
       thr = v3(1.f, 1.f, 1.f);
-      flags = scene << P.scene_shift;
+      flags = 0u;
       for (uint32_t k = 2u * (ALL ? P.S.num_lights : 1u); k < SL; ++k)  // slots no light writes
         P.pend[(size_t)p * SL + k] = make_float4(0.f, 0.f, 0.f, 0.f);
     } else {
@@ -518,7 +546,6 @@ __device__ __forceinline__ void shade_body(const WaveParams& P, const DevScene& 
   bool do_nee = shading && bounces >= P.min_depth && bsdf.matches(BSDF_ALL & ~BSDF_DELTA) &&
                 S.num_lights > 0u;
   uint32_t lights_loop = ALL ? P.S.num_lights : 1u;
-  const uint32_t scene_bits = scene << P.scene_shift;
   const bool use_bsdf = P.strategy != DGPU_DIRECT_LIGHT, use_light = P.strategy != DGPU_DIRECT_BSDF;
   for (uint32_t li = 0; li < lights_loop; ++li) {
     // one light: all of them in turn (sample_all_lights), or one picked by power with
@@ -551,7 +578,7 @@ __device__ __forceinline__ void shade_body(const WaveParams& P, const DevScene& 
       uint32_t qb = block_append(&P.counters[CNT_SHADOW + bounce], nb.emit, sh);
       if (nb.emit) {
         P.pend[slot_b] = make_float4(nb.contrib.x, nb.contrib.y, nb.contrib.z, 1.f);
-        if (BATCH) emit_shadow_batch(P, nb, p | scene_bits, 0u, qb);
+        if (BATCH) emit_shadow_batch(P, nb, p, 0u, qb);
         else emit_shadow(P, nb, slot_b, qb);
       } else if (active && bounce == 0u) {
         P.pend[slot_b] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -567,7 +594,7 @@ __device__ __forceinline__ void shade_body(const WaveParams& P, const DevScene& 
       uint32_t ql = block_append(&P.counters[CNT_SHADOW + bounce], nl.emit, sh);
       if (nl.emit) {
         P.pend[slot_l] = make_float4(nl.contrib.x, nl.contrib.y, nl.contrib.z, 1.f);
-        if (BATCH) emit_shadow_batch(P, nl, p | scene_bits, 1u, ql);
+        if (BATCH) emit_shadow_batch(P, nl, p, 1u, ql);
         else emit_shadow(P, nl, slot_l, ql);
       } else if (active && bounce == 0u) {
         P.pend[slot_l] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -611,7 +638,7 @@ __device__ __forceinline__ void shade_body(const WaveParams& P, const DevScene& 
   if (cont) {
     P.qo[nxt][qn] = make_float4(next_o.x, next_o.y, next_o.z, next_eps);
     P.qd[nxt][qn] = make_float4(next_d.x, next_d.y, next_d.z, INFINITY);
-    P.qpath[nxt][qn] = p | scene_bits;
+    P.qpath[nxt][qn] = p;
   }
 }
 
@@ -626,8 +653,19 @@ template <bool BATCH, bool ALL>
 __global__ __launch_bounds__(SHADE_BLOCK) __attribute__((amdgpu_waves_per_eu(DGPU_SHADE_WAVES)))
 void k_shade(WaveParams P, uint32_t bounce) {
   __shared__ uint32_t sh[SHADE_BLOCK / 64 + 1];
+  __shared__ uint32_t seg_begin[BATCH ? MAX_CHUNK_SEGS : 1], seg_scene[BATCH ? MAX_CHUNK_SEGS : 1];
   const uint32_t cur = bounce & 1u;
   const uint32_t n = P.counters[CNT_QUEUE + bounce];
+  uint32_t nsegs = 0;
+  if (BATCH) {
+    const ChunkDesc& cd = P.chunks[*P.chunk_cur];
+    nsegs = cd.n_segs;
+    for (uint32_t k = threadIdx.x; k < nsegs; k += SHADE_BLOCK) {
+      seg_begin[k] = cd.segs[k].slot_begin;
+      seg_scene[k] = cd.segs[k].scene;
+    }
+    __syncthreads();
+  }
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
     uint32_t i = base + threadIdx.x;
     bool active = i < n;
@@ -635,13 +673,16 @@ void k_shade(WaveParams P, uint32_t bounce) {
       uint32_t p = 0, scene = 0;
       uint2 ps = make_uint2(0u, 0u);
       if (active) {
-        uint32_t q = P.qpath[cur][i];
-        p = q & ((1u << P.scene_shift) - 1u);
-        scene = q >> P.scene_shift;
+        p = P.qpath[cur][i];
+        scene = seg_scene[seg_of(seg_begin, nsegs, p)];
         ps = P.pid[p];
       }
-      const BatchEntry& E = P.batch[scene];
-      shade_body<true, false>(P, E.S, bounce, active, i, p, scene, ps.x, ps.y, E.k0, E.k1, sh);
+      // the batch table as constant memory: the compiler then knows the pointers read
+      // from it address global memory (else every material, light and texture load is
+      // a flat load) and that the shade's stores do not change it
+      using CEntry = const __attribute__((address_space(4))) BatchEntry;
+      const CEntry& E = ((CEntry*)P.batch)[scene];
+      shade_body<true, false>(P, *(const DevScene*)&E.S, bounce, active, i, p, scene, ps.x, ps.y, E.k0, E.k1, sh);
     } else {
       uint32_t p = active ? P.qpath[cur][i] : 0u;
       uint64_t gidx = P.g0 + p;
@@ -718,7 +759,8 @@ __global__ __launch_bounds__(256) void k_finalize(WaveParams P) {
       if (cd.segs[m].pair_begin <= t) b = m; else e = m;
     }
     const ChunkSeg& sg = cd.segs[b];
-    const BatchEntry& E = P.batch[sg.scene];
+    using CEntry = const __attribute__((address_space(4))) BatchEntry;
+    const CEntry& E = ((CEntry*)P.batch)[sg.scene];  // constant: the film pointers are global
     uint32_t q = t - sg.pair_begin;
     uint64_t local = sg.local_begin + q;
     uint32_t px = (uint32_t)(local % E.npix);
@@ -1279,15 +1321,15 @@ hipError_t launch_extend(const WaveParams& P, const KernelCfg& cfg, uint32_t bou
   const uint32_t* count = P.counters + CNT_QUEUE + bounce;
   uint32_t* fetch = P.counters + CNT_FETCH_EXT + bounce;
   ExtendIO<false> io{P.qo[bounce & 1u], P.qd[bounce & 1u], P.qpath[bounce & 1u], P.hit,
-                     BvhRef{P.S.nodes, P.S.tris}, nullptr};
+                     BvhRef{P.S.nodes, P.S.tris}, SegTable{nullptr, nullptr, 0u}};
   ExtendIO<true> iob{P.qo[bounce & 1u], P.qd[bounce & 1u], P.qpath[bounce & 1u], P.hit,
-                     BvhRef{nullptr, nullptr}, nullptr, P.scene_shift};
+                     BvhRef{nullptr, nullptr}, SegTable{nullptr, nullptr, 0u}};
 #define X(W, L, B)                                                                               \
   if (cfg.width == W && cfg.lds == L && cfg.block == B) {                                        \
     if (batch) hipLaunchKernelGGL((k_extend<W, L, B, true>), dim3(cfg.grid), dim3(B), 0, st, iob, \
-                                  count, fetch, P.spill, P.stack_cap, P.batch, P.num_scenes); \
+                                  count, fetch, P.spill, P.stack_cap, P.batch, P.chunks, P.chunk_cur); \
     else hipLaunchKernelGGL((k_extend<W, L, B, false>), dim3(cfg.grid), dim3(B), 0, st, io, count, \
-                            fetch, P.spill, P.stack_cap, P.batch, P.num_scenes);      \
+                            fetch, P.spill, P.stack_cap, P.batch, P.chunks, P.chunk_cur); \
     return hipGetLastError();                                                                    \
   }
   DG_VARIANTS(X)
@@ -1300,14 +1342,14 @@ hipError_t launch_connect(const WaveParams& P, const KernelCfg& cfg, uint32_t bo
   const uint32_t* count = P.counters + CNT_SHADOW + bounce;
   uint32_t* fetch = P.counters + CNT_FETCH_SHD + bounce;
   float* pend_w = (float*)P.pend + 3;
-  ConnectIO<false> io{P.so, P.sd, P.smeta, pend_w, BvhRef{P.S.nodes, P.S.tris}, nullptr};
-  ConnectIO<true> iob{P.so, P.sd, P.smeta, pend_w, BvhRef{nullptr, nullptr}, nullptr, P.scene_shift};
+  ConnectIO<false> io{P.so, P.sd, P.smeta, pend_w, BvhRef{P.S.nodes, P.S.tris}, SegTable{nullptr, nullptr, 0u}};
+  ConnectIO<true> iob{P.so, P.sd, P.smeta, pend_w, BvhRef{nullptr, nullptr}, SegTable{nullptr, nullptr, 0u}};
 #define X(W, L, B)                                                                                \
   if (cfg.width == W && cfg.lds == L && cfg.block == B) {                                         \
     if (batch) hipLaunchKernelGGL((k_connect<W, L, B, true>), dim3(cfg.grid), dim3(B), 0, st, iob, \
-                                  count, fetch, P.spill, P.stack_cap, P.batch, P.num_scenes); \
+                                  count, fetch, P.spill, P.stack_cap, P.batch, P.chunks, P.chunk_cur); \
     else hipLaunchKernelGGL((k_connect<W, L, B, false>), dim3(cfg.grid), dim3(B), 0, st, io, count, \
-                            fetch, P.spill, P.stack_cap, P.batch, P.num_scenes);       \
+                            fetch, P.spill, P.stack_cap, P.batch, P.chunks, P.chunk_cur); \
     return hipGetLastError();                                                                     \
   }
   DG_VARIANTS(X)

@@ -285,6 +285,17 @@ DG_INLINE void order_pair(float& da, uint32_t& ca, float& db, uint32_t& cb) {
 // stacked node, or TRAV_END.  Any-hit rays (ANY) stop at the first hit anywhere, so
 // only the nearest child is picked (a three-comparison tournament) and the other hit
 // children are pushed unsorted.
+// One 16 B word of the BVH or triangle array.  Both are device allocations, and the load
+// says so: a batched scene's BVH pointers are read from LDS, and pointers read from
+// memory are otherwise generic, which makes every fetch a flat load.
+DG_INLINE float4 bvh_word(const float4* base, size_t i) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return ((const __attribute__((address_space(1))) float4*)base)[i];
+#else
+  return base[i];  // the host pass of the same source (host callers read host arrays)
+#endif
+}
+
 template <int WIDTH, bool ANY, bool FAST>
 DG_INLINE void trav_inner(const float4* __restrict__ nodes, TravState& s, LaneStack& st, uint4 A, uint4 Q,
                           uint4 R, uint4 K) {
@@ -310,8 +321,9 @@ DG_INLINE void trav_inner(const float4* __restrict__ nodes, TravState& s, LaneSt
       return;
     }
   } else {
-    const float4* n = nodes + 4 * (size_t)s.node;
-    float4 n0 = n[0], n1 = n[1], n2 = n[2], n3 = n[3];
+    size_t n = 4 * (size_t)s.node;
+    float4 n0 = bvh_word(nodes, n), n1 = bvh_word(nodes, n + 1), n2 = bvh_word(nodes, n + 2),
+           n3 = bvh_word(nodes, n + 3);
     float t0 = box_enter<FAST>(n0.x, n0.y, n0.z, n0.w, n2.x, n2.y, s.o, s.inv, s.tmin, s.tmax);
     float t1 = box_enter<FAST>(n1.x, n1.y, n1.z, n1.w, n2.z, n2.w, s.o, s.inv, s.tmin, s.tmax);
     uint32_t c0 = __float_as_uint(n3.x), c1 = __float_as_uint(n3.y);
@@ -359,8 +371,8 @@ DG_INLINE void trav_step(const float4* __restrict__ nodes, const float4* __restr
     // node: 4 x 16 B at 4 * node; triangle: 3 x 16 B at 3 * first (both indices < 2^27)
     size_t idx = s.node & 0x07FFFFFFu;
     const float4* p = (leaf ? tris : nodes) + ((idx << 2) - (leaf ? idx : 0u));
-    a = p[0]; b = p[1]; c = p[2];
-    k = nodes[(leaf ? 0u : (idx << 2)) + 3u];
+    a = bvh_word(p, 0); b = bvh_word(p, 1); c = bvh_word(p, 2);
+    k = bvh_word(nodes, (leaf ? 0u : (idx << 2)) + 3u);
     // The triangle test reads only xyz and no k; without this the compiler moves the
     // node-only words into the node branch as separate loads.  Takes values, no memory
     // access.
@@ -372,7 +384,7 @@ DG_INLINE void trav_step(const float4* __restrict__ nodes, const float4* __restr
   }
   if (WIDTH != 4) {
     uint32_t ti = leaf_first(s.node);
-    a = tris[3 * ti]; b = tris[3 * ti + 1]; c = tris[3 * ti + 2];
+    a = bvh_word(tris, 3 * (size_t)ti); b = bvh_word(tris, 3 * (size_t)ti + 1); c = bvh_word(tris, 3 * (size_t)ti + 2);
   }
   if (leaf_tri<ANY>(a, b, c, s.node, s)) s.node = TRAV_END;
   else if (leaf_count(s.node) > 1u) s.node += 1u - (1u << 27);  // next triangle: first + 1, count - 1

@@ -51,13 +51,14 @@ def test_batch_rejects_mismatched_options(gpu_ctx):
         gpu_ctx.render_batch([(gpu_ctx.scene(s1), o1), (gpu_ctx.scene(s2), o2)])
 
 
-@pytest.mark.parametrize("count", [1, 2, 130, 255])
-def test_batch_scene_index_widths(gpu_ctx, count):
-    """The scene index takes ceil(log2(count)) bits of the path and slot words (at least
-    one), and the chunk size the bits below: from 1 scene to the limit of 255, and with
-    chunks that end inside scenes, every film equals the scene's own render."""
+@pytest.mark.parametrize("count,tile", [(1, 500), (2, 500), (130, 500), (255, 500), (130, 0), (255, 0)])
+def test_batch_chunk_segments(gpu_ctx, count, tile):
+    """The kernels find a path's scene by a binary search over the chunk's segment starts:
+    from 1 scene to the limit of 255, with 500-path chunks that end inside scenes and
+    with chunks of the most segments one chunk holds (64; 255 scenes make chunks of 64,
+    64, 64 and 63), every film equals the scene's own render."""
     base = scenes.cornell()
-    uniq = [gpu_ctx.scene(base, queue_tile=500), gpu_ctx.scene(scenes.dort_test_box("area"), queue_tile=500)]
+    uniq = [gpu_ctx.scene(base, queue_tile=tile), gpu_ctx.scene(scenes.dort_test_box("area"), queue_tile=tile)]
     items = []
     for k in range(count):
         o = abi.render_opts(6 + k % 5, 4 + k % 3, spp=1 + k % 2, seed=1000 + k, max_depth=4, rr_depth=2,
