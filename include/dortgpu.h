@@ -261,8 +261,10 @@ dgpu_status dgpu_render_light(dgpu_scene* scene, const dgpu_render_opts* opts, f
  * the device.  Scenes may differ in triangles, resolution, sample range and seed;
  * they must share the depth / strategy options, BVH width and kernel knobs, and a
  * context.  Each item's film is accumulated like dgpu_render's, so batch results
- * equal per-scene renders.  flags: DGPU_BUF_DEVICE (films are device pointers),
- * DGPU_BATCH_NO_GRAPH. */
+ * equal per-scene renders.  n is in [1, 255].  Chunks hold at most item 0's scene
+ * queue_tile paths, or by default equal shares of the batch of at most 32 GiB of path
+ * state each (about 143M paths); a chunk spans at most 64 scenes.
+ * flags: DGPU_BUF_DEVICE (films are device pointers), DGPU_BATCH_NO_GRAPH. */
 typedef struct {
   dgpu_scene* scene;
   dgpu_render_opts opts;
