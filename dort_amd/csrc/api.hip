@@ -110,11 +110,11 @@ struct dgpu_ctx {
   }
 
   // Paths per batch chunk: up to a quarter of the free device memory, at most 32 GiB
-  // (143M paths; chunks larger than about 2^27 paths measured no faster).
+  // (148M paths; chunks larger than about 2^27 paths measured no faster).
   uint64_t batch_tile(uint32_t slots) {
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
-    const uint64_t per_path = 128ull + 56ull * slots;  // default_tile's, plus (pixel, sample)
+    const uint64_t per_path = 120ull + 56ull * slots;  // as default_tile's
     uint64_t budget = std::min<uint64_t>(32ull << 30, free_b / 4);
     return std::max<uint64_t>(1ull << 20, budget / per_path);
   }
@@ -1180,7 +1180,6 @@ dgpu_status dgpu_render_batch(dgpu_ctx* ctx, const dgpu_batch_item* items, uint3
   P.so = (float4*)ctx->buf("so", (size_t)cap * 2 * 16, &st);
   P.sd = (float4*)ctx->buf("sd", (size_t)cap * 2 * 16, &st);
   P.smeta = (uint2*)ctx->buf("smeta", (size_t)cap * 2 * 8, &st);
-  P.pid = (uint2*)ctx->buf("pid", (size_t)cap * 8, &st);
   P.counters = (uint32_t*)ctx->buf("counters", NUM_COUNTERS * 4, &st);
   P.stats = (unsigned long long*)ctx->buf("stats", STATS_WORDS * 8, &st);
   P.spill = (uint32_t*)ctx->buf("spill", (size_t)maxgrid * s0->cfg.block * stack_cap * 4, &st);

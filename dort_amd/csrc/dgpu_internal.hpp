@@ -89,7 +89,6 @@ struct WaveParams {
   uint32_t num_scenes;
   const ChunkDesc* chunks;
   uint32_t* chunk_cur;     // index of the chunk being rendered (advanced on the device)
-  uint2* pid;              // per path slot: (pixel, sample)
   // light tracing (dgpu_render_light)
   uint32_t min_len, max_len;  // path lengths in vertices (LightRenderer min_length / max_length)
   float2* fpos;            // film position of each camera-connection shadow ray

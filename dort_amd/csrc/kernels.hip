@@ -299,7 +299,6 @@ __global__ void k_raygen(WaveParams P) {
     pixel = (uint32_t)(local % E.npix);
     xr = E.xr; yr = E.yr; k0 = E.k0; k1 = E.k1;
     S = &E.S;
-    P.pid[i] = make_uint2(pixel, sample);
   } else {
     uint64_t g = P.g0 + i;
     sample = P.sample_begin + (uint32_t)(g / P.npix);
@@ -654,6 +653,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) __attribute__((amdgpu_waves_per_eu(DGP
 void k_shade(WaveParams P, uint32_t bounce) {
   __shared__ uint32_t sh[SHADE_BLOCK / 64 + 1];
   __shared__ uint32_t seg_begin[BATCH ? MAX_CHUNK_SEGS : 1], seg_scene[BATCH ? MAX_CHUNK_SEGS : 1];
+  __shared__ uint64_t seg_local[BATCH ? MAX_CHUNK_SEGS : 1];
   const uint32_t cur = bounce & 1u;
   const uint32_t n = P.counters[CNT_QUEUE + bounce];
   uint32_t nsegs = 0;
@@ -663,6 +663,7 @@ void k_shade(WaveParams P, uint32_t bounce) {
     for (uint32_t k = threadIdx.x; k < nsegs; k += SHADE_BLOCK) {
       seg_begin[k] = cd.segs[k].slot_begin;
       seg_scene[k] = cd.segs[k].scene;
+      seg_local[k] = cd.segs[k].local_begin;
     }
     __syncthreads();
   }
@@ -670,19 +671,22 @@ void k_shade(WaveParams P, uint32_t bounce) {
     uint32_t i = base + threadIdx.x;
     bool active = i < n;
     if (BATCH) {
-      uint32_t p = 0, scene = 0;
-      uint2 ps = make_uint2(0u, 0u);
+      uint32_t p = 0, scene = 0, sg = 0;
       if (active) {
         p = P.qpath[cur][i];
-        scene = seg_scene[seg_of(seg_begin, nsegs, p)];
-        ps = P.pid[p];
+        sg = seg_of(seg_begin, nsegs, p);
+        scene = seg_scene[sg];
       }
       // the batch table as constant memory: the compiler then knows the pointers read
       // from it address global memory (else every material, light and texture load is
       // a flat load) and that the shade's stores do not change it
       using CEntry = const __attribute__((address_space(4))) BatchEntry;
       const CEntry& E = ((CEntry*)P.batch)[scene];
-      shade_body<true, false>(P, *(const DevScene*)&E.S, bounce, active, i, p, scene, ps.x, ps.y, E.k0, E.k1, sh);
+      // the path's pixel and sample, as k_raygen derived them from its segment
+      uint64_t local = seg_local[sg] + (p - seg_begin[sg]);
+      uint32_t sample = E.sample_begin + (uint32_t)(local / E.npix), pixel = (uint32_t)(local % E.npix);
+      shade_body<true, false>(P, *(const DevScene*)&E.S, bounce, active, i, p, scene, pixel, sample, E.k0, E.k1,
+                              sh);
     } else {
       uint32_t p = active ? P.qpath[cur][i] : 0u;
       uint64_t gidx = P.g0 + p;

@@ -263,7 +263,7 @@ dgpu_status dgpu_render_light(dgpu_scene* scene, const dgpu_render_opts* opts, f
  * context.  Each item's film is accumulated like dgpu_render's, so batch results
  * equal per-scene renders.  n is in [1, 255].  Chunks hold at most item 0's scene
  * queue_tile paths, or by default equal shares of the batch of at most 32 GiB of path
- * state each (about 143M paths); a chunk spans at most 64 scenes.
+ * state each (about 148M paths); a chunk spans at most 64 scenes.
  * flags: DGPU_BUF_DEVICE (films are device pointers), DGPU_BATCH_NO_GRAPH. */
 typedef struct {
   dgpu_scene* scene;
