@@ -182,46 +182,59 @@ def adjoint_bench(ctx, args):
 def batch_bench(ctx, args):
     """64 independent scenes, triangle counts log-uniform in [1k, 500k] (displaced
     geodesic spheres and boxes on a ground plane), resolutions uniform in {256^2, 512^2,
-    768^2}, 16 spp, max depth 6, sky-lit.  Batched wavefront (one hipGraph replayed per
-    chunk) against one dgpu_render launch sequence per scene; films in HBM for both."""
+    768^2}, 16 spp, max depth 6, each lit by a 1024x512 sky.  Batched wavefront (one
+    hipGraph replayed per chunk) against one dgpu_render launch sequence per scene; each
+    leg has its own films in HBM, and before timing the batched films are checked
+    against the per-scene ones (weights bitwise, radiance to rtol 1e-5)."""
     from dort_amd import abi, gpu, scenes
     t0 = time.time()
     batch = scenes.batch_scenes(64, seed=3)
-    items, films = [], []
+    items, films_p, films_b = [], [], []
+
+    def zeroed(n):
+        a = gpu.DeviceArray(ctx, n, np.float32)
+        a.copy_from(np.zeros(n, np.float32))
+        return a
+
     for k, (sc, res) in enumerate(batch):
         gsc = ctx.scene(sc, width=args.width, max_leaf_size=args.leaf, stack_lds=args.stack,
                         block_size=args.block)
         o = abi.render_opts(res, res, spp=16, seed=500 + k, max_depth=6, rr_depth=3, scene_id=k)
         items.append((gsc, o))
-        f = gpu.DeviceArray(ctx, res * res * 3, np.float32)
-        f.copy_from(np.zeros(res * res * 3, np.float32))
-        w = gpu.DeviceArray(ctx, res * res, np.float32)
-        w.copy_from(np.zeros(res * res, np.float32))
-        films.append((f, w))
+        films_p.append((zeroed(res * res * 3), zeroed(res * res)))
+        films_b.append((zeroed(res * res * 3), zeroed(res * res)))
     tris = sum(sc.num_triangles for sc, _ in batch)
     paths = sum(o.x_res * o.y_res * 16 for _, o in items)
     log(f"batch workload: 64 scenes, {tris} triangles, {paths / 1e6:.1f} Mpaths ({time.time() - t0:.1f}s to build)")
 
     def per_scene():
-        for (gsc, o), (f, w) in zip(items, films):
+        for (gsc, o), (f, w) in zip(items, films_p):
             gsc.render(o, f, w, device=True)
 
     def batched():
-        ctx.render_batch(items, films=films, device=True)
+        ctx.render_batch(items, films=films_b, device=True)
 
     per_scene()
     batched()
+    mismatched = []
+    for k, ((fp, wp), (fb, wb)) in enumerate(zip(films_p, films_b)):
+        if not (np.array_equal(wp.numpy(), wb.numpy()) and
+                np.allclose(fb.numpy(), fp.numpy(), rtol=1e-5, atol=1e-6)):
+            mismatched.append(k)
+    log(f"batch films equal per-scene films: {not mismatched} (mismatched scenes: {mismatched})")
     tp, tb = [], []
     for _ in range(2):
         t0 = time.perf_counter(); per_scene(); tp.append(time.perf_counter() - t0)
         t0 = time.perf_counter(); batched(); tb.append(time.perf_counter() - t0)
     t_p, t_b = min(tp), min(tb)
-    for f, w in films:
+    for f, w in films_p + films_b:
         f.free()
         w.free()
-    return {"workload": f"64 scenes, {tris} tris (log-uniform 1k-500k), 256^2/512^2/768^2, 16 spp, depth 6",
+    return {"workload": f"64 scenes, {tris} tris (log-uniform 1k-500k), 256^2/512^2/768^2, 16 spp, depth 6, "
+                        "1024x512 sky each",
             "paths": paths, "per_scene_Mpaths_s": paths / t_p / 1e6, "batched_Mpaths_s": paths / t_b / 1e6,
-            "speedup": t_p / t_b, "per_scene_ms": t_p * 1e3, "batched_ms": t_b * 1e3}
+            "speedup": t_p / t_b, "per_scene_ms": t_p * 1e3, "batched_ms": t_b * 1e3,
+            "films_equal_per_scene": not mismatched}
 
 
 def cpu_baseline(sc, args):

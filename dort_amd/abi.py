@@ -37,8 +37,13 @@ class EnvLight(C.Structure):
                 ("up", f3), ("fwd", f3), ("scale", f3)]
 
 
+TEXMAP_UV, TEXMAP_XY = 0, 1
+
+
 class Texture(C.Structure):
-    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("rgb", C.POINTER(C.c_float))]
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("rgb", C.POINTER(C.c_float)),
+                ("rgb8", C.POINTER(C.c_uint8)), ("mapping", C.c_uint32),
+                ("world_to_texture", C.c_float * 12)]
 
 
 class Camera(C.Structure):
@@ -169,9 +174,25 @@ class PackedScene:
             d.infinite_radiance = fptr(inf)
         texs = (Texture * max(1, len(s.textures)))()
         for i, t in enumerate(s.textures):
-            t = np.ascontiguousarray(t, dtype=np.float32)
-            self._keep.append(t)
-            texs[i] = Texture(t.shape[1], t.shape[0], fptr(t))
+            # an (H, W, 3) image (float32 texels, or uint8 read as k / 256) addressed by the
+            # surface uv, or {"image": ..., "mapping": "xy", "world_to_texture": (3, 4)}
+            spec = t if isinstance(t, dict) else {"image": t}
+            img = np.asarray(spec["image"])
+            tx = Texture()
+            tx.width, tx.height = img.shape[1], img.shape[0]
+            if img.dtype == np.uint8:
+                img = np.ascontiguousarray(img)
+                tx.rgb8 = img.ctypes.data_as(C.POINTER(C.c_uint8))
+            else:
+                img = np.ascontiguousarray(img, dtype=np.float32)
+                tx.rgb = fptr(img)
+            self._keep.append(img)
+            if spec.get("mapping", "uv") == "xy":
+                tx.mapping = TEXMAP_XY
+                w2t = np.asarray(spec["world_to_texture"], dtype=np.float32).reshape(12)
+                for k in range(12):
+                    tx.world_to_texture[k] = float(w2t[k])
+            texs[i] = tx
         self._keep.append(texs)
         d.num_textures = len(s.textures)
         d.textures = C.cast(texs, C.POINTER(Texture))

@@ -161,7 +161,7 @@ struct dgpu_scene {
   uint32_t stack_cap = MAX_STACK;
   uint32_t num_area = 0;
   dgpu_scene_info info;
-  std::vector<uint4> tex_info_host;   // (w, h, texel offset, 0)
+  std::vector<uint4> tex_info_host;   // (w, h, texel offset, mapping)
   std::vector<DevMaterial> mats_host;
 
   ~dgpu_scene() {
@@ -208,9 +208,15 @@ dgpu_status validate_desc(const dgpu_scene_desc* d) {
     if (d->area_lights[l].triangle >= d->num_triangles) return fail(DGPU_EINVAL, "area light %u: bad triangle", l);
   if (d->env && (!d->env->rgb || d->env->width == 0 || d->env->height == 0))
     return fail(DGPU_EINVAL, "environment map without pixels");
-  for (uint32_t t = 0; t < d->num_textures; ++t)
-    if (!d->textures[t].rgb || d->textures[t].width == 0 || d->textures[t].height == 0)
+  for (uint32_t t = 0; t < d->num_textures; ++t) {
+    const dgpu_texture& tx = d->textures[t];
+    if ((!tx.rgb && !tx.rgb8) || tx.width == 0 || tx.height == 0)
       return fail(DGPU_EINVAL, "texture %u without texels", t);
+    if (tx.mapping > DGPU_TEXMAP_XY) return fail(DGPU_EUNSUPPORTED, "texture %u: unknown mapping %u", t, tx.mapping);
+    if (tx.mapping == DGPU_TEXMAP_XY)
+      for (int k = 0; k < 8; ++k)
+        if (!std::isfinite(tx.world_to_texture[k])) return fail(DGPU_EINVAL, "texture %u: non-finite transform", t);
+  }
   if (!(d->camera.fov > 0.f && d->camera.fov < 3.14159f)) return fail(DGPU_EINVAL, "bad camera fov");
   return DGPU_OK;
 }
@@ -523,16 +529,29 @@ dgpu_status dgpu_scene_create(dgpu_ctx* ctx, const dgpu_scene_desc* d, const dgp
   UP((const float4*)bvh.tris.data(), bvh.tris.size() / 4, S.tris);
   // textures
   if (d->num_textures) {
-    std::vector<float4> texels;
+    std::vector<float4> texels, xform;
     std::vector<uint4> info;
     for (uint32_t t = 0; t < d->num_textures; ++t) {
       const dgpu_texture& tx = d->textures[t];
-      info.push_back(make_uint4(tx.width, tx.height, (uint32_t)texels.size(), 0));
-      for (size_t i = 0; i < (size_t)tx.width * tx.height; ++i)
-        texels.push_back(make_float4(tx.rgb[3 * i], tx.rgb[3 * i + 1], tx.rgb[3 * i + 2], 0.f));
+      info.push_back(make_uint4(tx.width, tx.height, (uint32_t)texels.size(), tx.mapping));
+      for (size_t i = 0; i < (size_t)tx.width * tx.height; ++i) {
+        if (tx.rgb) {
+          texels.push_back(make_float4(tx.rgb[3 * i], tx.rgb[3 * i + 1], tx.rgb[3 * i + 2], 0.f));
+        } else {  // PixelRgb8::to_rgb: k * 2^-8 (image.hpp:66-71), exact in fp32
+          const uint8_t* c = tx.rgb8 + 3 * i;
+          texels.push_back(make_float4(std::ldexp((float)c[0], -8), std::ldexp((float)c[1], -8),
+                                       std::ldexp((float)c[2], -8), 0.f));
+        }
+      }
+      // rows 0 and 1 of world_to_texture: the texture's (x, y) of a world point
+      const float* m = tx.world_to_texture;
+      bool xy = tx.mapping == DGPU_TEXMAP_XY;
+      xform.push_back(xy ? make_float4(m[0], m[1], m[2], m[3]) : make_float4(0.f, 0.f, 0.f, 0.f));
+      xform.push_back(xy ? make_float4(m[4], m[5], m[6], m[7]) : make_float4(0.f, 0.f, 0.f, 0.f));
     }
     UP(texels.data(), texels.size(), S.tex_data);
     UP(info.data(), info.size(), S.tex_info);
+    UP(xform.data(), xform.size(), S.tex_xform);
     S.num_tex = d->num_textures;
     s->tex_info_host = info;
   }
@@ -1090,6 +1109,25 @@ dgpu_status dgpu_render_batch(dgpu_ctx* ctx, const dgpu_batch_item* items, uint3
     paths[i] = (uint64_t)E.npix * (o.sample_end - o.sample_begin);
     film_floats += 4 * (size_t)E.npix;
     stack_cap = std::max(stack_cap, items[i].scene->stack_cap);
+  }
+  if (dev) {
+    // The finalize kernel reads, adds and writes each film pixel without atomics, and
+    // two items of one chunk are finalized by different threads: device films (and an
+    // item's colour and weight arrays) must not share memory.  Host films do not
+    // matter: each item gets its own staging region, added to the host film at the end.
+    struct Span { uintptr_t lo, hi; uint32_t item; };
+    std::vector<Span> spans;
+    spans.reserve(2 * (size_t)n);
+    for (uint32_t i = 0; i < n; ++i) {
+      uintptr_t r = (uintptr_t)items[i].film_rgb_sum, w = (uintptr_t)items[i].film_weight;
+      spans.push_back({r, r + 12 * (uintptr_t)tab[i].npix, i});
+      spans.push_back({w, w + 4 * (uintptr_t)tab[i].npix, i});
+    }
+    std::sort(spans.begin(), spans.end(), [](const Span& a, const Span& b) { return a.lo < b.lo; });
+    for (size_t k = 1; k < spans.size(); ++k)
+      if (spans[k].lo < spans[k - 1].hi)
+        return fail(DGPU_EINVAL, "items %u and %u: device films overlap (each film must be its own memory)",
+                    std::min(spans[k - 1].item, spans[k].item), std::max(spans[k - 1].item, spans[k].item));
   }
   float* dfilm = nullptr;
   if (!dev) {

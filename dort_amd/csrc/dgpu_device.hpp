@@ -144,7 +144,8 @@ struct DevScene {
   V3 inf_rad;
   // textures
   const float4* tex_data;
-  const uint4* tex_info;     // (w, h, offset, 0)
+  const uint4* tex_info;     // (w, h, offset, mapping DGPU_TEXMAP_*)
+  const float4* tex_xform;   // 2 per texture: rows 0, 1 of world_to_texture (DGPU_TEXMAP_XY)
   uint32_t num_tex;
   // camera (pinhole_camera.cpp:6-11)
   float c2w[12];
@@ -444,6 +445,21 @@ DG_INLINE int32_t texel_of(uint4 info, float u, float v) {
   return py * (int32_t)info.x + px;
 }
 
+// The texture coordinates of surface point g: its uv (uv_texture_map, texture_maps.cpp:4-8)
+// or x, y of world_to_texture * p (xy_texture_map, texture_maps.cpp:10-15), summed in
+// the order of dort's mul_mat_1 (mat.cpp:58-63) without fused multiply-adds, as the
+// CPU computes it.
+DG_INLINE void tex_coords(const DevScene& S, uint32_t tex, uint4 info, const Surface& g, float& u, float& v) {
+  u = g.u;
+  v = g.v;
+  if (info.w == DGPU_TEXMAP_XY) {
+#pragma clang fp contract(off)
+    float4 r0 = S.tex_xform[2 * tex], r1 = S.tex_xform[2 * tex + 1];
+    u = r0.x * g.p.x + r0.y * g.p.y + r0.z * g.p.z + r0.w;
+    v = r1.x * g.p.x + r1.y * g.p.y + r1.z * g.p.z + r1.w;
+  }
+}
+
 // Material::get_bsdf -> Bsdf frame (bsdf.cpp:50-66) + the material's bxdf.
 DG_INLINE Bsdf make_bsdf(const DevScene& S, const Surface& g, int32_t* texel_out = nullptr) {
   Bsdf b;
@@ -461,7 +477,9 @@ DG_INLINE Bsdf make_bsdf(const DevScene& S, const Surface& g, int32_t* texel_out
     V3 a = xyz(m.albedo);
     if (m.tex >= 0) {
       uint4 info = S.tex_info[m.tex];
-      int32_t ti = texel_of(info, g.u, g.v);
+      float tu, tv;
+      tex_coords(S, (uint32_t)m.tex, info, g, tu, tv);
+      int32_t ti = texel_of(info, tu, tv);
       a = xyz(S.tex_data[info.z + (uint32_t)ti]);
       if (texel_out && m.tex == 0) *texel_out = ti;
     }

@@ -21,10 +21,34 @@ uint32_t GpuSceneDesc::add_material(const dgpu_material& m) {
   return (uint32_t)materials_.size() - 1;
 }
 
-uint32_t GpuSceneDesc::add_texture(uint32_t width, uint32_t height, std::vector<float> rgb) {
+namespace {
+dgpu_texture texture_record(uint32_t width, uint32_t height, uint32_t mapping, const float* world_to_texture) {
+  if (mapping > DGPU_TEXMAP_XY) throw std::invalid_argument("unknown texture mapping");
+  if (mapping == DGPU_TEXMAP_XY && !world_to_texture) throw std::invalid_argument("xy mapping without a transform");
+  dgpu_texture t{};
+  t.width = width;
+  t.height = height;
+  t.mapping = mapping;
+  if (world_to_texture) std::copy(world_to_texture, world_to_texture + 12, t.world_to_texture);
+  return t;
+}
+}  // namespace
+
+uint32_t GpuSceneDesc::add_texture(uint32_t width, uint32_t height, std::vector<float> rgb, uint32_t mapping,
+                                   const float* world_to_texture) {
   if (rgb.size() != size_t(width) * height * 3) throw std::invalid_argument("texture size mismatch");
+  textures_.push_back(texture_record(width, height, mapping, world_to_texture));
   texels_.push_back(std::move(rgb));
-  textures_.push_back(dgpu_texture{width, height, nullptr});
+  texels8_.emplace_back();
+  return (uint32_t)textures_.size() - 1;
+}
+
+uint32_t GpuSceneDesc::add_texture_rgb8(uint32_t width, uint32_t height, std::vector<uint8_t> rgb,
+                                        uint32_t mapping, const float* world_to_texture) {
+  if (rgb.size() != size_t(width) * height * 3) throw std::invalid_argument("texture size mismatch");
+  textures_.push_back(texture_record(width, height, mapping, world_to_texture));
+  texels_.emplace_back();
+  texels8_.push_back(std::move(rgb));
   return (uint32_t)textures_.size() - 1;
 }
 
@@ -79,7 +103,10 @@ void GpuSceneDesc::set_camera(const float camera_to_world[12], float fov) {
 }
 
 const dgpu_scene_desc& GpuSceneDesc::desc() {
-  for (size_t i = 0; i < textures_.size(); ++i) textures_[i].rgb = texels_[i].data();
+  for (size_t i = 0; i < textures_.size(); ++i) {
+    textures_[i].rgb = texels8_[i].empty() ? texels_[i].data() : nullptr;
+    textures_[i].rgb8 = texels8_[i].empty() ? nullptr : texels8_[i].data();
+  }
   env_.rgb = env_rgb_.data();
   desc_.num_vertices = (uint32_t)(positions_.size() / 3);
   desc_.num_triangles = (uint32_t)tri_material_.size();

@@ -33,8 +33,14 @@ namespace dort_amd {
 class GpuSceneDesc {
  public:
   uint32_t add_material(const dgpu_material& m);
-  // Texture texels: width * height RGB fp32, row-major; returns its index.
-  uint32_t add_texture(uint32_t width, uint32_t height, std::vector<float> rgb);
+  // Texture texels: width * height RGB fp32, row-major; returns its index.  mapping:
+  // DGPU_TEXMAP_UV (uv_texture_map) or DGPU_TEXMAP_XY with world_to_texture (row-major
+  // 3x4, the inverse of xy_texture_map's tex_to_world).
+  uint32_t add_texture(uint32_t width, uint32_t height, std::vector<float> rgb,
+                       uint32_t mapping = DGPU_TEXMAP_UV, const float* world_to_texture = nullptr);
+  // The same with dort's 8-bit texels (Image<PixelRgb8>, k reads as k / 256).
+  uint32_t add_texture_rgb8(uint32_t width, uint32_t height, std::vector<uint8_t> rgb,
+                            uint32_t mapping = DGPU_TEXMAP_UV, const float* world_to_texture = nullptr);
   // A mesh: points (3 floats each), triangles (3 vertex indices each, into points),
   // optional uvs (2 floats per point), one material.  Returns the index of its
   // first triangle in the scene.
@@ -60,6 +66,7 @@ class GpuSceneDesc {
   std::vector<dgpu_material> materials_;
   std::vector<dgpu_area_light> lights_;
   std::vector<std::vector<float>> texels_;
+  std::vector<std::vector<uint8_t>> texels8_;
   std::vector<dgpu_texture> textures_;
   std::vector<float> env_rgb_;
   dgpu_env_light env_{};

@@ -9,7 +9,7 @@ Transform conventions follow dort's `Transform` (src/dort/transform.cpp):
 `look_at` (:146-165) builds camera_to_world with columns (right, up, dir, eye);
 `rotate_y` (:104-114) maps x' = cos*x + sin*z, z' = -sin*x + cos*z.
 """
-from dataclasses import dataclass, field
+from dataclasses import dataclass, field, replace
 
 import numpy as np
 
@@ -242,6 +242,46 @@ def cornell(seed=0):
                        (0, -1, 0), (17.0, 12.0, 4.0), blk)
     cam = look_at((0.5, 0.5, -1.4), (0.5, 0.5, 0.0), (0, 1, 0)) @ scale_m(1, -1, 1)
     return b.build(cam, 0.69, name="cornell")
+
+
+def glass_cornell():
+    """The Cornell box with its tall block made a smooth dielectric of IOR 1.5
+    (dielectric_material.cpp:32-104): camera paths through it are delta bounces."""
+    sc = cornell()
+    mats = list(sc.materials) + [dielectric(1.5)]
+    tri_mat = sc.tri_material.copy()
+    tri_mat[22:34] = len(mats) - 1  # 10 wall + 12 short-box triangles come first
+    return replace(sc, materials=mats, tri_material=tri_mat)
+
+
+def textured_cornell(seed=0, texels="u8"):
+    """The Cornell box with image textures on two walls, as dort's ImageTexture composes
+    with its texture maps (image_texture.cpp:5-11, texture_maps.cpp:4-15): the floor
+    carries a 16 x 16 texture through uv_texture_map (vertex uvs span [0, 2]^2, so it
+    wraps twice), the back wall an 8 x 12 texture through xy_texture_map (rotated by 30
+    degrees about z, scaled by 2.5, shifted).  Texels are seeded 8-bit values, given as
+    uint8 (texels="u8") or as the float32 values k / 256 they read as ("f32").  The other
+    surfaces get all-zero vertex uvs, which takes the degenerate-uv branch of
+    TriangleUv::hit's tangent (triangle.cpp:137-171)."""
+    sc = cornell()
+    rng = np.random.default_rng(seed)
+    t_floor = rng.integers(0, 256, size=(16, 16, 3), dtype=np.uint8)
+    t_back = rng.integers(0, 256, size=(8, 12, 3), dtype=np.uint8)
+    if texels == "f32":
+        t_floor, t_back = t_floor.astype(np.float32) / 256, t_back.astype(np.float32) / 256
+    a = np.radians(30.0)
+    rot = np.array([[np.cos(a), -np.sin(a), 0.0], [np.sin(a), np.cos(a), 0.0], [0.0, 0.0, 1.0]])
+    w2t = np.zeros((3, 4))
+    w2t[:, :3] = 2.5 * rot
+    w2t[:, 3] = (0.3, -0.7, 0.0)
+    mats = list(sc.materials) + [lambert((0, 0, 0), tex=0), lambert((0, 0, 0), tex=1)]
+    tri_mat = sc.tri_material.copy()
+    tri_mat[0:2] = len(mats) - 2   # floor
+    tri_mat[4:6] = len(mats) - 1   # back wall
+    uvs = np.zeros((len(sc.positions), 2), np.float32)
+    uvs[0:4] = [(0, 0), (2, 0), (2, 2), (0, 2)]  # the floor quad's vertices come first
+    return replace(sc, materials=mats, tri_material=tri_mat, uvs=uvs, name="textured_cornell",
+                   textures=[t_floor, {"image": t_back, "mapping": "xy", "world_to_texture": w2t}])
 
 
 # ------------------------------------------------- dort's own test box scene
@@ -591,10 +631,11 @@ def terrain_scene(quads=1000, seed=2, patches=4, sky=None, textured=False, tex_r
 
 # ---------------------------------------------------------------- config 3
 
-def batch_scenes(count=64, seed=3, min_tris=1000, max_tris=500000, sky_res=(256, 128)):
+def batch_scenes(count=64, seed=3, min_tris=1000, max_tris=500000, sky_res=(1024, 512)):
     """`count` independent scenes, triangle counts log-uniform in [min_tris, max_tris]
     (displaced geodesic spheres and boxes on a ground plane), resolutions uniform in
-    {256^2, 512^2, 768^2}, materials as in the terrain scene, lit by a sky."""
+    {256^2, 512^2, 768^2}, materials as in the terrain scene, lit by a procedural sky of
+    the terrain's resolution (1024 x 512, sun lobe of peak 50) with a seeded sun."""
     rng = np.random.default_rng(seed)
     out = []
     for k in range(count):

@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define DGPU_ABI_VERSION 3u
+#define DGPU_ABI_VERSION 4u
 
 typedef enum {
   DGPU_OK = 0,
@@ -83,9 +83,22 @@ typedef struct {
   float scale[3];
 } dgpu_env_light;
 
+/* How a texture is addressed at a surface point (texture_maps.cpp:4-15). */
+enum {
+  DGPU_TEXMAP_UV = 0,  /* uv_texture_map: the surface's uv */
+  DGPU_TEXMAP_XY = 1   /* xy_texture_map(tex_to_world): x, y of world_to_texture applied to the point */
+};
+
+/* An image texture composed with its mapping: ImageTexture (image_texture.cpp:5-11), nearest
+ * texel, coordinates wrapped to [0, 1), row y = floor(height * v) from the image's first row. */
 typedef struct {
   uint32_t width, height;
-  const float* rgb;        /* width*height*3 fp32 texels, nearest + wrap (image_texture.cpp:5-11) */
+  const float* rgb;        /* width*height*3 fp32 texels, or NULL when rgb8 is given */
+  const uint8_t* rgb8;     /* width*height*3 8-bit texels (dort's Image<PixelRgb8>): k reads as
+                              k / 256 (PixelRgb8::to_rgb, image.hpp:66-71); used when rgb is NULL */
+  uint32_t mapping;        /* DGPU_TEXMAP_* */
+  float world_to_texture[12]; /* DGPU_TEXMAP_XY: row-major 3x4, the inverse of the map's
+                                 tex_to_world (Transform::apply_inv of texture_maps.cpp:12) */
 } dgpu_texture;
 
 typedef struct {
@@ -264,7 +277,10 @@ dgpu_status dgpu_render_light(dgpu_scene* scene, const dgpu_render_opts* opts, f
  * equal per-scene renders.  n is in [1, 255].  Chunks hold at most item 0's scene
  * queue_tile paths, or by default equal shares of the batch of at most 32 GiB of path
  * state each (about 148M paths); a chunk spans at most 64 scenes.
- * flags: DGPU_BUF_DEVICE (films are device pointers), DGPU_BATCH_NO_GRAPH. */
+ * flags: DGPU_BUF_DEVICE (films are device pointers), DGPU_BATCH_NO_GRAPH.  Device films
+ * are added to without atomics, so no two of them (nor an item's colour and weight
+ * arrays) may overlap: such a batch fails with DGPU_EINVAL.  Host films may be shared
+ * (each item is staged separately and added at the end). */
 typedef struct {
   dgpu_scene* scene;
   dgpu_render_opts opts;

@@ -12,6 +12,7 @@
 //   orc_render                 PathRenderer::render, tiles x iterations on a thread pool
 //   orc_adjoint64              FP64 path-replay adjoint (albedo texels, GGX roughness)
 //   orc_bsdf_*, orc_env_*      per-function probes for closed-form tests
+//   orc_fresnel_*, orc_philox  the Fresnel terms and the counter-based generator alone
 #include <algorithm>
 #include <atomic>
 #include <cmath>
@@ -236,6 +237,44 @@ void orc_env_eval(void* p, uint32_t n, const float* dir, float* pdf, float* rad)
     Spec r = sc.env.background(d);
     rad[3 * i] = r.r; rad[3 * i + 1] = r.g; rad[3 * i + 2] = r.b;
   }
+}
+
+// Fresnel probes for closed-form tests.  Conductor: unpolarised reflectance for complex
+// IOR eta + i k at cos_i, in the renderer's float arithmetic or in double.  Dielectric:
+// dort's reflectance of a ray going from ior_refl into ior_trans (fresnel_dielectric with
+// the refracted cosine of refract_dielectric, fresnel.cpp:5-35; 1 under total internal
+// reflection), as the dielectric lobe computes it.
+void orc_fresnel_conductor(uint32_t n, const double* cos_i, double eta, double k, int fp64, double* out) {
+  for (uint32_t i = 0; i < n; ++i)
+    out[i] = fp64 ? orc_d::fresnel_conductor1(cos_i[i], eta, k)
+                  : (double)orc_f::fresnel_conductor1((float)cos_i[i], (float)eta, (float)k);
+}
+
+void orc_fresnel_dielectric(uint32_t n, const double* cos_i, double ior_refl, double ior_trans, int fp64,
+                            double* out) {
+  for (uint32_t i = 0; i < n; ++i) {
+    if (fp64) {
+      using namespace orc_d;
+      real c = cos_i[i];
+      V3 w(std::sqrt(std::max(0.0, 1.0 - c * c)), 0.0, c), t;
+      out[i] = refract_dielectric(w, t, ior_refl, ior_trans) ? fresnel_dielectric(c, std::abs(t.z), ior_refl, ior_trans)
+                                                            : 1.0;
+    } else {
+      using namespace orc_f;
+      real c = (real)cos_i[i];
+      V3 w(std::sqrt(std::max(0.f, 1.f - c * c)), 0.f, c), t;
+      out[i] = refract_dielectric(w, t, (real)ior_refl, (real)ior_trans)
+                   ? fresnel_dielectric(c, std::abs(t.z), (real)ior_refl, (real)ior_trans)
+                   : 1.0;
+    }
+  }
+}
+
+// Philox4x32-10 of include/dgpu_rng.h (the generator the GPU path and the oracle's counter
+// mode draw from) on one counter and key, for known-answer tests.
+void orc_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out) {
+  dgpu_u32x4 r = dgpu_philox4x32_10(ctr[0], ctr[1], ctr[2], ctr[3], key[0], key[1]);
+  for (int i = 0; i < 4; ++i) out[i] = r.v[i];
 }
 
 // Light table: kinds (0 area, 1 env, 2 infinite) and the pick CDF (n+1 entries).

@@ -50,6 +50,9 @@ def lib():
         L.orc_env_eval.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_lights.restype = C.c_uint32
         L.orc_lights.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_fresnel_conductor.argtypes = [C.c_uint32, C.c_void_p, C.c_double, C.c_double, C.c_int, C.c_void_p]
+        L.orc_fresnel_dielectric.argtypes = [C.c_uint32, C.c_void_p, C.c_double, C.c_double, C.c_int, C.c_void_p]
+        L.orc_philox.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         _lib = L
     return _lib
 
@@ -176,3 +179,29 @@ def bsdf_eval(material, wi, wo, request=BSDF_NON_DELTA):
     lib().orc_bsdf_eval(C.byref(material), n, wi.ctypes.data, wo.ctypes.data, request,
                         f.ctypes.data, pdf.ctypes.data)
     return f, pdf
+
+
+def fresnel_conductor(cos_i, eta, k, fp64=False):
+    """The oracle's conductor Fresnel reflectance at each cos_i (complex IOR eta + i k)."""
+    c = np.ascontiguousarray(cos_i, dtype=np.float64)
+    out = np.zeros_like(c)
+    lib().orc_fresnel_conductor(len(c), c.ctypes.data, float(eta), float(k), int(fp64), out.ctypes.data)
+    return out
+
+
+def fresnel_dielectric(cos_i, ior_refl, ior_trans, fp64=False):
+    """The oracle's dielectric Fresnel reflectance (dort's fresnel.cpp) at each cos_i."""
+    c = np.ascontiguousarray(cos_i, dtype=np.float64)
+    out = np.zeros_like(c)
+    lib().orc_fresnel_dielectric(len(c), c.ctypes.data, float(ior_refl), float(ior_trans), int(fp64),
+                                 out.ctypes.data)
+    return out
+
+
+def philox4x32_10(ctr, key):
+    """include/dgpu_rng.h's Philox4x32-10 of one (4-word counter, 2-word key)."""
+    c = np.ascontiguousarray(ctr, dtype=np.uint32)
+    k = np.ascontiguousarray(key, dtype=np.uint32)
+    out = np.zeros(4, np.uint32)
+    lib().orc_philox(c.ctypes.data, k.ctypes.data, out.ctypes.data)
+    return out

@@ -10,8 +10,10 @@
 //                      Film with the default 1 px box filter (lua_render.cpp:108-115)
 //   ref_render_light   LightRenderer::render (light_renderer.cpp:13-45), the splat sums
 //
-// Scenes with a GGX conductor or an albedo texture are refused (dort has no rough
-// conductor; its ImageTexture is 8-bit).
+// Albedo textures become dort's ImageTexture (8-bit texels) composed with uv_texture_map or
+// xy_texture_map, as lua_texture.cpp:346-388 composes them; float texels must be exact
+// 8-bit values k / 256.  Scenes with a GGX conductor, or textures that are not 8-bit, are
+// refused (dort has no rough conductor; its ImageTexture is Image<PixelRgb8>).
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -27,6 +29,7 @@
 #include "dort/film.hpp"
 #include "dort/filter.hpp"
 #include "dort/image.hpp"
+#include "dort/image_texture.hpp"
 #include "dort/infinite_light.hpp"
 #include "dort/lambert_material.hpp"
 #include "dort/list_primitive.hpp"
@@ -41,6 +44,7 @@
 #include "dort/rng.hpp"
 #include "dort/scene.hpp"
 #include "dort/texture.hpp"
+#include "dort/texture_maps.hpp"
 #include "dort/thread_pool.hpp"
 #include "dort/transform.hpp"
 #include "dort/triangle_shape.hpp"
@@ -106,12 +110,59 @@ Transform camera_transform(const dgpu_camera& cam) {
   return Transform(Mat4x4(f), Mat4x4(fi));
 }
 
-std::shared_ptr<Material> make_material(const dgpu_material& m) {
+// dort's 8-bit image of texture t, or null when a float texel is not some k / 256.
+std::shared_ptr<Image<PixelRgb8>> texture_image(const dgpu_texture& t) {
+  auto img = std::make_shared<Image<PixelRgb8>>(t.width, t.height);
+  for (uint32_t y = 0; y < t.height; ++y)
+    for (uint32_t x = 0; x < t.width; ++x) {
+      size_t i = 3 * (size_t(y) * t.width + x);
+      uint8_t c[3];
+      for (int k = 0; k < 3; ++k) {
+        if (t.rgb8) { c[k] = t.rgb8[i + k]; continue; }
+        float v = t.rgb[i + k] * 256.f;
+        if (!(v >= 0.f && v <= 255.f && v == std::floor(v))) return nullptr;
+        c[k] = (uint8_t)v;
+      }
+      img->set_pixel(x, y, PixelRgb8(c[0], c[1], c[2]));
+    }
+  return img;
+}
+
+// ImageTexture composed with the texture's mapping (lua_texture.cpp:346-388).
+std::shared_ptr<TextureGeom<Spectrum>> albedo_texture(const dgpu_texture& t) {
+  auto img = texture_image(t);
+  if (!img) return nullptr;
+  std::shared_ptr<Texture<Spectrum, Vec2>> tex = std::make_shared<ImageTexture>(img);
+  std::shared_ptr<TextureGeom<Vec2>> map;
+  if (t.mapping == DGPU_TEXMAP_XY) {
+    // xy_texture_map applies tex_to_world's inverse matrix, which is world_to_texture as given
+    double m[4][4], mi[4][4];
+    for (int c = 0; c < 4; ++c) {
+      for (int r = 0; r < 3; ++r) m[c][r] = t.world_to_texture[r * 4 + c];
+      m[c][3] = c == 3 ? 1.0 : 0.0;
+    }
+    invert4(m, mi);
+    float f[4][4], fi[4][4];
+    for (int c = 0; c < 4; ++c)
+      for (int r = 0; r < 4; ++r) { f[c][r] = (float)mi[c][r]; fi[c][r] = (float)m[c][r]; }
+    map = xy_texture_map(Transform(Mat4x4(f), Mat4x4(fi)));
+  } else {
+    map = uv_texture_map();
+  }
+  return compose_texture(tex, map);
+}
+
+std::shared_ptr<Material> make_material(const dgpu_material& m, const dgpu_scene_desc& d) {
   switch (m.type) {
     case DGPU_MAT_NONE:
       return std::make_shared<LambertMaterial>(const_texture(Spectrum(0.f)));
-    case DGPU_MAT_LAMBERT:
-      return std::make_shared<LambertMaterial>(const_texture(spec3(m.albedo)));
+    case DGPU_MAT_LAMBERT: {
+      if (m.albedo_tex < 0) return std::make_shared<LambertMaterial>(const_texture(spec3(m.albedo)));
+      if (m.albedo_tex >= (int32_t)d.num_textures) return nullptr;
+      auto tex = albedo_texture(d.textures[m.albedo_tex]);
+      if (!tex) return nullptr;
+      return std::make_shared<LambertMaterial>(tex);
+    }
     case DGPU_MAT_DIELECTRIC:
       return std::make_shared<DielectricMaterial>(const_texture(spec3(m.reflect_tint)),
                                                   const_texture(spec3(m.transmit_tint)),
@@ -134,8 +185,10 @@ extern "C" {
 void* ref_scene_create(const dgpu_scene_desc* d, uint32_t split_method, uint32_t leaf_size,
                        uint32_t max_leaf_size, uint32_t threads) {
   if (!d || d->num_triangles == 0) return nullptr;
+  std::vector<std::shared_ptr<Material>> mats;
   for (uint32_t i = 0; i < d->num_materials; ++i) {
-    if (d->materials[i].type == DGPU_MAT_GGX_CONDUCTOR || d->materials[i].albedo_tex >= 0) return nullptr;
+    mats.push_back(make_material(d->materials[i], *d));
+    if (!mats.back()) return nullptr;
   }
   auto rs = new RefScene();
   rs->mesh = std::make_shared<Mesh>();
@@ -147,8 +200,6 @@ void* ref_scene_create(const dgpu_scene_desc* d, uint32_t split_method, uint32_t
   }
   mesh.vertices.assign(d->indices, d->indices + 3 * size_t(d->num_triangles));
 
-  std::vector<std::shared_ptr<Material>> mats;
-  for (uint32_t i = 0; i < d->num_materials; ++i) mats.push_back(make_material(d->materials[i]));
   std::vector<const dgpu_area_light*> tri_light(d->num_triangles, nullptr);
   for (uint32_t i = 0; i < d->num_area_lights; ++i) tri_light[d->area_lights[i].triangle] = &d->area_lights[i];
 

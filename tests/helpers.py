@@ -12,7 +12,9 @@ def classify_hits(cpu, gpu, rel_t=1e-5, edge_eps=1e-4):
     both_miss = (cp < 0) & (gp < 0)
     same = (cp == gp) & (cp >= 0)
     hit_both = (cp >= 0) & (gp >= 0)
-    tie = hit_both & ~same & (np.abs(ct - gt) <= rel_t * np.abs(ct))
+    # distances are compared only where both rays hit (a miss is t = inf)
+    dt = np.abs(np.where(hit_both, ct, 0.0) - np.where(hit_both, gt, 0.0))
+    tie = hit_both & ~same & (dt <= rel_t * np.abs(np.where(hit_both, ct, 0.0)))
     b0 = 1.0 - cpu["b1"] - cpu["b2"]
     edge = (np.minimum(np.minimum(b0, cpu["b1"]), cpu["b2"]) <= edge_eps) & (cp >= 0)
     gb0 = 1.0 - gpu["b1"] - gpu["b2"]

@@ -30,7 +30,7 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_abi_version():
-    assert gpu.lib().dgpu_abi_version() == 3
+    assert gpu.lib().dgpu_abi_version() == 4
 
 
 def test_struct_layouts_match_header(tmp_path):

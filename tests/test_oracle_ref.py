@@ -28,15 +28,6 @@ def dortable(sc):
     return dataclasses.replace(sc, materials=mats)
 
 
-def glass_cornell():
-    """Cornell box whose tall block is a smooth dielectric (IOR 1.5)."""
-    sc = scenes.cornell()
-    mats = list(sc.materials) + [scenes.dielectric(1.5)]
-    tri_mat = sc.tri_material.copy()
-    tri_mat[22:34] = len(mats) - 1  # 10 wall + 12 short-box triangles come first
-    return dataclasses.replace(sc, materials=mats, tri_material=tri_mat)
-
-
 def terrain():
     return dortable(scenes.terrain_scene(quads=12, sky_res=(64, 32)))
 
@@ -45,8 +36,10 @@ SCENES = {
     "cornell": scenes.cornell,
     "box_area": lambda: scenes.dort_test_box("area"),
     "box_infi": lambda: scenes.dort_test_box("infi"),
-    "glass_cornell": glass_cornell,
+    "glass_cornell": scenes.glass_cornell,
     "terrain_sky": terrain,
+    "textured_cornell": scenes.textured_cornell,
+    "textured_cornell_f32": lambda: scenes.textured_cornell(texels="f32"),
 }
 
 
@@ -77,6 +70,8 @@ RENDER_CASES = [
     ("box_area", {"min_depth": 2}),
     ("box_area", {"only_direct": 1}),
     ("glass_cornell", {"max_depth": 12}),
+    ("textured_cornell", {}),
+    ("textured_cornell_f32", {}),
 ]
 
 
@@ -135,3 +130,23 @@ def test_counter_rng_estimator_has_dorts_relmse():
     ratio = np.mean(a) / np.mean(b)
     print(f"relMSE counter {np.mean(a):.5f}, dort {np.mean(b):.5f}, ratio {ratio:.4f}")
     assert 0.92 < ratio < 1.08
+
+
+def test_textures_are_dorts_image_textures():
+    """The textured case above is sharp for the texel lookup: shifting the floor texture by
+    one texel (a wrong row or column offset) or flipping the xy map's rows fails it, and
+    the 8-bit and float forms of the same texels render the same image."""
+    sc = scenes.textured_cornell()
+    opts = abi.render_opts(32, 24, spp=6, seed=11, max_depth=6)
+    ra, _ = ref.RefScene(sc).render(opts, threads=1)
+    f32, _, _ = oracle.OracleScene(scenes.textured_cornell(texels="f32")).render(opts, rng="mt", threads=1)
+    u8, _, _ = oracle.OracleScene(sc).render(opts, rng="mt", threads=1)
+    np.testing.assert_array_equal(f32, u8)
+    shifted = dataclasses.replace(sc, textures=[np.roll(sc.textures[0], 1, axis=1), sc.textures[1]])
+    rs, _, _ = oracle.OracleScene(shifted).render(opts, rng="mt", threads=1)
+    assert not np.allclose(rs, ra, rtol=2e-4, atol=1e-5)
+    back = dict(sc.textures[1])
+    back["image"] = back["image"][::-1].copy()
+    flipped = dataclasses.replace(sc, textures=[sc.textures[0], back])
+    rf, _, _ = oracle.OracleScene(flipped).render(opts, rng="mt", threads=1)
+    assert not np.allclose(rf, ra, rtol=2e-4, atol=1e-5)
