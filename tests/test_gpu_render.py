@@ -26,12 +26,17 @@ def _pixel_parity(img_g, img_c, rel=1e-3, abs_=1e-4):
 
 
 def _paired_z(img_g, img_c):
-    """Largest |mean / standard error| over the channels of the difference image.  With
-    the same counter-based random numbers the two renders trace the same paths, so the
-    difference is zero except at pixels where a path diverged on an FP tie; for an
-    unbiased GPU path those differences have zero mean, while an error in one branch
-    (a delta lobe, total internal reflection) shifts them all one way."""
+    """Largest |mean / standard error| over the channels of the difference image, taken
+    as zero where the pixels agree to the parity tolerance.  With the same counter-based
+    random numbers the two renders trace the same paths, so the difference is zero
+    except at pixels where a path diverged on an FP tie; for an unbiased GPU path those
+    differences have zero mean, while an error in one branch (a delta lobe, total
+    internal reflection) shifts them all one way."""
     d = (img_g.astype(np.float64) - img_c.astype(np.float64)).reshape(-1, 3)
+    # pixels within rounding of each other hold the same paths: their last-bit
+    # differences are not samples of the divergence and are left out
+    same = np.abs(d) <= 1e-3 * np.abs(img_c.astype(np.float64).reshape(-1, 3)) + 1e-4
+    d = np.where(same.all(axis=1, keepdims=True), 0.0, d)
     se = d.std(axis=0) / np.sqrt(len(d))
     z = np.where(se > 0, np.abs(d.mean(axis=0)) / np.where(se > 0, se, 1.0), 0.0)
     return float(z.max())
