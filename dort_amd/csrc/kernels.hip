@@ -70,19 +70,86 @@ __device__ __forceinline__ bool trace_one(BvhRef b, const TraceRay& r, LaneStack
 // seven, also where a few registers spill.
 #define DG_TRACE_WAVES __attribute__((amdgpu_waves_per_eu(8)))
 
+// The wave's batch, traced 64 rays at a time: each part of the batch runs until its
+// longest ray ends.
+template <bool ANY, int WIDTH, bool FAST, uint32_t NBATCH, class IO>
+__device__ __forceinline__ void trace_batch_parts(IO& io, uint32_t n, uint32_t base, LaneStack& st) {
+  for (uint32_t j = 0; j < NBATCH; ++j) {
+    uint32_t i = base + 64u * j + lane_id();
+    if (i < n) {
+      TraceRay r;
+      BvhRef bvh = io.load(i, r);
+      TraceHit h;
+      bool found = trace_one<ANY, WIDTH, FAST>(bvh, r, st, h);
+      io.store(i, h, found);
+    }
+  }
+}
+
+// The wave's batch with refills: a lane whose ray has ended takes the next entry of the
+// batch (a neighbouring ray of the queue) once at least REFILL lanes of the wave are
+// idle, so the wave no longer runs at the pace of its longest ray with the other lanes
+// masked off.  Each refill hands the idle lanes consecutive entries in lane order.
+template <bool ANY, int WIDTH, bool FAST, uint32_t NBATCH, uint32_t REFILL, class IO>
+__device__ __forceinline__ void trace_batch_refill(IO& io, uint32_t n, uint32_t base, LaneStack& st) {
+  const uint32_t lane = lane_id();
+  const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64u - lane));
+  const uint32_t end = min(base + 64u * NBATCH, n);
+  uint32_t next = base + 64u;  // first entry of the batch that no lane has taken
+  uint32_t i = base + lane;
+  bool busy = i < end;
+  TravState s;
+  BvhRef bvh{nullptr, nullptr};
+  if (busy) {
+    TraceRay r;
+    bvh = io.load(i, r);
+    trav_begin<FAST>(s, r, st);
+  }
+  while (__ballot(busy) != 0ull) {
+    if (busy) {
+      trav_step<ANY, WIDTH, FAST>(bvh.nodes, bvh.tris, s, st);
+      if (s.node == TRAV_END) {
+        io.store(i, s.h, s.h.tri != 0xFFFFFFFFu);
+        busy = false;
+      }
+    }
+    const unsigned long long idle = __ballot(!busy);
+    const uint32_t n_idle = (uint32_t)__popcll(idle);
+    if (next < end && n_idle >= REFILL) {  // uniform across the wave
+      if (!busy) {
+        uint32_t j = next + (uint32_t)__popcll(idle & below);
+        if (j < end) {
+          i = j;
+          busy = true;
+          TraceRay r;
+          bvh = io.load(i, r);
+          trav_begin<FAST>(s, r, st);
+        }
+      }
+      next += n_idle;
+    }
+  }
+}
+
 // Shared persistent loop: `io.load(i, ray)` returns the ray's BVH, `io.store(i, hit, found)`.
-// Each wave takes TRACE_BATCH x 64 consecutive queue entries per atomic (the fetch counter
-// is one address for the whole chip) and traces them 64 at a time.
-// Consecutive entries are neighbouring pixels (queues are compacted in order), so the
-// lanes of a wave fetch largely the same nodes and triangles.  Refilling finished
-// lanes from the queue as they drain (Aila & Laine 2009, dynamic fetch) was measured
-// 1.2-3.8x slower on MI355X: the refilled lanes hold rays from distant parts of the
-// queue and that coherence is lost.  Giving each XCD (blockIdx % 8) its own contiguous
-// eighth of the queue was also slower (7%): with one queue front the whole chip works
-// on one window of neighbouring rays, which the shared 256 MB Infinity Cache serves.
+// Each wave takes NBATCH x 64 consecutive queue entries per atomic (the fetch counter is
+// one address for the whole chip).  Consecutive entries are neighbouring pixels (queues
+// are compacted in order), so the lanes of a wave fetch largely the same nodes and
+// triangles.  Any-hit rays refill idle lanes from the wave's batch (DGPU_REFILL_ANY):
+// shadow rays end at their first hit, so a wave's rays differ most in length, and
+// refilling cut the terrain render's connect time by a fifth.  Closest-hit rays trace
+// each 64-ray part to its end: refilled, they took 11% longer (the camera rays of
+// bounce 0 are coherent enough that mixing traversal stages costs more than the idle
+// lanes).  Refilling from the global queue instead (Aila & Laine 2009, dynamic fetch)
+// was 1.2-3.8x slower: the refilled lanes hold rays from distant parts of the queue and
+// that coherence is lost.  Giving each XCD (blockIdx % 8) its own contiguous eighth of
+// the queue was also slower (7%): with one queue front the whole chip works on one
+// window of neighbouring rays, which the shared 256 MB Infinity Cache serves.
 template <bool ANY, int WIDTH, int LDS, int BLOCK, bool FAST, class IO>
 __device__ __forceinline__ void trace_loop(IO& io, uint32_t n, uint32_t* fetch, uint32_t* spill,
                                            uint32_t stack_cap) {
+  constexpr uint32_t REFILL = ANY ? DGPU_REFILL_ANY : DGPU_REFILL_CLOSEST;
+  constexpr uint32_t NBATCH = ANY ? TRACE_BATCH_ANY : TRACE_BATCH;
   __shared__ uint32_t stk[LDS * BLOCK];
   LaneStack st;
   st.lds = stk + threadIdx.x;
@@ -94,19 +161,11 @@ __device__ __forceinline__ void trace_loop(IO& io, uint32_t n, uint32_t* fetch, 
   st.cap = stack_cap;
   for (;;) {
     uint32_t base = 0;
-    if (lane_id() == 0) base = atomicAdd(fetch, 64u * TRACE_BATCH);
+    if (lane_id() == 0) base = atomicAdd(fetch, 64u * NBATCH);
     base = __shfl(base, 0);
     if (base >= n) break;
-    for (uint32_t j = 0; j < TRACE_BATCH; ++j) {
-      uint32_t i = base + 64u * j + lane_id();
-      if (i < n) {
-        TraceRay r;
-        BvhRef bvh = io.load(i, r);
-        TraceHit h;
-        bool found = trace_one<ANY, WIDTH, FAST>(bvh, r, st, h);
-        io.store(i, h, found);
-      }
-    }
+    if constexpr (REFILL != 0) trace_batch_refill<ANY, WIDTH, FAST, NBATCH, REFILL>(io, n, base, st);
+    else trace_batch_parts<ANY, WIDTH, FAST, NBATCH>(io, n, base, st);
   }
 }
 
@@ -684,7 +743,8 @@ void k_shade(WaveParams P, uint32_t bounce) {
       const CEntry& E = ((CEntry*)P.batch)[scene];
       // the path's pixel and sample, as k_raygen derived them from its segment
       uint64_t local = seg_local[sg] + (p - seg_begin[sg]);
-      uint32_t sample = E.sample_begin + (uint32_t)(local / E.npix), pixel = (uint32_t)(local % E.npix);
+      uint32_t sample = E.sample_begin + (uint32_t)(local / E.npix);
+      uint32_t pixel = (uint32_t)(local % E.npix);
       shade_body<true, false>(P, *(const DevScene*)&E.S, bounce, active, i, p, scene, pixel, sample, E.k0, E.k1,
                               sh);
     } else {
