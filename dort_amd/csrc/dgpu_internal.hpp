@@ -26,13 +26,13 @@ constexpr uint32_t SHADE_BLOCK = DGPU_SHADE_BLOCK;  // threads per shade workgro
 #endif
 constexpr uint32_t TRACE_BATCH = DGPU_TRACE_BATCH;  // 64-ray batches a trace wave takes per atomic
 // Trace waves refill lanes whose ray has ended from their own batch once this many lanes
-// are idle (0: each 64-ray part of the batch is traced to its end before the next), per
-// ray kind: any-hit (shadow) and closest-hit (extension) rays.
+// are idle (0: each 64-ray part of the batch is traced to its end before the next):
+// any-hit (shadow) rays, and closest-hit (extension) rays after the first bounce.
 #ifndef DGPU_REFILL_ANY
 #define DGPU_REFILL_ANY 16
 #endif
-#ifndef DGPU_REFILL_CLOSEST
-#define DGPU_REFILL_CLOSEST 0
+#ifndef DGPU_REFILL_LATE
+#define DGPU_REFILL_LATE 32
 #endif
 #ifndef DGPU_TRACE_BATCH_ANY
 #define DGPU_TRACE_BATCH_ANY 8

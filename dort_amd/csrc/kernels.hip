@@ -90,8 +90,9 @@ __device__ __forceinline__ void trace_batch_parts(IO& io, uint32_t n, uint32_t b
 // batch (a neighbouring ray of the queue) once at least REFILL lanes of the wave are
 // idle, so the wave no longer runs at the pace of its longest ray with the other lanes
 // masked off.  Each refill hands the idle lanes consecutive entries in lane order.
-template <bool ANY, int WIDTH, bool FAST, uint32_t NBATCH, uint32_t REFILL, class IO>
-__device__ __forceinline__ void trace_batch_refill(IO& io, uint32_t n, uint32_t base, LaneStack& st) {
+template <bool ANY, int WIDTH, bool FAST, uint32_t NBATCH, class IO>
+__device__ __forceinline__ void trace_batch_refill(IO& io, uint32_t n, uint32_t base, LaneStack& st,
+                                                   uint32_t refill) {
   const uint32_t lane = lane_id();
   const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64u - lane));
   const uint32_t end = min(base + 64u * NBATCH, n);
@@ -115,7 +116,7 @@ __device__ __forceinline__ void trace_batch_refill(IO& io, uint32_t n, uint32_t 
     }
     const unsigned long long idle = __ballot(!busy);
     const uint32_t n_idle = (uint32_t)__popcll(idle);
-    if (next < end && n_idle >= REFILL) {  // uniform across the wave
+    if (next < end && n_idle >= refill) {  // uniform across the wave
       if (!busy) {
         uint32_t j = next + (uint32_t)__popcll(idle & below);
         if (j < end) {
@@ -137,18 +138,18 @@ __device__ __forceinline__ void trace_batch_refill(IO& io, uint32_t n, uint32_t 
 // are compacted in order), so the lanes of a wave fetch largely the same nodes and
 // triangles.  Any-hit rays refill idle lanes from the wave's batch (DGPU_REFILL_ANY):
 // shadow rays end at their first hit, so a wave's rays differ most in length, and
-// refilling cut the terrain render's connect time by a fifth.  Closest-hit rays trace
-// each 64-ray part to its end: refilled, they took 11% longer (the camera rays of
-// bounce 0 are coherent enough that mixing traversal stages costs more than the idle
-// lanes).  Refilling from the global queue instead (Aila & Laine 2009, dynamic fetch)
+// refilling cut the terrain render's connect time by a fifth.  Closest-hit rays are
+// refilled after the first bounce (DGPU_REFILL_LATE), which cut extend by an eighth;
+// the camera rays of bounce 0 trace each 64-ray part to its end (refilled, they are
+// coherent enough that mixing traversal stages cost more than the idle lanes).  The
+// threshold is a kernel argument.  Refilling from the global queue instead (Aila & Laine 2009, dynamic fetch)
 // was 1.2-3.8x slower: the refilled lanes hold rays from distant parts of the queue and
 // that coherence is lost.  Giving each XCD (blockIdx % 8) its own contiguous eighth of
 // the queue was also slower (7%): with one queue front the whole chip works on one
 // window of neighbouring rays, which the shared 256 MB Infinity Cache serves.
 template <bool ANY, int WIDTH, int LDS, int BLOCK, bool FAST, class IO>
 __device__ __forceinline__ void trace_loop(IO& io, uint32_t n, uint32_t* fetch, uint32_t* spill,
-                                           uint32_t stack_cap) {
-  constexpr uint32_t REFILL = ANY ? DGPU_REFILL_ANY : DGPU_REFILL_CLOSEST;
+                                           uint32_t stack_cap, uint32_t refill) {
   constexpr uint32_t NBATCH = ANY ? TRACE_BATCH_ANY : TRACE_BATCH;
   __shared__ uint32_t stk[LDS * BLOCK];
   LaneStack st;
@@ -164,7 +165,7 @@ __device__ __forceinline__ void trace_loop(IO& io, uint32_t n, uint32_t* fetch, 
     if (lane_id() == 0) base = atomicAdd(fetch, 64u * NBATCH);
     base = __shfl(base, 0);
     if (base >= n) break;
-    if constexpr (REFILL != 0) trace_batch_refill<ANY, WIDTH, FAST, NBATCH, REFILL>(io, n, base, st);
+    if (refill != 0u) trace_batch_refill<ANY, WIDTH, FAST, NBATCH>(io, n, base, st, refill);
     else trace_batch_parts<ANY, WIDTH, FAST, NBATCH>(io, n, base, st);
   }
 }
@@ -194,7 +195,7 @@ struct ApiIO {
 template <bool ANY, int WIDTH, int LDS, int BLOCK, bool FAST>
 __global__ __launch_bounds__(BLOCK) DG_TRACE_WAVES void k_trace_api(ApiIO io, uint32_t n, uint32_t* fetch,
                                                      uint32_t* spill, uint32_t stack_cap) {
-  trace_loop<ANY, WIDTH, LDS, BLOCK, FAST>(io, n, fetch, spill, stack_cap);
+  trace_loop<ANY, WIDTH, LDS, BLOCK, FAST>(io, n, fetch, spill, stack_cap, ANY ? DGPU_REFILL_ANY : 0u);
 }
 
 // Batched queues carry chunk path indices, like a single render's: a path's scene is the
@@ -295,11 +296,11 @@ template <int WIDTH, int LDS, int BLOCK, bool BATCH>
 __global__ __launch_bounds__(BLOCK) DG_TRACE_WAVES void k_extend(ExtendIO<BATCH> io, const uint32_t* count,
                                                   uint32_t* fetch, uint32_t* spill, uint32_t stack_cap,
                                                   const BatchEntry* batch, const ChunkDesc* chunks,
-                                                  const uint32_t* chunk_cur) {
+                                                  const uint32_t* chunk_cur, uint32_t refill) {
   __shared__ uint32_t seg_begin[BATCH ? MAX_CHUNK_SEGS : 1];
   __shared__ BvhRef seg_bvh[BATCH ? MAX_CHUNK_SEGS : 1];
   io.segs = load_seg_table<BATCH, BLOCK>(batch, chunks, chunk_cur, seg_begin, seg_bvh);
-  trace_loop<false, WIDTH, LDS, BLOCK, true>(io, *count, fetch, spill, stack_cap);
+  trace_loop<false, WIDTH, LDS, BLOCK, true>(io, *count, fetch, spill, stack_cap, refill);
 }
 
 template <int WIDTH, int LDS, int BLOCK, bool BATCH>
@@ -310,7 +311,7 @@ __global__ __launch_bounds__(BLOCK) DG_TRACE_WAVES void k_connect(ConnectIO<BATC
   __shared__ uint32_t seg_begin[BATCH ? MAX_CHUNK_SEGS : 1];
   __shared__ BvhRef seg_bvh[BATCH ? MAX_CHUNK_SEGS : 1];
   io.segs = load_seg_table<BATCH, BLOCK>(batch, chunks, chunk_cur, seg_begin, seg_bvh);
-  trace_loop<true, WIDTH, LDS, BLOCK, true>(io, *count, fetch, spill, stack_cap);
+  trace_loop<true, WIDTH, LDS, BLOCK, true>(io, *count, fetch, spill, stack_cap, DGPU_REFILL_ANY);
 }
 
 __global__ void k_zero_u32(uint32_t* p, uint32_t n) {
@@ -1388,12 +1389,14 @@ hipError_t launch_extend(const WaveParams& P, const KernelCfg& cfg, uint32_t bou
                      BvhRef{P.S.nodes, P.S.tris}, SegTable{nullptr, nullptr, 0u}};
   ExtendIO<true> iob{P.qo[bounce & 1u], P.qd[bounce & 1u], P.qpath[bounce & 1u], P.hit,
                      BvhRef{nullptr, nullptr}, SegTable{nullptr, nullptr, 0u}};
+  // camera rays (bounce 0) are traced in 64-ray parts, scattered rays with refills
+  const uint32_t refill = bounce == 0u ? 0u : (uint32_t)DGPU_REFILL_LATE;
 #define X(W, L, B)                                                                               \
   if (cfg.width == W && cfg.lds == L && cfg.block == B) {                                        \
     if (batch) hipLaunchKernelGGL((k_extend<W, L, B, true>), dim3(cfg.grid), dim3(B), 0, st, iob, \
-                                  count, fetch, P.spill, P.stack_cap, P.batch, P.chunks, P.chunk_cur); \
+                                  count, fetch, P.spill, P.stack_cap, P.batch, P.chunks, P.chunk_cur, refill); \
     else hipLaunchKernelGGL((k_extend<W, L, B, false>), dim3(cfg.grid), dim3(B), 0, st, io, count, \
-                            fetch, P.spill, P.stack_cap, P.batch, P.chunks, P.chunk_cur); \
+                            fetch, P.spill, P.stack_cap, P.batch, P.chunks, P.chunk_cur, refill); \
     return hipGetLastError();                                                                    \
   }
   DG_VARIANTS(X)
