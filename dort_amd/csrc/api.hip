@@ -50,7 +50,20 @@ struct DevBuf {
   size_t bytes = 0;
 };
 
+// Scratch of dgpu_trace_closest_noalloc, for callers that cannot allocate device memory:
+// the persistent kernel's fetch counter (the first 256 bytes) and the traversal stack's
+// spill slices.  One array per GPU, held from the library's first use until the process
+// ends.  Host code never uses the array's name as an address: dgpu_create looks up each
+// device's copy with hipGetSymbolAddress and keeps it in scratch_ptr.
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(32) << 20;
+constexpr size_t NOALLOC_COUNTER_BYTES = 256;
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex scratch_mu[NOALLOC_MAX_DEVICES];       // one call at a time per GPU
+uint8_t* scratch_ptr[NOALLOC_MAX_DEVICES] = {};   // the array's address on each GPU
+
 }  // namespace
+
+static __device__ __align__(256) uint8_t scratch[NOALLOC_SCRATCH_BYTES];
 
 struct dgpu_ctx {
   int device = 0;
@@ -158,6 +171,7 @@ struct dgpu_scene {
   std::vector<void*> allocs;
   KernelCfg cfg;
   uint32_t queue_tile = 0;
+  uint32_t closest_grid = 0;  // persistent grid of the closest-hit intersection kernel
   uint32_t stack_cap = MAX_STACK;
   uint32_t num_area = 0;
   dgpu_scene_info info;
@@ -256,6 +270,18 @@ dgpu_status dgpu_create(int device, dgpu_ctx** out) {
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) != hipSuccess) { delete c; return fail(DGPU_EDEVICE, "hipGetDeviceProperties failed"); }
   c->num_cus = prop.multiProcessorCount;
+  if (device < NOALLOC_MAX_DEVICES) {
+    std::lock_guard<std::mutex> lk(scratch_mu[device]);
+    if (!scratch_ptr[device]) {
+      void* p = nullptr;
+      hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+      if (e != hipSuccess || !p) {
+        delete c;
+        return fail(DGPU_EDEVICE, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+      }
+      scratch_ptr[device] = (uint8_t*)p;
+    }
+  }
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
     delete c;
@@ -347,6 +373,7 @@ dgpu_status dgpu_scene_create(dgpu_ctx* ctx, const dgpu_scene_desc* d, const dgp
   dgpu_scene* s = new dgpu_scene();
   s->ctx = ctx;
   s->cfg = cfg;
+  s->closest_grid = ctx->grid_for(0, cfg);  // kept for dgpu_trace_closest_noalloc, which allocates nothing
   s->queue_tile = o.queue_tile;
   DevScene& S = s->S;
   memset(&S, 0, sizeof S);
@@ -675,6 +702,50 @@ dgpu_status dgpu_trace_closest(dgpu_scene* s, const dgpu_rays* rays, uint32_t n,
                                uint32_t flags, double* kernel_ms) {
   if (!out) return fail(DGPU_EINVAL, "null hits");
   return trace_common(s, rays, n, out, nullptr, flags, kernel_ms);
+}
+
+dgpu_status dgpu_trace_closest_noalloc(dgpu_scene* s, const dgpu_rays* rays, uint32_t n, dgpu_hits* out,
+                                       uint32_t flags, double* kernel_ms) {
+  if (!s || !rays || !out) return fail(DGPU_EINVAL, "null scene, rays or hits");
+  if (!out->t || !out->prim || !out->b1 || !out->b2) return fail(DGPU_EINVAL, "null hit buffers");
+  const void* rp[8] = {rays->ox, rays->oy, rays->oz, rays->dx, rays->dy, rays->dz, rays->tmin, rays->tmax};
+  for (const void* p : rp) if (!p) return fail(DGPU_EINVAL, "null ray component");
+  if (!(flags & DGPU_BUF_DEVICE))
+    return fail(DGPU_EINVAL, "dgpu_trace_closest_noalloc takes device buffers only (DGPU_BUF_DEVICE)");
+  if (flags & ~(uint32_t)(DGPU_BUF_DEVICE | DGPU_TRACE_BOUNDED)) return fail(DGPU_EINVAL, "unknown flags 0x%x", flags);
+  if (kernel_ms) *kernel_ms = 0.0;
+  if (n == 0) return DGPU_OK;
+  dgpu_ctx* ctx = s->ctx;
+  if (ctx->device >= NOALLOC_MAX_DEVICES || !scratch_ptr[ctx->device])
+    return fail(DGPU_EUNSUPPORTED, "no scratch array on device %d", ctx->device);
+  // The spill slices of G threads take G * (stack entries beyond LDS) words after the
+  // counter area; the persistent grid shrinks to what the array holds.
+  KernelCfg cfg = s->cfg;
+  const size_t spill_words = s->stack_cap > cfg.lds ? s->stack_cap - cfg.lds : 0;
+  const size_t block_bytes = (size_t)cfg.block * spill_words * 4;
+  const size_t avail = NOALLOC_SCRATCH_BYTES - NOALLOC_COUNTER_BYTES;
+  const size_t max_blocks = block_bytes ? avail / block_bytes : (size_t)s->closest_grid;
+  if (max_blocks == 0)
+    return fail(DGPU_EUNSUPPORTED, "traversal stack of %u entries needs %zu bytes per workgroup, above the %zu-byte "
+                "scratch", s->stack_cap, block_bytes, avail);
+  cfg.grid = (uint32_t)std::min<size_t>(s->closest_grid, max_blocks);
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  std::lock_guard<std::mutex> dlk(scratch_mu[ctx->device]);
+  HIP_TRY(hipSetDevice(ctx->device));
+  uint32_t* fetch = (uint32_t*)scratch_ptr[ctx->device];
+  uint32_t* spill = (uint32_t*)(scratch_ptr[ctx->device] + NOALLOC_COUNTER_BYTES);
+  hipStream_t stream = ctx->stream;
+  HIP_TRY(hipEventRecord(ctx->ev0, stream));
+  HIP_TRY(launch_trace_api(s->S, cfg, *rays, out, nullptr, n, fetch, spill, (flags & DGPU_TRACE_BOUNDED) != 0,
+                           stream));
+  HIP_TRY(hipEventRecord(ctx->ev1, stream));
+  HIP_TRY(hipStreamSynchronize(stream));  // the device's scratch is free again once this returns
+  if (kernel_ms) {
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    *kernel_ms = ms;
+  }
+  return DGPU_OK;
 }
 
 dgpu_status dgpu_trace_any(dgpu_scene* s, const dgpu_rays* rays, uint32_t n, uint32_t* occluded,

@@ -19,7 +19,7 @@ _lib = None
 EXPORTS = ["dgpu_abi_version", "dgpu_last_error", "dgpu_create", "dgpu_destroy", "dgpu_device_alloc",
            "dgpu_device_free", "dgpu_copy", "dgpu_scene_create", "dgpu_scene_destroy",
            "dgpu_scene_get_info", "dgpu_scene_update_texture", "dgpu_scene_update_material",
-           "dgpu_trace_closest", "dgpu_trace_any", "dgpu_render", "dgpu_render_light", "dgpu_render_batch",
+           "dgpu_trace_closest", "dgpu_trace_any", "dgpu_trace_closest_noalloc", "dgpu_render", "dgpu_render_light", "dgpu_render_batch",
            "dgpu_adjoint",
            "dgpu_read_ply", "dgpu_free", "dgpu_read_rgbe", "dgpu_write_rgbe", "dgpu_read_png", "dgpu_write_png",
            "dgpu_test_convergence", "dgpu_bias_variance"]
@@ -52,6 +52,8 @@ def lib():
                                          C.POINTER(abi.Hits), C.c_uint32, C.POINTER(C.c_double)]
         L.dgpu_trace_any.argtypes = [C.c_void_p, C.POINTER(abi.Rays), C.c_uint32, C.c_void_p,
                                      C.c_uint32, C.POINTER(C.c_double)]
+        L.dgpu_trace_closest_noalloc.argtypes = [C.c_void_p, C.POINTER(abi.Rays), C.c_uint32,
+                                                 C.POINTER(abi.Hits), C.c_uint32, C.POINTER(C.c_double)]
         L.dgpu_render.argtypes = [C.c_void_p, C.POINTER(abi.RenderOpts), C.c_void_p, C.c_void_p,
                                   C.POINTER(abi.RenderStats), C.c_void_p]
         L.dgpu_render_light.argtypes = [C.c_void_p, C.POINTER(abi.RenderOpts), C.c_void_p,

@@ -239,6 +239,19 @@ dgpu_status dgpu_trace_closest(dgpu_scene* scene, const dgpu_rays* rays, uint32_
 dgpu_status dgpu_trace_any(dgpu_scene* scene, const dgpu_rays* rays, uint32_t n,
                            uint32_t* occluded, uint32_t flags, double* kernel_ms);
 
+/* dgpu_trace_closest that allocates no memory.  No counterpart in dort: it is for callers
+ * that cannot allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Rays and hits are device buffers (DGPU_BUF_DEVICE must be set;
+ * DGPU_TRACE_BOUNDED may be); the hits equal dgpu_trace_closest's.  The kernel's scratch
+ * (its work counter and the traversal stack's spill slices) is a fixed 32 MiB array on
+ * each GPU, held from the library's first use until the process ends; the persistent grid
+ * shrinks to fit it, and a scene whose stack would not fit one workgroup is refused with
+ * DGPU_EUNSUPPORTED.  Every call on a GPU shares that GPU's array, so calls that overlap on
+ * one GPU would race: the library serialises them per GPU and returns once the kernel is
+ * done. */
+dgpu_status dgpu_trace_closest_noalloc(dgpu_scene* scene, const dgpu_rays* rays, uint32_t n,
+                                       dgpu_hits* out, uint32_t flags, double* kernel_ms);
+
 /* Replace the texels of texture `tex` (width*height*3 fp32, host memory) or one
  * material, without rebuilding the BVH (inverse rendering, finite differences). */
 dgpu_status dgpu_scene_update_texture(dgpu_scene* scene, uint32_t tex, const float* rgb);
