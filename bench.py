@@ -44,9 +44,10 @@ def parse():
     ap.add_argument("--res", type=int, default=1024)
     ap.add_argument("--spp", type=int, default=64)
     ap.add_argument("--quads", type=int, default=1000)
-    ap.add_argument("--cpu-spp", type=int, default=64,
-                    help="samples per pixel of the CPU sample (64: the whole terrain workload, about 15 s on 16 threads)")
-    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-spp", type=int, default=16,
+                    help="samples per pixel of the oracle's CPU sample (16: about 4 s on 16 threads)")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="CPU workers (default: the CPUs this process may use, capped by OMP_NUM_THREADS)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-intersect", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
@@ -140,10 +141,59 @@ def light_bench(gsc, film, args):
             "shadow_rays_per_path": st["shadow_rays"] / st["paths"]}
 
 
+def adjoint_errors(ctx, gsc, guess):
+    """RMS errors of the GPU adjoint and of central finite differences (h = 1e-3, common
+    random numbers) against the FP64 adjoint, on the inverse-rendering scene rendered at
+    64x64, 4 spp (tests/golden/adjoint_terrain200k_64x64_4spp_fp64.npz, written by the
+    oracle's double-precision path replay).  The unknowns compared: the 64 texel entries
+    with the largest reference gradients and every GGX roughness; the loss is the
+    fixture's seeded linear functional sum(dLdI * image)."""
+    from dort_amd import abi
+    fix = np.load(os.path.join(ROOT, "tests", "golden", "adjoint_terrain200k_64x64_4spp_fp64.npz"))
+    r_tex, r_rough, dldi = fix["g_tex"], fix["g_rough"], fix["dldi"]
+    opts = abi.render_opts(64, 64, spp=4, seed=77, max_depth=6, rr_depth=3)
+    g_tex, g_rough, _, _ = gsc.adjoint(opts, dldi)
+    texels = np.argsort(-np.abs(r_tex).ravel())[:64]
+    mats = [m for m in range(len(guess.materials)) if guess.materials[m].type == abi.MAT_GGX_CONDUCTOR]
+    ref = np.concatenate([r_tex.ravel()[texels], r_rough[mats]])
+    adj = np.concatenate([g_tex.ravel()[texels], g_rough[mats]])
+    h = 1e-3
+
+    def loss():
+        return float((gsc.image(opts).astype(np.float64) * dldi).sum())
+
+    fd = []
+    tex = guess.textures[0].astype(np.float32)
+    for k in texels:
+        vals = []
+        for sgn in (1.0, -1.0):
+            t = tex.copy()
+            t.reshape(-1)[k] += sgn * h
+            gsc.update_texture(0, t)
+            vals.append(loss())
+        fd.append((vals[0] - vals[1]) / (2 * h))
+    gsc.update_texture(0, tex)
+    for m in mats:
+        vals = []
+        for sgn in (1.0, -1.0):
+            mm = abi.Material.from_buffer_copy(guess.materials[m])
+            mm.roughness += sgn * h
+            gsc.update_material(m, mm)
+            vals.append(loss())
+        fd.append((vals[0] - vals[1]) / (2 * h))
+        gsc.update_material(m, guess.materials[m])
+    rms = lambda v: float(np.sqrt(np.mean(np.square(v))))
+    return rms(adj - ref) / rms(ref), rms(np.asarray(fd) - ref) / rms(ref), len(ref)
+
+
 def adjoint_bench(ctx, args):
     """Differentiable inverse rendering workload: 200k-triangle terrain, 512x512, 32 spp,
     max depth 6; unknowns = 256x256 RGB albedo texture + 8 per-object GGX roughness;
-    loss = pixelwise L2 against a target rendered from hidden seeded parameters."""
+    loss = pixelwise L2 against a target rendered from hidden seeded parameters.  Central
+    differences need two renders per unknown for one gradient; their error and the
+    adjoint's are measured against the FP64 adjoint (adjoint_errors), so the adjoint's
+    speed-up holds at equal gradient error whenever central differences are no more
+    accurate than the adjoint."""
     from dort_amd import abi, scenes
     hidden = scenes.terrain_scene(quads=316, seed=8, textured=True, tex_res=256, sky_res=(1024, 512),
                                   ggx_roughness=np.linspace(0.08, 0.45, 8))
@@ -170,11 +220,15 @@ def adjoint_bench(ctx, args):
     t_fwd, t_adj = float(np.median(fwd)), float(np.median(adj))
     grads_per_s = 1.0 / (t_fwd + t_adj)
     fd_grads_per_s = 1.0 / (2 * n_params * t_fwd)
+    err_adj, err_fd, n_cmp = adjoint_errors(ctx, gsc, guess)
+    equal_error = err_fd >= err_adj
     return {"workload": f"terrain {guess.num_triangles} tris, 512x512, 32 spp, depth 6, "
                         f"{n_params} unknowns (256x256x3 albedo + 8 roughness)",
             "grads_per_s": grads_per_s, "fd_grads_per_s": fd_grads_per_s,
-            "speedup_vs_fd": grads_per_s / fd_grads_per_s, "forward_ms": t_fwd * 1e3,
-            "adjoint_ms": t_adj * 1e3, "adjoint_kernel_ms": st["kernel_ms"],
+            "forward_ms": t_fwd * 1e3, "adjoint_ms": t_adj * 1e3, "adjoint_kernel_ms": st["kernel_ms"],
+            "rms_rel_error_vs_fp64": {"adjoint": err_adj, "central_fd_h1e-3": err_fd, "unknowns": n_cmp,
+                                      "scene": "the same scene at 64x64, 4 spp"},
+            "speedup_vs_fd_at_equal_error": (grads_per_s / fd_grads_per_s) if equal_error else None,
             "loss": float(((img - target) ** 2).sum()),
             "grad_norm": float(np.sqrt((g_tex.astype(np.float64) ** 2).sum() + (g_rough ** 2).sum()))}
 
@@ -238,19 +292,37 @@ def batch_bench(ctx, args):
 
 
 def cpu_baseline(sc, args):
+    """dort itself (oracle/_ref/libdortref.so, its sources built by oracle/ref.mk) on the
+    host's CPU share: one single-threaded dort process per CPU, each rendering the whole
+    1024x1024 image for one iteration with its own seed (oracle/dort_baseline.py), timed
+    together; and, beside it, the oracle (this build's C++ restatement of dort's pt, which
+    traces the same paths as dort, 1.5-1.9x faster single-threaded) on the exact scene.
+    Where dort is not built the oracle is the baseline."""
     from dort_amd import abi
-    from oracle import oracle
-    threads = args.cpu_threads or max(1, min(os.cpu_count() or 1, 16))
+    from oracle import dort_baseline, oracle, ref
+    info = dort_baseline.cpu_info()
+    procs = args.cpu_threads or dort_baseline.cpu_share()
     o = oracle.OracleScene(sc)
     opts = abi.render_opts(args.res, args.res, spp=args.cpu_spp, seed=1234, max_depth=8, rr_depth=3)
     t0 = time.time()
-    _, _, cnt = o.render(opts, rng="mt", threads=threads)
+    _, _, cnt = o.render(opts, rng="mt", threads=procs)
     dt = time.time() - t0
-    return {"value": cnt["paths"] / dt / 1e6, "unit": "Mpaths/s", "cores": threads, "kind": "port",
+    port = {"value": cnt["paths"] / dt / 1e6, "unit": "Mpaths/s", "cores": procs, "kind": "port",
             "sample": f"{args.res}x{args.res} x {args.cpu_spp} spp of the same terrain scene "
                       f"({cnt['paths']} paths, {dt:.1f}s, dort's mt19937 sampler)",
             "extension_rays_per_path": cnt["extension_rays"] / cnt["paths"],
             "shadow_rays_per_path": cnt["shadow_rays"] / cnt["paths"]}
+    host = {"nproc": info["nproc"], "allowed_cpus": info["allowed_cpus"], "cpu_model": info["cpu_model"],
+            "smt": info["smt"], "cpu_share": procs}
+    if not ref.available():
+        return {**port, **host}
+    d = dort_baseline.run(procs, 1, res=args.res, quads=args.quads)
+    return {"value": d["Mpaths_s"], "unit": "Mpaths/s", "cores": procs, "kind": "reference",
+            "sample": f"dort's PathRenderer, {procs} single-threaded processes x 1 iteration of "
+                      f"{args.res}x{args.res} ({d['paths']} paths in {d['seconds']:.1f}s), the terrain "
+                      "with its GGX conductors as Lambert (dort has no rough conductor) and dort's RR "
+                      "from bounce 2",
+            **host, "port": port}
 
 
 def main():
@@ -344,8 +416,10 @@ def main():
     if not args.no_adjoint:
         secondary["adjoint"] = adjoint_bench(ctx, args)
         a = secondary["adjoint"]
-        log(f"adjoint: {a['grads_per_s']:.2f} grads/s, FD {a['fd_grads_per_s']:.2e} grads/s "
-            f"({a['speedup_vs_fd']:.0f}x), forward {a['forward_ms']:.0f} ms, adjoint {a['adjoint_ms']:.0f} ms")
+        e = a["rms_rel_error_vs_fp64"]
+        log(f"adjoint: {a['grads_per_s']:.2f} grads/s, FD {a['fd_grads_per_s']:.2e} grads/s, forward "
+            f"{a['forward_ms']:.0f} ms, adjoint {a['adjoint_ms']:.0f} ms; RMS error vs FP64: adjoint "
+            f"{e['adjoint']:.2e}, central FD {e['central_fd_h1e-3']:.2e}")
 
     if not args.no_batch:
         secondary["batch"] = batch_bench(ctx, args)
@@ -355,9 +429,10 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        log("timing the CPU baseline (oracle) ...")
+        log("timing the CPU baseline ...")
         cpu = cpu_baseline(sc, args)
-        log(f"cpu: {cpu['value']:.2f} Mpaths/s on {cpu['cores']} threads")
+        log(f"cpu ({cpu['kind']}): {cpu['value']:.3f} Mpaths/s on {cpu['cores']} cores of {cpu['nproc']} "
+            f"({cpu['cpu_model']}, SMT {cpu['smt']})")
 
     line = {
         "metric": "Mpaths/s at equal rMSE vs CPU", "value": round(value, 3), "unit": "Mpaths/s",
