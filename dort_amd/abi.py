@@ -11,7 +11,7 @@ DGPU_OK, DGPU_EINVAL, DGPU_ENOMEM, DGPU_EDEVICE, DGPU_EUNSUPPORTED, DGPU_ECANCEL
 STATUS_NAMES = {0: "DGPU_OK", 1: "DGPU_EINVAL", 2: "DGPU_ENOMEM", 3: "DGPU_EDEVICE",
                 4: "DGPU_EUNSUPPORTED", 5: "DGPU_ECANCELLED"}
 
-MAT_NONE, MAT_LAMBERT, MAT_GGX_CONDUCTOR, MAT_DIELECTRIC = range(4)
+MAT_NONE, MAT_LAMBERT, MAT_GGX_CONDUCTOR, MAT_DIELECTRIC, MAT_ROUGH_DIELECTRIC = range(5)
 DIRECT_MIS, DIRECT_BSDF, DIRECT_LIGHT = range(3)
 BUF_DEVICE = 1
 RENDER_PROFILE = 2

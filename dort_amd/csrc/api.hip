@@ -212,7 +212,10 @@ dgpu_status validate_desc(const dgpu_scene_desc* d) {
     if (!std::isfinite(d->positions[i])) return fail(DGPU_EINVAL, "non-finite vertex coordinate");
   for (uint32_t m = 0; m < d->num_materials; ++m) {
     const dgpu_material& mt = d->materials[m];
-    if (mt.type > DGPU_MAT_DIELECTRIC) return fail(DGPU_EUNSUPPORTED, "material %u: unsupported type %u", m, mt.type);
+    if (mt.type > DGPU_MAT_ROUGH_DIELECTRIC)
+      return fail(DGPU_EUNSUPPORTED, "material %u: unsupported type %u", m, mt.type);
+    if (mt.type == DGPU_MAT_ROUGH_DIELECTRIC && !(mt.roughness >= 0.f && std::isfinite(mt.roughness)))
+      return fail(DGPU_EINVAL, "material %u: rough dielectric roughness must be finite and >= 0", m);
     if (mt.type == DGPU_MAT_LAMBERT && mt.albedo_tex >= (int32_t)d->num_textures)
       return fail(DGPU_EINVAL, "material %u: texture %d out of range", m, mt.albedo_tex);
     if (mt.type == DGPU_MAT_GGX_CONDUCTOR && !(mt.roughness > 0.f))
@@ -1037,7 +1040,7 @@ dgpu_status dgpu_scene_update_texture(dgpu_scene* s, uint32_t tex, const float* 
 dgpu_status dgpu_scene_update_material(dgpu_scene* s, uint32_t material, const dgpu_material* m) {
   if (!s || !m) return fail(DGPU_EINVAL, "null scene or material");
   if (material >= s->mats_host.size()) return fail(DGPU_EINVAL, "material %u out of range", material);
-  if (m->type > DGPU_MAT_DIELECTRIC) return fail(DGPU_EUNSUPPORTED, "unsupported material type %u", m->type);
+  if (m->type > DGPU_MAT_ROUGH_DIELECTRIC) return fail(DGPU_EUNSUPPORTED, "unsupported material type %u", m->type);
   if (m->type == DGPU_MAT_LAMBERT && m->albedo_tex >= (int32_t)s->tex_info_host.size())
     return fail(DGPU_EINVAL, "texture %d out of range", m->albedo_tex);
   if (m->type == DGPU_MAT_GGX_CONDUCTOR && !(m->roughness > 0.f))

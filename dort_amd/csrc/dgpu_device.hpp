@@ -171,9 +171,10 @@ enum { BSDF_REFLECTION = 1, BSDF_TRANSMISSION = 2, BSDF_DIFFUSE = 4, BSDF_GLOSSY
        BSDF_DELTA = 16, BSDF_MODES = 3, BSDF_LOBES = 28, BSDF_ALL = 31 };
 
 // The lobe parameters share three colour slots and two scalars; which kind owns them:
-//   Lambert:        c0 = albedo
-//   GGX conductor:  c0 = tint, c1 = eta, c2 = k, a0 = alpha
-//   dielectric:     c0 = reflect tint, c1 = transmit tint, a0 = ior inside, a1 = ior outside
+//   Lambert:          c0 = albedo
+//   GGX conductor:    c0 = tint, c1 = eta, c2 = k, a0 = alpha
+//   dielectric:       c0 = reflect tint, c1 = transmit tint, a0 = ior inside, a1 = ior outside
+//   rough dielectric: as the dielectric, and c2.x = alpha
 // Every field is always written, so the record lives in registers, never on the stack.
 struct Bsdf {
   V3 nn, sn;       // geometric == shading normal (no shading normals on the GPU path); tn = nn x sn
@@ -181,12 +182,13 @@ struct Bsdf {
   V3 c0, c1, c2;
   float a0, a1;
   // lobe flags by kind: NONE -, LAMBERT reflection|diffuse, GGX reflection|glossy,
-  // DIELECTRIC reflection|transmission|delta
+  // DIELECTRIC reflection|transmission|delta, ROUGH_DIELECTRIC reflection|transmission|glossy
   DG_INLINE uint32_t flags() const {
-    constexpr uint32_t T = ((uint32_t)(BSDF_REFLECTION | BSDF_TRANSMISSION | BSDF_DELTA) << 24) |
-                           ((uint32_t)(BSDF_REFLECTION | BSDF_GLOSSY) << 16) |
-                           ((uint32_t)(BSDF_REFLECTION | BSDF_DIFFUSE) << 8);
-    return (T >> (8u * kind)) & 0xFFu;
+    constexpr uint64_t T = ((uint64_t)(BSDF_REFLECTION | BSDF_TRANSMISSION | BSDF_GLOSSY) << 32) |
+                           ((uint64_t)(BSDF_REFLECTION | BSDF_TRANSMISSION | BSDF_DELTA) << 24) |
+                           ((uint64_t)(BSDF_REFLECTION | BSDF_GLOSSY) << 16) |
+                           ((uint64_t)(BSDF_REFLECTION | BSDF_DIFFUSE) << 8);
+    return (uint32_t)(T >> (8u * kind)) & 0xFFu;
   }
   DG_INLINE V3 tn() const { return cross(nn, sn); }
   DG_INLINE V3 albedo() const { return c0; }
@@ -301,6 +303,74 @@ DG_INLINE float conductor_dlnf_dalpha(const Bsdf& b, V3 wi, V3 wo) {
   return ggx_dlnd(b.alpha(), h) + ggx_dlng1(b.alpha(), wi) + ggx_dlng1(b.alpha(), wo);
 }
 
+// dort's RoughDielectricBxdf as dort runs it (rough_dielectric_material.cpp:7-185), GGX
+// distribution (every MicrofacetType falls through to the GGX formulas, microfacet.cpp).
+// get_iors picks the indices by the side of its argument (:169-179).
+DG_INLINE void rough_iors(const Bsdf& b, V3 w, float& ir, float& it) {
+  if (w.z >= 0.f) { ir = b.ior_out(); it = b.ior_in(); } else { ir = b.ior_in(); it = b.ior_out(); }
+}
+// eval_f (:7-37): reflection or transmission by the request's mode (the Bsdf layer chose
+// it from the hemispheres); the Fresnel term takes both directions' dot products with h.
+DG_INLINE V3 rough_dielectric_f(const Bsdf& b, V3 wi, V3 wo, bool reflect) {
+  float ir, it;
+  rough_iors(b, wi, ir, it);
+  const float alpha = b.c2.x;
+  V3 h = reflect ? normalize(wi + wo) : -normalize(wi * ir + wo * it);
+  float cin = fabsf(wi.z), con = fabsf(wo.z);
+  float cih = fabsf(dot(wi, h)), coh = fabsf(dot(wo, h));
+  float fr = fresnel_dielectric(cih, coh, ir, it);
+  float g = ggx_g1(alpha, wi, h) * ggx_g1(alpha, wo, h);
+  float d = ggx_d(alpha, h);
+  if (reflect) return b.refl() * (fr * g * d * 0.25f / (cin * con));
+  return b.trans() * (cih * coh * sqf(it) * (1.f - fr) * g * d / (cin * con * sqf(ir * cih - it * coh)));
+}
+// sample_f (:69-147): a normal from D(m) cos(m) (MicrofacetDistrib::sample_m, GGX,
+// microfacet.cpp:67-86: x = sin(theta) sin(phi), y = sin(theta) cos(phi)) turned to the
+// fixed direction's side; reflection is picked with fresnel_dielectric_refl's probability
+// (fresnel.cpp:38-47); a transmission pick returns pdf 0 (dort's transmission sampling is
+// commented out, :114-128), as does a reflection whose pdf is below 1e-5.
+DG_INLINE V3 rough_dielectric_sample(const Bsdf& b, V3 wf, uint32_t req, float u, float v, float c, V3& wg,
+                                     float& pdf, uint32_t& fl) {
+  V3 zero = v3(0.f, 0.f, 0.f);
+  bool s_refl = req & BSDF_REFLECTION, s_trans = req & BSDF_TRANSMISSION;
+  float ir, it;
+  rough_iors(b, wf, ir, it);
+  const float alpha = b.c2.x;
+  float ct = 1.f / sqrtf((sqf(alpha) * u) / (1.f - u) + 1.f);
+  float st = sqrtf(1.f - sqf(ct));
+  float sp, cp;
+  sincos_small(v * TWO_PI_F, sp, cp);
+  V3 m = v3(st * sp, st * cp, ct);
+  float m_pdf = ggx_d(alpha, m) * ct;
+  m.z = copysignf(m.z, wf.z);
+  float wfm = dot(wf, m);
+  float cr = fabsf(wfm);
+  float ct2 = 1.f - sqf(ir / it) * (1.f - sqf(cr));
+  float fr = ct2 <= 0.f ? 1.f : fresnel_dielectric(cr, sqrtf(ct2), ir, it);
+  bool reflect;
+  float pick;
+  if (s_refl && s_trans) {
+    reflect = c <= fr;
+    pick = reflect ? fr : 1.f - fr;
+  } else if (s_refl) {
+    reflect = true; pick = 1.f;
+  } else if (s_trans && fr < 1.f) {
+    reflect = false; pick = 1.f;
+  } else {
+    return zero;
+  }
+  if (!reflect) return zero;
+  V3 w = m * (2.f * wfm) - wf;
+  float base_pdf = m_pdf * 0.25f / fabsf(wfm);
+  if (base_pdf < 1e-5f) return zero;
+  float g = ggx_g1(alpha, wf, m) * ggx_g1(alpha, w, m);
+  float d = ggx_d(alpha, m);
+  wg = w;
+  pdf = pick * base_pdf;
+  fl = BSDF_GLOSSY | BSDF_REFLECTION;
+  return b.refl() * (0.25f * fr) * (g * d / fabsf(wf.z * w.z));
+}
+
 // Bsdf::eval_f (bsdf.cpp:72-93); shading frame == geometric frame, so shading_to_geom = 1
 // except where |n.wi| == 0.
 DG_INLINE V3 bsdf_eval(const Bsdf& b, V3 wi, V3 wo, uint32_t req) {
@@ -313,6 +383,7 @@ DG_INLINE V3 bsdf_eval(const Bsdf& b, V3 wi, V3 wo, uint32_t req) {
   if (ds * ds == 0.f) return v3(0.f, 0.f, 0.f);  // shading_to_geom == 0
   if (b.kind == DGPU_MAT_LAMBERT) return b.albedo() * INV_PI_F;
   if (b.kind == DGPU_MAT_GGX_CONDUCTOR) return conductor_f(b, wil, wol);
+  if (b.kind == DGPU_MAT_ROUGH_DIELECTRIC) return rough_dielectric_f(b, wil, wol, (ereq & BSDF_REFLECTION) != 0u);
   return v3(0.f, 0.f, 0.f);
 }
 
@@ -321,7 +392,7 @@ DG_INLINE float bsdf_pdf(const Bsdf& b, V3 wi, V3 wo, uint32_t req) {
   V3 wil = b.to_local(wi), wol = b.to_local(wo);
   if (b.kind == DGPU_MAT_LAMBERT) return fabsf(wil.z * INV_PI_F);
   if (b.kind == DGPU_MAT_GGX_CONDUCTOR) return conductor_pdf(b, wil, wol);
-  return 0.f;
+  return 0.f;  // dielectric; rough dielectric: f_pdf is 0 (rough_dielectric_material.cpp:149-155)
 }
 
 // Bsdf::sample_light_f with one lobe (bsdf.cpp:95-192) and the lobes'
@@ -361,6 +432,9 @@ DG_INLINE V3 bsdf_sample(const Bsdf& b, V3 wo, uint32_t req, float u, float v, f
     if (!(p > 0.f)) return zero;
     fl = BSDF_REFLECTION | BSDF_GLOSSY;
     f = conductor_f(b, wil, wol);
+  } else if (b.kind == DGPU_MAT_ROUGH_DIELECTRIC) {
+    f = rough_dielectric_sample(b, wol, req, u, v, c, wil, p, fl);
+    if (p == 0.f) return zero;
   } else if (!NON_DELTA && b.kind == DGPU_MAT_DIELECTRIC) {
     bool s_refl = req & BSDF_REFLECTION, s_trans = req & BSDF_TRANSMISSION;
     float ir, it;
@@ -495,6 +569,15 @@ DG_INLINE Bsdf make_bsdf(const DevScene& S, const Surface& g, int32_t* texel_out
     if (m.ior_in > 0.f && m.albedo.w > 0.f && !(is_black(r) && is_black(t))) {
       b.kind = DGPU_MAT_DIELECTRIC;
       b.c0 = r; b.c1 = t; b.a0 = m.ior_in; b.a1 = m.albedo.w;
+    }
+  } else if (m.type == DGPU_MAT_ROUGH_DIELECTRIC) {
+    // RoughDielectricMaterial::add_bxdfs (rough_dielectric_material.cpp:166-185): no lobe for
+    // black tints, the smooth dielectric below roughness 0.001
+    V3 r = xyz(m.refl), t = xyz(m.trans);
+    if (!(is_black(r) && is_black(t))) {
+      b.kind = (double)m.roughness < 0.001 ? DGPU_MAT_DIELECTRIC : DGPU_MAT_ROUGH_DIELECTRIC;
+      b.c0 = r; b.c1 = t; b.a0 = m.ior_in; b.a1 = m.albedo.w;
+      b.c2 = v3(m.roughness, 0.f, 0.f);
     }
   }
   return b;

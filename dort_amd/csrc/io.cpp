@@ -439,6 +439,10 @@ struct Inflater {
     uint16_t symbol[320];
   };
 
+  // Canonical Huffman codes (RFC 1951 3.2.2) kept as a count of codes per length and the
+  // symbols in code order, decoded one bit at a time by comparing the code read so far
+  // with the first code of each length: the count/symbol scheme of Mark Adler's puff.c
+  // (zlib's contrib/puff, public domain), written here from that published method.
   // Returns false for an over-subscribed code; incomplete codes are allowed (a lone
   // distance code is legal) and fail only if an unused code turns up.
   static bool build(Huff& h, const uint8_t* len, int nsym) {

@@ -89,6 +89,14 @@ def dielectric(ior_inside=1.5, ior_outside=1.0, reflect=(1, 1, 1), transmit=(1, 
     return m
 
 
+def rough_dielectric(roughness, ior_inside=1.5, ior_outside=1.0, reflect=(1, 1, 1), transmit=(1, 1, 1)):
+    """dort's rough dielectric (rough_dielectric_material.cpp) as dort runs it."""
+    m = dielectric(ior_inside, ior_outside, reflect, transmit)
+    m.type = abi.MAT_ROUGH_DIELECTRIC
+    m.roughness = float(roughness)
+    return m
+
+
 # ---------------------------------------------------------------- transforms
 
 def _normalize(v):
@@ -252,6 +260,43 @@ def glass_cornell():
     tri_mat = sc.tri_material.copy()
     tri_mat[22:34] = len(mats) - 1  # 10 wall + 12 short-box triangles come first
     return replace(sc, materials=mats, tri_material=tri_mat)
+
+
+def rough_glass_cornell(roughness=0.2):
+    """The Cornell box with its tall block made of dort's rough dielectric (IOR 1.5, GGX
+    alpha = roughness) and its short block a smooth dielectric: glossy reflection sampled
+    with the Fresnel probability, transmission reached only through light sampling
+    (light paths that enter the block through its smooth-glass neighbour's caustics and
+    the walls' bounces)."""
+    sc = cornell()
+    mats = list(sc.materials) + [rough_dielectric(roughness, 1.5), dielectric(1.5)]
+    tri_mat = sc.tri_material.copy()
+    tri_mat[22:34] = len(mats) - 2   # tall block
+    tri_mat[10:22] = len(mats) - 1   # short block
+    # the blocks' triangles are wound with normalize(cross(e2, e1)) pointing into the
+    # block; the rough block is rewound so its normals point out, which puts the camera
+    # and the light on the lobe's outer (z > 0) side, where its reflection is nonzero
+    idx = sc.indices.copy()
+    idx[22:34] = idx[22:34][:, [0, 2, 1]]
+    return replace(sc, materials=mats, tri_material=tri_mat, indices=idx, name=f"rough_glass_cornell_{roughness}")
+
+
+def rough_pane(roughness=0.3):
+    """A square pane of dort's rough dielectric (IOR 1.5) standing between the camera
+    and a square area light of radiance (20, 18, 14), over a grey Lambert floor: light
+    reaches the camera through the pane by light sampling (the lobe's transmission term)
+    and is reflected off it toward the floor."""
+    b = _Builder()
+    glass = b.material(rough_dielectric(roughness, 1.5))
+    floor = b.material(lambert((0.5, 0.5, 0.5)))
+    blk = b.material(black())
+    b.mesh([(-3, 0, -3), (3, 0, -3), (3, 0, 3), (-3, 0, 3)], [[0, 2, 1], [0, 3, 2]], floor)
+    # the pane in the plane z = 0, normal -z (toward the camera)
+    b.mesh([(-1, 0.2, 0), (1, 0.2, 0), (1, 1.8, 0), (-1, 1.8, 0)], [[0, 2, 1], [0, 3, 2]], glass)
+    _light_quad_facing(b, [(-0.5, 0.6, 1.5), (0.5, 0.6, 1.5), (0.5, 1.4, 1.5), (-0.5, 1.4, 1.5)],
+                       (0, 0, -1), (20.0, 18.0, 14.0), blk)
+    cam = look_at((0.4, 1.1, -3.0), (0, 0.9, 0), (0, 1, 0)) @ scale_m(1, -1, 1)
+    return b.build(cam, 0.8, name=f"rough_pane_{roughness}")
 
 
 def textured_cornell(seed=0, texels="u8"):

@@ -54,20 +54,28 @@ enum {
   DGPU_MAT_NONE = 0,          /* no BSDF (e.g. the black Lambert dort puts on area lights) */
   DGPU_MAT_LAMBERT = 1,       /* lambert_material.cpp:5-35, two-sided */
   DGPU_MAT_GGX_CONDUCTOR = 2, /* GGX microfacet conductor (no dort counterpart) */
-  DGPU_MAT_DIELECTRIC = 3     /* dielectric_material.cpp:32-104, delta reflect/refract */
+  DGPU_MAT_DIELECTRIC = 3,    /* dielectric_material.cpp:32-104, delta reflect/refract */
+  DGPU_MAT_ROUGH_DIELECTRIC = 4 /* dort's RoughDielectricBxdf as dort runs it
+                                   (rough_dielectric_material.cpp:7-185): GGX of alpha =
+                                   roughness; sampling reflects with the Fresnel probability
+                                   and a transmission pick yields nothing; eval gives the
+                                   reflection or transmission term by hemisphere; its pdf is
+                                   0 (so MIS weights are dort's); roughness < 0.001 is the
+                                   smooth dielectric.  ior_*, reflect_tint, transmit_tint as
+                                   for the dielectric */
 };
 
 typedef struct {
   uint32_t type;           /* DGPU_MAT_* */
   int32_t albedo_tex;      /* Lambert: texture index (uv lookup) or -1 for constant albedo */
   float albedo[3];         /* Lambert albedo / conductor tint */
-  float roughness;         /* GGX alpha (no remapping, microfacet.cpp:10-12) */
+  float roughness;         /* GGX alpha (no remapping, microfacet.cpp:10-12): conductor, rough dielectric */
   float eta[3];            /* conductor complex IOR, real part (per RGB channel) */
   float k[3];              /* conductor complex IOR, imaginary part */
-  float ior_inside;        /* dielectric */
-  float ior_outside;       /* dielectric (dort's Lua default is 1, lua_material.cpp:118) */
-  float reflect_tint[3];   /* dielectric */
-  float transmit_tint[3];  /* dielectric */
+  float ior_inside;        /* dielectric, rough dielectric */
+  float ior_outside;       /* (dort's Lua default is 1, lua_material.cpp:118) */
+  float reflect_tint[3];
+  float transmit_tint[3];
 } dgpu_material;
 
 typedef struct {

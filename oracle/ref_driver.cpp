@@ -40,6 +40,7 @@
 #include "dort/path_renderer.hpp"
 #include "dort/pinhole_camera.hpp"
 #include "dort/primitive.hpp"
+#include "dort/rough_dielectric_material.hpp"
 #include "dort/random_sampler.hpp"
 #include "dort/rng.hpp"
 #include "dort/scene.hpp"
@@ -163,6 +164,11 @@ std::shared_ptr<Material> make_material(const dgpu_material& m, const dgpu_scene
       if (!tex) return nullptr;
       return std::make_shared<LambertMaterial>(tex);
     }
+    case DGPU_MAT_ROUGH_DIELECTRIC:
+      return std::make_shared<RoughDielectricMaterial>(const_texture(spec3(m.reflect_tint)),
+                                                       const_texture(spec3(m.transmit_tint)),
+                                                       const_texture(m.roughness), m.ior_inside, m.ior_outside,
+                                                       MicrofacetType::GGX);
     case DGPU_MAT_DIELECTRIC:
       return std::make_shared<DielectricMaterial>(const_texture(spec3(m.reflect_tint)),
                                                   const_texture(spec3(m.transmit_tint)),

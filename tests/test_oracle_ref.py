@@ -39,6 +39,9 @@ SCENES = {
     "glass_cornell": scenes.glass_cornell,
     "terrain_sky": terrain,
     "textured_cornell": scenes.textured_cornell,
+    "rough_glass_cornell": scenes.rough_glass_cornell,
+    "rough_glass_cornell_0.6": lambda: scenes.rough_glass_cornell(0.6),
+    "rough_pane": scenes.rough_pane,
     "textured_cornell_f32": lambda: scenes.textured_cornell(texels="f32"),
 }
 
@@ -72,6 +75,12 @@ RENDER_CASES = [
     ("glass_cornell", {"max_depth": 12}),
     ("textured_cornell", {}),
     ("textured_cornell_f32", {}),
+    ("rough_glass_cornell", {}),
+    ("rough_glass_cornell", {"direct_strategy": abi.DIRECT_BSDF}),
+    ("rough_glass_cornell", {"direct_strategy": abi.DIRECT_LIGHT}),
+    ("rough_glass_cornell_0.6", {"max_depth": 10}),
+    ("rough_pane", {}),
+    ("rough_pane", {"direct_strategy": abi.DIRECT_LIGHT}),
 ]
 
 
