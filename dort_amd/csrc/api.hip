@@ -133,18 +133,19 @@ struct dgpu_ctx {
   }
 
   // Workgroups of the strided shade kernel: enough to fill every CU.
-  uint32_t shade_blocks(bool batch, bool all) {
-    int& cached = batch ? shade_blocks_batch : (all ? shade_blocks_all : shade_blocks_single);
+  uint32_t shade_blocks(bool batch, bool all, bool rough) {
+    int& cached = shade_grid_cache[batch][all][rough];
     if (cached == 0) {
       int nb = 0;
-      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, shade_kernel_ptr(batch, all), SHADE_BLOCK, 0) != hipSuccess ||
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, shade_kernel_ptr(batch, all, rough), SHADE_BLOCK, 0) !=
+              hipSuccess ||
           nb <= 0)
         nb = 2;
       cached = nb * num_cus * 4;
     }
     return (uint32_t)cached;
   }
-  int shade_blocks_single = 0, shade_blocks_all = 0, shade_blocks_batch = 0;
+  int shade_grid_cache[2][2][2] = {};  // [batch][sample_all][rough]
 
   uint32_t grid_for(int which, const KernelCfg& c) {
     uint64_t key = ((uint64_t)which << 48) | ((uint64_t)c.width << 32) | ((uint64_t)c.lds << 16) | c.block;
@@ -419,6 +420,7 @@ dgpu_status dgpu_scene_create(dgpu_ctx* ctx, const dgpu_scene_desc* d, const dgp
   for (uint32_t m = 0; m < d->num_materials; ++m) mats[m] = to_dev_material(d->materials[m]);
   UP(mats.data(), mats.size(), S.mats);
   S.num_mats = d->num_materials;
+  for (const DevMaterial& m : mats) S.has_rough |= m.type == DGPU_MAT_ROUGH_DIELECTRIC ? 1u : 0u;
   s->mats_host = mats;
 
   // lights, in dort's add order: area lights, environment, infinite
@@ -779,7 +781,7 @@ dgpu_status dgpu_render(dgpu_scene* s, const dgpu_render_opts* o, float* film_rg
   uint32_t slots = o->sample_all_lights ? 2 * std::max(1u, s->S.num_lights) : 2;
   uint64_t tile = s->queue_tile ? s->queue_tile : ctx->default_tile(slots);
   uint32_t C = (uint32_t)std::min<uint64_t>(std::min<uint64_t>(total ? total : 1, tile), 0xFFFFFFFFull / slots);
-  const uint32_t shade_grid = ctx->shade_blocks(false, o->sample_all_lights != 0);
+  const uint32_t shade_grid = ctx->shade_blocks(false, o->sample_all_lights != 0, s->S.has_rough != 0);
 
   KernelCfg ce = s->cfg, cc = s->cfg;
   ce.grid = ctx->grid_for(2, ce);
@@ -792,6 +794,7 @@ dgpu_status dgpu_render(dgpu_scene* s, const dgpu_render_opts* o, float* film_rg
   P.min_depth = o->min_depth; P.max_depth = o->max_depth; P.rr_depth = o->rr_depth;
   P.only_direct = o->only_direct; P.sample_all = o->sample_all_lights; P.strategy = o->direct_strategy;
   P.slots = slots;
+  P.rough = s->S.has_rough;
   dgpu_rng_key(o->seed, o->scene_id, &P.k0, &P.k1);
   P.sample_begin = o->sample_begin;
   P.thr = (float4*)ctx->buf("thr", (size_t)C * 16, &st);
@@ -915,7 +918,7 @@ dgpu_status dgpu_render_light(dgpu_scene* s, const dgpu_render_opts* o, float* f
   const uint64_t total = (uint64_t)npix * (o->sample_end - o->sample_begin);
   uint64_t tile = s->queue_tile ? s->queue_tile : ctx->default_tile(2);
   uint32_t C = (uint32_t)std::min<uint64_t>(std::min<uint64_t>(total ? total : 1, tile), 0x7FFFFFFFull);
-  const uint32_t grid = ctx->shade_blocks(false, false);
+  const uint32_t grid = ctx->shade_blocks(false, false, false);
 
   KernelCfg ce = s->cfg, cc = s->cfg;
   ce.grid = ctx->grid_for(2, ce);
@@ -1045,12 +1048,16 @@ dgpu_status dgpu_scene_update_material(dgpu_scene* s, uint32_t material, const d
     return fail(DGPU_EINVAL, "texture %d out of range", m->albedo_tex);
   if (m->type == DGPU_MAT_GGX_CONDUCTOR && !(m->roughness > 0.f))
     return fail(DGPU_EINVAL, "GGX roughness must be > 0");
+  if (m->type == DGPU_MAT_ROUGH_DIELECTRIC && !(m->roughness >= 0.f && std::isfinite(m->roughness)))
+    return fail(DGPU_EINVAL, "rough dielectric roughness must be finite and >= 0");
   DevMaterial dm = to_dev_material(*m);
   dgpu_ctx* ctx = s->ctx;
   std::lock_guard<std::mutex> lk(ctx->mu);
   HIP_TRY(hipSetDevice(ctx->device));
   HIP_TRY(hipMemcpy((void*)(s->S.mats + material), &dm, sizeof dm, hipMemcpyHostToDevice));
   s->mats_host[material] = dm;
+  s->S.has_rough = 0;
+  for (const DevMaterial& h : s->mats_host) s->S.has_rough |= h.type == DGPU_MAT_ROUGH_DIELECTRIC ? 1u : 0u;
   return DGPU_OK;
 }
 
@@ -1267,7 +1274,9 @@ dgpu_status dgpu_render_batch(dgpu_ctx* ctx, const dgpu_batch_item* items, uint3
   if (chunks.empty()) return DGPU_OK;
   uint32_t cap = 0;
   for (const ChunkDesc& cd : chunks) cap = std::max(cap, cd.n_chunk);
-  const uint32_t shade_grid = ctx->shade_blocks(true, false);
+  uint32_t any_rough = 0;
+  for (uint32_t i = 0; i < n; ++i) any_rough |= items[i].scene->S.has_rough;
+  const uint32_t shade_grid = ctx->shade_blocks(true, false, any_rough != 0);
 
   KernelCfg ce = s0->cfg, cc = s0->cfg;
   ce.grid = ctx->grid_for(4, ce);
@@ -1279,6 +1288,7 @@ dgpu_status dgpu_render_batch(dgpu_ctx* ctx, const dgpu_batch_item* items, uint3
   P.min_depth = o0.min_depth; P.max_depth = o0.max_depth; P.rr_depth = o0.rr_depth;
   P.only_direct = o0.only_direct; P.sample_all = 0; P.strategy = o0.direct_strategy;
   P.slots = 2;
+  P.rough = any_rough;
   P.thr = (float4*)ctx->buf("thr", (size_t)cap * 16, &st);
   P.rad = (float4*)ctx->buf("rad", (size_t)cap * 16, &st);
   P.pend = (float4*)ctx->buf("pend", (size_t)cap * 2 * 16, &st);

@@ -146,6 +146,7 @@ struct DevScene {
   const float4* tex_data;
   const uint4* tex_info;     // (w, h, offset, mapping DGPU_TEXMAP_*)
   const float4* tex_xform;   // 2 per texture: rows 0, 1 of world_to_texture (DGPU_TEXMAP_XY)
+  uint32_t has_rough;        // some material is a rough dielectric (kernels with that lobe)
   uint32_t num_tex;
   // camera (pinhole_camera.cpp:6-11)
   float c2w[12];
@@ -372,7 +373,9 @@ DG_INLINE V3 rough_dielectric_sample(const Bsdf& b, V3 wf, uint32_t req, float u
 }
 
 // Bsdf::eval_f (bsdf.cpp:72-93); shading frame == geometric frame, so shading_to_geom = 1
-// except where |n.wi| == 0.
+// except where |n.wi| == 0.  ROUGH: the rough dielectric's branch is compiled in (kernels
+// for scenes without one leave it out: it raised the shade kernel's register spill).
+template <bool ROUGH = true>
 DG_INLINE V3 bsdf_eval(const Bsdf& b, V3 wi, V3 wo, uint32_t req) {
   V3 wil = b.to_local(wi), wol = b.to_local(wo);
   bool refl = wol.z * wil.z > 0.f;
@@ -383,7 +386,8 @@ DG_INLINE V3 bsdf_eval(const Bsdf& b, V3 wi, V3 wo, uint32_t req) {
   if (ds * ds == 0.f) return v3(0.f, 0.f, 0.f);  // shading_to_geom == 0
   if (b.kind == DGPU_MAT_LAMBERT) return b.albedo() * INV_PI_F;
   if (b.kind == DGPU_MAT_GGX_CONDUCTOR) return conductor_f(b, wil, wol);
-  if (b.kind == DGPU_MAT_ROUGH_DIELECTRIC) return rough_dielectric_f(b, wil, wol, (ereq & BSDF_REFLECTION) != 0u);
+  if (ROUGH && b.kind == DGPU_MAT_ROUGH_DIELECTRIC)
+    return rough_dielectric_f(b, wil, wol, (ereq & BSDF_REFLECTION) != 0u);
   return v3(0.f, 0.f, 0.f);
 }
 
@@ -398,7 +402,8 @@ DG_INLINE float bsdf_pdf(const Bsdf& b, V3 wi, V3 wo, uint32_t req) {
 // Bsdf::sample_light_f with one lobe (bsdf.cpp:95-192) and the lobes'
 // sample functions (lambert_material.cpp:10-19, dielectric_material.cpp:50-104).
 // NON_DELTA: the caller requests no delta lobe (NEE), so the dielectric branch is not compiled.
-template <bool NON_DELTA = false>
+// ROUGH: the rough dielectric's branch is compiled in (see bsdf_eval).
+template <bool NON_DELTA = false, bool ROUGH = true>
 DG_INLINE V3 bsdf_sample(const Bsdf& b, V3 wo, uint32_t req, float u, float v, float c,
                          V3& wi, float& pdf, uint32_t& flags) {
   pdf = 0.f; flags = 0;
@@ -432,7 +437,7 @@ DG_INLINE V3 bsdf_sample(const Bsdf& b, V3 wo, uint32_t req, float u, float v, f
     if (!(p > 0.f)) return zero;
     fl = BSDF_REFLECTION | BSDF_GLOSSY;
     f = conductor_f(b, wil, wol);
-  } else if (b.kind == DGPU_MAT_ROUGH_DIELECTRIC) {
+  } else if (ROUGH && b.kind == DGPU_MAT_ROUGH_DIELECTRIC) {
     f = rough_dielectric_sample(b, wol, req, u, v, c, wil, p, fl);
     if (p == 0.f) return zero;
   } else if (!NON_DELTA && b.kind == DGPU_MAT_DIELECTRIC) {

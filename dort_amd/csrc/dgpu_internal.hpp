@@ -77,6 +77,7 @@ struct WaveParams {
   uint32_t min_depth, max_depth, rr_depth;
   uint32_t only_direct, sample_all, strategy;
   uint32_t slots;          // pending NEE slots per path (2 per light considered)
+  uint32_t rough;          // some scene of the render has a rough dielectric (shade with that lobe)
   uint32_t k0, k1;         // RNG key
   uint32_t sample_begin;
   uint32_t n_chunk;        // paths in this chunk
@@ -122,7 +123,7 @@ bool variant_supported(uint32_t width, uint32_t lds, uint32_t block);
 // which: 0 closest-hit API, 1 any-hit API, 2 extend, 3 connect, 4/5 batched extend/connect
 // (the bounded API variants share the occupancy of 0/1)
 const void* trace_kernel_ptr(int which, uint32_t width, uint32_t lds, uint32_t block);
-const void* shade_kernel_ptr(bool batch, bool all);
+const void* shade_kernel_ptr(bool batch, bool all, bool rough);
 
 // bounded: the ray origins lie within the range the box padding covers (fast box test)
 hipError_t launch_trace_api(const DevScene& S, const KernelCfg& cfg, const dgpu_rays& rays,

@@ -450,6 +450,7 @@ __device__ __forceinline__ V3 light_tri_normal(const DevScene& S, uint32_t tri) 
 // estimate_direct_from_bsdf (path_renderer.cpp:149-182) for light L: samples the BSDF
 // with (u, v, c) and, if the direction reaches L, returns the shadow ray and the
 // contribution it adds when unoccluded.  use_light: MIS against light sampling.
+template <bool ROUGH = true>
 __device__ __forceinline__ NeeOut nee_from_bsdf(const DevScene& S, const Surface& g, V3 wo, const Bsdf& bsdf,
                                                 const DevLight& L, float scale, V3 thr, bool use_light,
                                                 float u, float v, float c) {
@@ -457,7 +458,7 @@ __device__ __forceinline__ NeeOut nee_from_bsdf(const DevScene& S, const Surface
   NeeOut nb;
   nb.emit = false;
   V3 wi; float pdf; uint32_t fl;
-  V3 f = bsdf_sample<true>(bsdf, wo, NON_DELTA, u, v, c, wi, pdf, fl);
+  V3 f = bsdf_sample<true, ROUGH>(bsdf, wo, NON_DELTA, u, v, c, wi, pdf, fl);
   if (pdf == 0.f || is_black(f)) return nb;
   V3 rad = v3(0.f, 0.f, 0.f);
   float tmax = INFINITY, lp = 0.f;
@@ -487,6 +488,7 @@ __device__ __forceinline__ NeeOut nee_from_bsdf(const DevScene& S, const Surface
 
 // estimate_direct_from_light (path_renderer.cpp:184-206) for light L with the light
 // sample (u, v).  use_bsdf: MIS against BSDF sampling.
+template <bool ROUGH = true>
 __device__ __forceinline__ NeeOut nee_from_light(const DevScene& S, const Surface& g, V3 wo, const Bsdf& bsdf,
                                                  const DevLight& L, float scale, V3 thr, bool use_bsdf,
                                                  float u, float v) {
@@ -496,7 +498,7 @@ __device__ __forceinline__ NeeOut nee_from_light(const DevScene& S, const Surfac
   V3 wi; float pdf;
   V3 rad = light_sample(S, L, g.p, g.eps, u, v, wi, pdf, nl);
   if (pdf == 0.f || is_black(rad)) return nl;
-  V3 f = bsdf_eval(bsdf, wi, wo, NON_DELTA);
+  V3 f = bsdf_eval<ROUGH>(bsdf, wi, wo, NON_DELTA);
   if (is_black(f)) return nl;
   float w = 1.f;
   if (use_bsdf) {
@@ -542,7 +544,7 @@ __device__ __forceinline__ void emit_shadow_batch(const WaveParams& P, const Nee
 // Every lane of a wave runs the same number of queue appends (inactive lanes
 // append nothing), so the ballot-based compaction sees the whole wave.
 // sh: the workgroup's LDS words for block_append.
-template <bool BATCH, bool ALL>
+template <bool BATCH, bool ALL, bool ROUGH>
 __device__ __forceinline__ void shade_body(const WaveParams& P, const DevScene& S, uint32_t bounce,
                                            bool active, uint32_t i, uint32_t p, uint32_t scene,
                                            uint32_t pixel, uint32_t sample, uint32_t k0, uint32_t k1,
@@ -633,7 +635,7 @@ __device__ __forceinline__ void shade_body(const WaveParams& P, const DevScene& 
     {
       NeeOut nb;
       nb.emit = false;
-      if (nee && use_bsdf) nb = nee_from_bsdf(S, g, wo, bsdf, L, scale, thr, use_light, rb0[1], rb0[2], rb0[3]);
+      if (nee && use_bsdf) nb = nee_from_bsdf<ROUGH>(S, g, wo, bsdf, L, scale, thr, use_light, rb0[1], rb0[2], rb0[3]);
       uint32_t qb = block_append(&P.counters[CNT_SHADOW + bounce], nb.emit, sh);
       if (nb.emit) {
         P.pend[slot_b] = make_float4(nb.contrib.x, nb.contrib.y, nb.contrib.z, 1.f);
@@ -648,7 +650,7 @@ __device__ __forceinline__ void shade_body(const WaveParams& P, const DevScene& 
       nl.emit = false;
       if (nee && use_light) {
         dgpu_rng_block(k0, k1, pixel, sample, bounce, 1, rb1);
-        nl = nee_from_light(S, g, wo, bsdf, L, scale, thr, use_bsdf, rb1[0], rb1[1]);
+        nl = nee_from_light<ROUGH>(S, g, wo, bsdf, L, scale, thr, use_bsdf, rb1[0], rb1[1]);
       }
       uint32_t ql = block_append(&P.counters[CNT_SHADOW + bounce], nl.emit, sh);
       if (nl.emit) {
@@ -669,7 +671,7 @@ __device__ __forceinline__ void shade_body(const WaveParams& P, const DevScene& 
     V3 wi; float pdf; uint32_t fl;
     dgpu_rng_block(k0, k1, pixel, sample, bounce, 1, rb1);
     dgpu_rng_block(k0, k1, pixel, sample, bounce, 2, rb2);
-    V3 f = bsdf_sample(bsdf, wo, req, rb1[2], rb1[3], rb2[0], wi, pdf, fl);
+    V3 f = bsdf_sample<false, ROUGH>(bsdf, wo, req, rb1[2], rb1[3], rb2[0], wi, pdf, fl);
     if (!is_black(f) && pdf != 0.f) {
       V3 contrib = f * (fabsf(dot(wi, g.nn)) / pdf);
       bool delta = (fl & BSDF_DELTA) != 0u;
@@ -708,7 +710,8 @@ __device__ __forceinline__ void shade_body(const WaveParams& P, const DevScene& 
 #define DGPU_SHADE_WAVES 4
 #endif
 // ALL: sample_all_lights (one NEE per light and vertex) instead of one light picked by power.
-template <bool BATCH, bool ALL>
+// ROUGH: with the rough dielectric's lobe (scenes that have one).
+template <bool BATCH, bool ALL, bool ROUGH>
 __global__ __launch_bounds__(SHADE_BLOCK) __attribute__((amdgpu_waves_per_eu(DGPU_SHADE_WAVES)))
 void k_shade(WaveParams P, uint32_t bounce) {
   __shared__ uint32_t sh[SHADE_BLOCK / 64 + 1];
@@ -746,14 +749,14 @@ void k_shade(WaveParams P, uint32_t bounce) {
       uint64_t local = seg_local[sg] + (p - seg_begin[sg]);
       uint32_t sample = E.sample_begin + (uint32_t)(local / E.npix);
       uint32_t pixel = (uint32_t)(local % E.npix);
-      shade_body<true, false>(P, *(const DevScene*)&E.S, bounce, active, i, p, scene, pixel, sample, E.k0, E.k1,
+      shade_body<true, false, ROUGH>(P, *(const DevScene*)&E.S, bounce, active, i, p, scene, pixel, sample, E.k0, E.k1,
                               sh);
     } else {
       uint32_t p = active ? P.qpath[cur][i] : 0u;
       uint64_t gidx = P.g0 + p;
       uint32_t sample = P.sample_begin + (uint32_t)(gidx / P.npix);
       uint32_t pixel = (uint32_t)(gidx % P.npix);
-      shade_body<false, ALL>(P, P.S, bounce, active, i, p, 0u, pixel, sample, P.k0, P.k1, sh);
+      shade_body<false, ALL, ROUGH>(P, P.S, bounce, active, i, p, 0u, pixel, sample, P.k0, P.k1, sh);
     }
   }
 }
@@ -1431,17 +1434,21 @@ hipError_t launch_raygen(const WaveParams& P, uint32_t capacity, bool batch, hip
   return hipGetLastError();
 }
 
-const void* shade_kernel_ptr(bool batch, bool all) {
-  if (batch) return (const void*)&k_shade<true, false>;
-  return all ? (const void*)&k_shade<false, true> : (const void*)&k_shade<false, false>;
+const void* shade_kernel_ptr(bool batch, bool all, bool rough) {
+  if (batch) return rough ? (const void*)&k_shade<true, false, true> : (const void*)&k_shade<true, false, false>;
+  if (all) return rough ? (const void*)&k_shade<false, true, true> : (const void*)&k_shade<false, true, false>;
+  return rough ? (const void*)&k_shade<false, false, true> : (const void*)&k_shade<false, false, false>;
 }
 
 hipError_t launch_shade(const WaveParams& P, uint32_t capacity, uint32_t bounce, bool batch, hipStream_t st,
                         uint32_t max_blocks) {
   uint32_t blocks = std::min((capacity + SHADE_BLOCK - 1u) / SHADE_BLOCK, std::max(1u, max_blocks));
-  if (batch) hipLaunchKernelGGL((k_shade<true, false>), dim3(blocks), dim3(SHADE_BLOCK), 0, st, P, bounce);
-  else if (P.sample_all) hipLaunchKernelGGL((k_shade<false, true>), dim3(blocks), dim3(SHADE_BLOCK), 0, st, P, bounce);
-  else hipLaunchKernelGGL((k_shade<false, false>), dim3(blocks), dim3(SHADE_BLOCK), 0, st, P, bounce);
+#define DG_SHADE(B, A, R) hipLaunchKernelGGL((k_shade<B, A, R>), dim3(blocks), dim3(SHADE_BLOCK), 0, st, P, bounce)
+  const bool rough = P.rough != 0u;
+  if (batch) { if (rough) DG_SHADE(true, false, true); else DG_SHADE(true, false, false); }
+  else if (P.sample_all) { if (rough) DG_SHADE(false, true, true); else DG_SHADE(false, true, false); }
+  else { if (rough) DG_SHADE(false, false, true); else DG_SHADE(false, false, false); }
+#undef DG_SHADE
   return hipGetLastError();
 }
 
