@@ -361,7 +361,11 @@ def dort_test_box(light_kind="area", open_box=False):
     kw = {}
     if light_kind == "area":
         lp = np.array([(343, 548.8, 227), (343, 548.8, 332), (213, 548.8, 332), (213, 548.8, 227)]) * s
-        lp[:, 1] -= 1 * s
+        # The block's translate(0, -s, 0) moves the light twice in dort: mesh{} applies the
+        # builder transform to the points (dsl.lua:148-153, lua_shape.cpp:171) and
+        # add_shape(triangle, light) places the shape under it again
+        # (lua_builder.cpp:327-346); DiffuseLight samples it under the same transform.
+        lp[:, 1] -= 2 * s
         first = b.mesh(lp, [[2, 1, 0], [3, 2, 0]], blk)
         b.area_lights += [(first, (100.0, 100.0, 100.0)), (first + 1, (100.0, 100.0, 100.0))]
     elif light_kind == "infi":

@@ -22,6 +22,8 @@ from dort_amd import abi
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_ref", "libdortref.so")
 GPU_LIB_PATH = os.path.join(HERE, "_ref", "libdortref_gpu.so")
+LUA_HOST_PATH = os.path.join(HERE, "_ref", "dort_lua")
+LUA_PRELUDE = os.path.join(HERE, "..", "dort_amd", "host", "dort_gpu.lua")
 REF_ROOT = "/root/reference"
 _lib = None
 _gpu_lib = None
@@ -35,14 +37,30 @@ def gpu_available():
     return os.path.exists(GPU_LIB_PATH)
 
 
+def lua_available():
+    return os.path.exists(LUA_HOST_PATH)
+
+
+def run_lua(script, *args, lua_dir=None, cwd=None, timeout=300):
+    """Runs `script` in oracle/_ref/dort_lua (dort's Lua front-end with the dort_gpu
+    module and its prelude loaded); lua_dir: dort's lua/ tree, for scripts that require
+    dort/dsl.  Returns the CompletedProcess (text output)."""
+    cmd = [LUA_HOST_PATH, "--prelude", LUA_PRELUDE]
+    if lua_dir:
+        cmd += ["--lua-dir", lua_dir]
+    cmd += ["--", script, *[str(a) for a in args]]
+    return subprocess.run(cmd, cwd=cwd, capture_output=True, text=True, timeout=timeout)
+
+
 def build():
     """Compiles dort's sources into oracle/_ref (only where the checkout exists), and
-    dort with dort_amd's GpuPathRenderer when libdortgpu.so has been built."""
+    dort with dort_amd's GpuPathRenderer and its Lua front-end (dort_lua) when
+    libdortgpu.so has been built."""
     if not os.path.isdir(os.path.join(REF_ROOT, "src", "dort")):
         return False
     targets = ["all"]
     if os.path.exists(os.path.join(HERE, "..", "dort_amd", "libdortgpu.so")):
-        targets.append("gpu")
+        targets += ["gpu", "lua"]
     subprocess.check_call(["make", "-s", "-j8", "-f", os.path.join(HERE, "ref.mk"), f"REF={REF_ROOT}", *targets])
     return True
 
