@@ -17,7 +17,7 @@ LIB = os.path.join(HERE, "libdortgpu.so")
 ARCH = os.environ.get("DGPU_ARCH", "gfx950")
 
 HIP_SOURCES = ["kernels.hip", "api.hip", "compare.hip"]
-CXX_SOURCES = ["bvh_build.cpp", "io.cpp"]
+CXX_SOURCES = ["bvh_build.cpp", "io.cpp", "status.cpp"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-Wno-unused-variable"]
 HIP_FLAGS = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -23,19 +23,6 @@
 
 using namespace dgpu;
 
-namespace {
-thread_local std::string g_err;
-}  // namespace
-
-dgpu_status dgpu::fail(dgpu_status st, const char* fmt, ...) {
-  char buf[512];
-  va_list ap;
-  va_start(ap, fmt);
-  vsnprintf(buf, sizeof buf, fmt, ap);
-  va_end(ap);
-  g_err = buf;
-  return st;
-}
 
 namespace {
 
@@ -259,8 +246,6 @@ DevMaterial to_dev_material(const dgpu_material& a) {
 extern "C" {
 
 uint32_t dgpu_abi_version(void) { return DGPU_ABI_VERSION; }
-
-const char* dgpu_last_error(void) { return g_err.c_str(); }
 
 dgpu_status dgpu_create(int device, dgpu_ctx** out) {
   if (!out) return fail(DGPU_EINVAL, "null output");

@@ -13,7 +13,9 @@ import numpy as np
 from dort_amd import abi
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboracle.so")
+# DORT_ORACLE_LIB names another build of the oracle (the sanitizer build of
+# oracle/Makefile's `san` target, run by tests/test_sanitizers.py)
+LIB_PATH = os.environ.get("DORT_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 _lib = None
 
 

@@ -244,7 +244,10 @@ struct Checker {
 
 }  // namespace
 
-int main() {
+// With the argument "threaded" only the threaded-build case runs (the thread-sanitizer
+// run of tests/test_sanitizers.py).
+int main(int argc, char** argv) {
+  const bool threaded_only = argc > 1 && std::strcmp(argv[1], "threaded") == 0;
   struct Case {
     const char* name;
     Soup s;
@@ -265,10 +268,14 @@ int main() {
   }
   same.n = 40;
   cases.push_back({"40 identical triangles", same});
+  // large enough that the top subtrees are built on threads of their own (PAR_MIN)
+  cases.push_back({"random 300k (threaded build)", random_soup(300000, 10.f, 0.01f, 10)});
   int n = 0;
   for (auto& c : cases) {
+    if (threaded_only && c.s.n <= 100000) continue;
     for (uint32_t width : {2u, 4u})
       for (uint32_t leaf : {1u, 4u, 8u, 16u}) {
+        if (c.s.n > 100000 && leaf != 4u) continue;
         BvhParams p;
         p.width = width;
         p.max_leaf_size = leaf;
