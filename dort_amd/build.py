@@ -16,7 +16,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libdortgpu.so")
 ARCH = os.environ.get("DGPU_ARCH", "gfx950")
 
-HIP_SOURCES = ["kernels.hip", "api.hip", "compare.hip"]
+HIP_SOURCES = ["kernels.hip", "api.hip", "compare.hip", "bdpt.hip"]
 CXX_SOURCES = ["bvh_build.cpp", "io.cpp", "status.cpp"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-Wno-unused-variable"]

@@ -433,6 +433,7 @@ dgpu_status dgpu_scene_create(dgpu_ctx* ctx, const dgpu_scene_desc* d, const dgp
     powers.push_back((area * al.radiance[0] + area * al.radiance[1] + area * al.radiance[2]) / 3.f);
   }
   S.env_light = -1;
+  S.inf_light = -1;
   if (d->env) {
     const dgpu_env_light& e = *d->env;
     uint32_t w = e.width, h = e.height;
@@ -516,9 +517,19 @@ dgpu_status dgpu_scene_create(dgpu_ctx* ctx, const dgpu_scene_desc* d, const dgp
     L.radiance = make_float4(d->infinite_radiance[0], d->infinite_radiance[1], d->infinite_radiance[2], 0.f);
     S.has_inf = 1;
     S.inf_rad = v3(d->infinite_radiance[0], d->infinite_radiance[1], d->infinite_radiance[2]);
+    S.inf_light = (int32_t)lights.size();
     lights.push_back(L);
     float a = PI_F * radius * radius;  // InfiniteLight::approximate_power
     powers.push_back((a * L.radiance.x + a * L.radiance.y + a * L.radiance.z) / 3.f);
+  }
+  // the background lights' own DiscreteDistrib1d (environment first, then infinite;
+  // discrete_distrib_1d.cpp:4-20)
+  {
+    float pe = S.env_light >= 0 ? powers[S.env_light] : 0.f;
+    float pi = S.inf_light >= 0 ? powers[S.inf_light] : 0.f;
+    float sum = pe + pi, inv = sum == 0.f ? 0.f : 1.f / sum;
+    S.bg_env_prob = S.env_light >= 0 ? pe * inv : 0.f;
+    S.bg_inf_prob = S.inf_light >= 0 ? (pe + pi) * inv - (S.env_light >= 0 ? pe * inv : 0.f) : 0.f;
   }
   // DiscreteDistrib1d over light powers (scene.cpp:20-28, discrete_distrib_1d.cpp:4-20)
   std::vector<float> cdf;
@@ -1043,6 +1054,105 @@ dgpu_status dgpu_scene_update_material(dgpu_scene* s, uint32_t material, const d
   s->mats_host[material] = dm;
   s->S.has_rough = 0;
   for (const DevMaterial& h : s->mats_host) s->S.has_rough |= h.type == DGPU_MAT_ROUGH_DIELECTRIC ? 1u : 0u;
+  return DGPU_OK;
+}
+
+dgpu_status dgpu_render_bdpt(dgpu_scene* s, const dgpu_render_opts* o, uint32_t use_t1_paths, float* film_rgb,
+                             float* film_w, float* film_splat_rgb, dgpu_stats* out_stats,
+                             const volatile int* cancel) {
+  if (!s || !o) return fail(DGPU_EINVAL, "null scene or options");
+  if (!film_rgb || !film_w || !film_splat_rgb) return fail(DGPU_EINVAL, "null film buffer");
+  if (o->x_res == 0 || o->y_res == 0) return fail(DGPU_EINVAL, "empty film");
+  if (o->sample_end < o->sample_begin) return fail(DGPU_EINVAL, "sample_end < sample_begin");
+  if (o->max_depth >= 30) return fail(DGPU_EINVAL, "max_depth must be < 30");
+  if (o->min_depth > o->max_depth) return fail(DGPU_EINVAL, "min_depth > max_depth");
+  if ((uint64_t)o->x_res * o->y_res > 0xFFFFFFFFull) return fail(DGPU_EINVAL, "film too large");
+  if (out_stats) memset(out_stats, 0, sizeof *out_stats);
+  dgpu_ctx* ctx = s->ctx;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HIP_TRY(hipSetDevice(ctx->device));
+  hipStream_t stream = ctx->stream;
+  dgpu_status st = DGPU_OK;
+  const uint32_t npix = o->x_res * o->y_res;
+  // BdptRenderer::render returns at once without lights (bdpt_renderer.cpp:11)
+  const uint64_t total = s->S.num_lights ? (uint64_t)npix * (o->sample_end - o->sample_begin) : 0;
+  const uint32_t nv = o->max_depth + 2u;
+  // per path: two subpaths of nv vertices, the staged sample, the traversal spill slice
+  const uint64_t per_path = 2ull * nv * BDPT_VERTEX_BYTES + 16ull + 4ull * s->stack_cap;
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
+  const uint64_t budget = std::min<uint64_t>(8ull << 30, free_b / 4);
+  uint64_t cap = std::max<uint64_t>(1ull << 16, budget / per_path);
+  if (s->queue_tile) cap = s->queue_tile;
+  const uint32_t C = (uint32_t)std::min<uint64_t>(std::min<uint64_t>(total ? total : 1, cap), 0x7FFFFFC0ull);
+  const size_t threads = ((size_t)C + 63) / 64 * 64;
+  WaveParams P;
+  memset(&P, 0, sizeof P);
+  P.S = s->S;
+  P.xr = o->x_res; P.yr = o->y_res; P.npix = npix;
+  P.min_depth = o->min_depth; P.max_depth = o->max_depth;
+  dgpu_rng_key(o->seed, o->scene_id, &P.k0, &P.k1);
+  P.sample_begin = o->sample_begin;
+  BdptArgsHost A;
+  memset(&A, 0, sizeof A);
+  A.n = C;
+  A.nv = nv;
+  A.use_t1 = use_t1_paths ? 1u : 0u;
+  A.lv = (float4*)ctx->buf("bd_lv", (size_t)C * nv * BDPT_VERTEX_BYTES, &st);
+  A.cv = (float4*)ctx->buf("bd_cv", (size_t)C * nv * BDPT_VERTEX_BYTES, &st);
+  P.rad = (float4*)ctx->buf("rad", (size_t)C * 16, &st);
+  P.stats = (unsigned long long*)ctx->buf("stats", STATS_WORDS * 8, &st);
+  P.spill = (uint32_t*)ctx->buf("bd_spill", threads * s->stack_cap * 4, &st);
+  P.stack_cap = s->stack_cap;
+  const bool dev = o->flags & DGPU_BUF_DEVICE;
+  P.film_rgb = dev ? film_rgb : (float*)ctx->buf("film_rgb", (size_t)npix * 12, &st);
+  P.film_w = dev ? film_w : (float*)ctx->buf("film_w", (size_t)npix * 4, &st);
+  A.splat = dev ? film_splat_rgb : (float*)ctx->buf("film_splat", (size_t)npix * 12, &st);
+  if (st != DGPU_OK) return st;
+  HIP_TRY(hipMemsetAsync(P.stats, 0, STATS_WORDS * 8, stream));
+  if (!dev) {
+    HIP_TRY(hipMemsetAsync(P.film_rgb, 0, (size_t)npix * 12, stream));
+    HIP_TRY(hipMemsetAsync(P.film_w, 0, (size_t)npix * 4, stream));
+    HIP_TRY(hipMemsetAsync(A.splat, 0, (size_t)npix * 12, stream));
+  }
+  HIP_TRY(hipEventRecord(ctx->ev0, stream));
+  bool cancelled = false;
+  for (uint64_t g0 = 0; g0 < total; g0 += C) {
+    if (cancel && *cancel) { cancelled = true; break; }
+    P.g0 = g0;
+    P.n_chunk = (uint32_t)std::min<uint64_t>(C, total - g0);
+    HIP_TRY(launch_bdpt(P, A, s->S.width, stream));
+  }
+  HIP_TRY(hipEventRecord(ctx->ev1, stream));
+  unsigned long long hstats[STATS_WORDS] = {};
+  HIP_TRY(hipMemcpyAsync(hstats, P.stats, sizeof hstats, hipMemcpyDeviceToHost, stream));
+  std::vector<float> hrgb, hw, hsp;
+  if (!dev && !cancelled) {
+    hrgb.resize((size_t)npix * 3);
+    hw.resize(npix);
+    hsp.resize((size_t)npix * 3);
+    HIP_TRY(hipMemcpyAsync(hrgb.data(), P.film_rgb, (size_t)npix * 12, hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipMemcpyAsync(hw.data(), P.film_w, (size_t)npix * 4, hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipMemcpyAsync(hsp.data(), A.splat, (size_t)npix * 12, hipMemcpyDeviceToHost, stream));
+  }
+  HIP_TRY(hipStreamSynchronize(stream));
+  if (cancelled) return fail(DGPU_ECANCELLED, "render cancelled");
+  if (!dev) {
+    for (size_t k = 0; k < (size_t)npix * 3; ++k) {
+      film_rgb[k] += hrgb[k];
+      film_splat_rgb[k] += hsp[k];
+    }
+    for (size_t k = 0; k < npix; ++k) film_w[k] += hw[k];
+  }
+  if (out_stats) {
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    out_stats->paths = hstats[0];
+    out_stats->extension_rays = hstats[1];
+    out_stats->shadow_rays = hstats[2];
+    out_stats->dropped_samples = hstats[3];
+    out_stats->kernel_ms = ms;
+  }
   return DGPU_OK;
 }
 

@@ -142,6 +142,11 @@ struct DevScene {
   V3 env_up, env_s, env_t;
   int32_t has_inf;
   V3 inf_rad;
+  // background lights (environment, infinite) picked by power among themselves, as
+  // compute_light_distrib(scene, true) does (bdpt_renderer.cpp:13): the infinite light's
+  // index (or -1) and the two lights' probabilities in that pick
+  int32_t inf_light;
+  float bg_env_prob, bg_inf_prob;
   // textures
   const float4* tex_data;
   const uint4* tex_info;     // (w, h, offset, mapping DGPU_TEXMAP_*)

@@ -119,6 +119,17 @@ struct AdjArgs {
 // Paths per adjoint launch (bounds the traversal spill area: one slice per path).
 constexpr uint32_t ADJ_CHUNK = 1u << 20;
 
+// Bidirectional path tracing (bdpt.hip): the subpaths' vertex arrays of one chunk.
+struct BdptArgsHost {
+  float4* lv;        // light subpaths: nv vertices x 4 float4 x n paths
+  float4* cv;        // camera subpaths, likewise
+  uint32_t n;        // paths per chunk (the arrays' stride)
+  uint32_t nv;       // vertex slots per subpath (max_depth + 2)
+  uint32_t use_t1;
+  float* splat;      // splat sums, npix * 3
+};
+constexpr uint32_t BDPT_VERTEX_BYTES = 64;
+
 bool variant_supported(uint32_t width, uint32_t lds, uint32_t block);
 // which: 0 closest-hit API, 1 any-hit API, 2 extend, 3 connect, 4/5 batched extend/connect
 // (the bounded API variants share the occupancy of 0/1)
@@ -130,6 +141,8 @@ hipError_t launch_trace_api(const DevScene& S, const KernelCfg& cfg, const dgpu_
                             const dgpu_hits* hits, uint32_t* occluded, uint32_t n, uint32_t* fetch,
                             uint32_t* spill, bool bounded, hipStream_t st);
 hipError_t launch_raygen(const WaveParams& P, uint32_t capacity, bool batch, hipStream_t st);
+// k_bdpt over the chunk (P.n_chunk paths) and the film update of its t >= 2 strategies
+hipError_t launch_bdpt(const WaveParams& P, const BdptArgsHost& A, uint32_t width, hipStream_t st);
 hipError_t launch_extend(const WaveParams& P, const KernelCfg& cfg, uint32_t bounce, bool batch,
                          hipStream_t st);
 // max_blocks: the shade grid (workgroups of 256 that stride over the queue)

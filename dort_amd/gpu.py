@@ -20,7 +20,7 @@ EXPORTS = ["dgpu_abi_version", "dgpu_last_error", "dgpu_create", "dgpu_destroy",
            "dgpu_device_free", "dgpu_copy", "dgpu_scene_create", "dgpu_scene_destroy",
            "dgpu_scene_get_info", "dgpu_scene_update_texture", "dgpu_scene_update_material",
            "dgpu_trace_closest", "dgpu_trace_any", "dgpu_trace_closest_noalloc", "dgpu_render", "dgpu_render_light", "dgpu_render_batch",
-           "dgpu_adjoint",
+           "dgpu_render_bdpt", "dgpu_adjoint",
            "dgpu_read_ply", "dgpu_free", "dgpu_read_rgbe", "dgpu_write_rgbe", "dgpu_read_png", "dgpu_write_png",
            "dgpu_test_convergence", "dgpu_bias_variance"]
 
@@ -60,6 +60,8 @@ def lib():
                                         C.POINTER(abi.RenderStats), C.c_void_p]
         L.dgpu_render_batch.argtypes = [C.c_void_p, C.POINTER(abi.BatchItem), C.c_uint32, C.c_uint32,
                                         C.POINTER(abi.RenderStats)]
+        L.dgpu_render_bdpt.argtypes = [C.c_void_p, C.POINTER(abi.RenderOpts), C.c_uint32, C.c_void_p,
+                                       C.c_void_p, C.c_void_p, C.POINTER(abi.RenderStats), C.c_void_p]
         L.dgpu_scene_update_texture.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p]
         L.dgpu_scene_update_material.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(abi.Material)]
         L.dgpu_adjoint.argtypes = [C.c_void_p, C.POINTER(abi.RenderOpts), C.c_void_p, C.c_void_p,
@@ -86,7 +88,7 @@ def lib():
         for n in ("dgpu_create", "dgpu_device_alloc", "dgpu_device_free", "dgpu_copy",
                   "dgpu_scene_create", "dgpu_scene_get_info", "dgpu_scene_update_texture",
                   "dgpu_scene_update_material", "dgpu_trace_closest", "dgpu_trace_any", "dgpu_render",
-                  "dgpu_render_light",
+                  "dgpu_render_light", "dgpu_render_bdpt",
                   "dgpu_render_batch", "dgpu_adjoint"):
             getattr(L, n).restype = C.c_int
         _lib = L
@@ -353,6 +355,30 @@ class Scene:
         """The light tracer's image: splats / iterations (Film::to_image's splat_scale)."""
         rgb, _ = self.render_light(opts)
         return rgb / float(opts.sample_end - opts.sample_begin)
+
+    def render_bdpt(self, opts, use_t1_paths=True, films=None, device=False, cancel=None):
+        """dort's bidirectional path tracer: adds to films (rgb (H,W,3), weight (H,W),
+        splat (H,W,3)); returns (rgb, weight, splat, stats)."""
+        if films is None:
+            films = (np.zeros((opts.y_res, opts.x_res, 3), np.float32),
+                     np.zeros((opts.y_res, opts.x_res), np.float32),
+                     np.zeros((opts.y_res, opts.x_res, 3), np.float32))
+        rgb, w, splat = films
+        ptr = lambda a: a.data_ptr() if hasattr(a, "data_ptr") else a.ctypes.data
+        o = abi.RenderOpts.from_buffer_copy(opts)
+        if device:
+            o.flags |= abi.BUF_DEVICE
+        st = abi.RenderStats()
+        cptr = cancel.ctypes.data if cancel is not None else None
+        check(lib().dgpu_render_bdpt(self.h, C.byref(o), 1 if use_t1_paths else 0, ptr(rgb), ptr(w), ptr(splat),
+                                     C.byref(st), cptr))
+        return rgb, w, splat, abi.stats_dict(st)
+
+    def bdpt_image(self, opts, use_t1_paths=True):
+        """The bidirectional image as dort's film gives it: rgb / weight + splat / iterations."""
+        rgb, w, splat, _ = self.render_bdpt(opts, use_t1_paths)
+        img = np.where(w[..., None] > 0, rgb / np.maximum(w, 1e-30)[..., None], 0.0)
+        return (img + splat / float(opts.sample_end - opts.sample_begin)).astype(np.float32)
 
     def update_texture(self, tex, rgb):
         rgb = np.ascontiguousarray(rgb, dtype=np.float32)

@@ -288,6 +288,26 @@ dgpu_status dgpu_render(dgpu_scene* scene, const dgpu_render_opts* opts,
 dgpu_status dgpu_render_light(dgpu_scene* scene, const dgpu_render_opts* opts, float* film_splat_rgb,
                               dgpu_stats* out_stats, const volatile int* cancel);
 
+/* dort's bidirectional path tracer, BdptRenderer::render (src/dort/bdpt_renderer.cpp:9-749,
+ * the "bdpt" renderer of lua_render.cpp:151-160): per iteration and pixel one light
+ * subpath (a light picked by power) and one camera subpath through pixel + a uniform
+ * jitter, each of at most max_depth + 2 vertices with dort's Russian roulette (from the
+ * third vertex, survival min(mean throughput scale, 0.9)); every strategy of s light and
+ * t camera vertices with min_depth + 2 <= s + t <= max_depth + 2 is weighted with dort's
+ * balance heuristic.  Strategies with t >= 2 form the pixel's sample, added like
+ * dgpu_render's (film_rgb_sum += L, film_weight += 1; samples exactly on a pixel edge,
+ * and non-finite ones, are dropped); strategies with t = 1 (light subpath to the camera,
+ * only with use_t1_paths) are splatted into film_splat_rgb with the 1 px box filter.
+ * dort's image is film_rgb_sum / film_weight + film_splat_rgb / (total iterations).  The
+ * strategy s = t = 1 has weight 0 for area and background lights and is not formed.
+ * Uses x_res, y_res, sample_begin/sample_end, seed, scene_id, flags (DGPU_BUF_DEVICE:
+ * all three films are device memory), min_depth and max_depth (< 30); the other options
+ * are the path tracer's.  A scene without lights adds nothing (bdpt_renderer.cpp:11).
+ * Splats are float atomics, so their sums vary in the last bits from run to run. */
+dgpu_status dgpu_render_bdpt(dgpu_scene* scene, const dgpu_render_opts* opts, uint32_t use_t1_paths,
+                             float* film_rgb_sum, float* film_weight, float* film_splat_rgb,
+                             dgpu_stats* out_stats, const volatile int* cancel);
+
 /* ---- batched rendering (no dort counterpart) -------------------------------
  * Renders many scenes in one wavefront: every scene's paths go into shared queues,
  * the per-chunk bounce loop (raygen, extend/shade/connect per bounce, film) is

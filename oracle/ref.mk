@@ -27,7 +27,7 @@ SRCS := path_renderer renderer film bvh bvh_primitive mesh_bvh_primitive triangl
   pinhole_camera camera transform random_sampler sampler sampled_filter filter \
   stats thread_pool box geometry math image image_texture texture_maps \
   list_primitive mat triangle_shape triangle_shape_primitive mesh_triangle_primitive \
-  ply_mesh convergence_test light_renderer
+  ply_mesh convergence_test light_renderer bdpt_renderer
 OBJS := $(addprefix $(OUT)/obj/,$(addsuffix .o,$(SRCS))) $(OUT)/obj/rply.o $(OUT)/obj/ref_io_driver.o
 
 all: $(OUT)/libdortref.so

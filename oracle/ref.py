@@ -78,6 +78,9 @@ def lib():
         L.ref_render.argtypes = [C.c_void_p, C.POINTER(abi.RenderOpts), C.c_uint32, C.c_void_p, C.c_void_p]
         L.ref_render_light.restype = C.c_int
         L.ref_render_light.argtypes = [C.c_void_p, C.POINTER(abi.RenderOpts), C.c_uint32, C.c_void_p]
+        L.ref_render_bdpt.restype = C.c_int
+        L.ref_render_bdpt.argtypes = [C.c_void_p, C.POINTER(abi.RenderOpts), C.c_uint32, C.c_uint32, C.c_void_p,
+                                      C.c_void_p, C.c_void_p]
         L.ref_read_ply.argtypes = [C.c_char_p, C.POINTER(C.POINTER(C.c_float)), C.POINTER(C.c_uint32),
                                    C.POINTER(C.POINTER(C.c_uint32)), C.POINTER(C.c_uint32)]
         L.ref_free.argtypes = [C.c_void_p]
@@ -203,6 +206,22 @@ class RefScene:
         if lib().ref_render_light(self.h, C.byref(opts), threads, rgb.ctypes.data) != 0:
             raise RuntimeError("ref_render_light failed")
         return rgb
+
+    def render_bdpt(self, opts, use_t1_paths=True, threads=1):
+        """BdptRenderer::render (dort's `bdpt`) with RandomSampler(Rng(seed)): the film's
+        colour sums (H, W, 3), weights (H, W) and splat sums (H, W, 3)."""
+        rgb = np.zeros((opts.y_res, opts.x_res, 3), np.float32)
+        w = np.zeros((opts.y_res, opts.x_res), np.float32)
+        splat = np.zeros((opts.y_res, opts.x_res, 3), np.float32)
+        if lib().ref_render_bdpt(self.h, C.byref(opts), threads, 1 if use_t1_paths else 0, rgb.ctypes.data,
+                                 w.ctypes.data, splat.ctypes.data) != 0:
+            raise RuntimeError("ref_render_bdpt failed")
+        return rgb, w, splat
+
+    def bdpt_image(self, opts, use_t1_paths=True):
+        rgb, w, splat = self.render_bdpt(opts, use_t1_paths)
+        img = np.where(w[..., None] > 0, rgb / np.maximum(w, 1e-30)[..., None], 0.0)
+        return (img + splat / float(opts.sample_end - opts.sample_begin)).astype(np.float32)
 
 
 def gpu_lib():

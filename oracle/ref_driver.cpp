@@ -9,6 +9,7 @@
 //   ref_render         PathRenderer::render (path_renderer.cpp:11-21) into a
 //                      Film with the default 1 px box filter (lua_render.cpp:108-115)
 //   ref_render_light   LightRenderer::render (light_renderer.cpp:13-45), the splat sums
+//   ref_render_bdpt    BdptRenderer::render (bdpt_renderer.cpp:9-41): colour, weight and splat sums
 //
 // Albedo textures become dort's ImageTexture (8-bit texels) composed with uv_texture_map or
 // xy_texture_map, as lua_texture.cpp:346-388 composes them; float texels must be exact
@@ -34,6 +35,7 @@
 #include "dort/lambert_material.hpp"
 #include "dort/list_primitive.hpp"
 #include "dort/bvh_primitive.hpp"
+#include "dort/bdpt_renderer.hpp"
 #include "dort/mat.hpp"
 #include "dort/mesh.hpp"
 #include "dort/light_renderer.hpp"
@@ -336,6 +338,36 @@ int ref_render_light(void* p, const dgpu_render_opts* o, uint32_t threads, float
     rgb[3 * i] = sp.red();
     rgb[3 * i + 1] = sp.green();
     rgb[3 * i + 2] = sp.blue();
+  }
+  return 0;
+}
+
+// BdptRenderer::render (bdpt_renderer.cpp:9-41) with opts->sample_end iterations, depths
+// min_depth .. max_depth and use_t1_paths (lua_render.cpp:151-160).  Outputs the film's
+// per-pixel colour sums, weights and splat sums (the image is colour / weight +
+// splat / iterations).
+int ref_render_bdpt(void* p, const dgpu_render_opts* o, uint32_t threads, uint32_t use_t1, float* rgb, float* w,
+                    float* splat) {
+  const RefScene& rs = *(const RefScene*)p;
+  if (o->sample_begin != 0 || o->sample_end == 0) return 1;
+  CtxG ctx;
+  ctx.pool = make_pool(threads);
+  auto film = std::make_shared<Film>(o->x_res, o->y_res, std::make_shared<BoxFilter>(Vec2(0.5f, 0.5f)));
+  auto sampler = std::make_shared<RandomSampler>(1, Rng((uint32_t)o->seed));
+  BdptRenderer r(rs.scene, film, sampler, rs.camera, o->sample_end, o->min_depth, o->max_depth, use_t1 != 0, "");
+  NoProgress progress;
+  r.render(ctx, progress);
+  ctx.pool->stop();
+  for (size_t i = 0; i < film->pixels.size(); ++i) {
+    const Film::Pixel& px = film->pixels[i];
+    Spectrum sp = px.splat.load_relaxed();
+    rgb[3 * i] = px.color.red();
+    rgb[3 * i + 1] = px.color.green();
+    rgb[3 * i + 2] = px.color.blue();
+    w[i] = px.weight;
+    splat[3 * i] = sp.red();
+    splat[3 * i + 1] = sp.green();
+    splat[3 * i + 2] = sp.blue();
   }
   return 0;
 }
