@@ -8,13 +8,13 @@
 --    (lua_builder.cpp:104-710), the typed scene the GPU path takes: triangles of
 --    meshes, PLY meshes and cube() shapes, the materials on them, triangle area
 --    lights, an infinite light and the pinhole camera;
---  * adds the renderers "pt_gpu" and "lt_gpu" to dort.render.make, with the options
---    of "pt" and "lt" (lua_render.cpp:121-150), so dort.render.render_sync,
+--  * adds the renderers "pt_gpu", "lt_gpu" and "bdpt_gpu" to dort.render.make, with the
+--    options of "pt", "lt" and "bdpt" (lua_render.cpp:121-160), so dort.render.render_sync,
 --    get_image and the DSL's render() work on their jobs unchanged.
 --
 -- Recording never stops a script: what the GPU path cannot express (a phong or mirror
 -- material, a sphere, a point light, textures, frames) is noted, and making a
--- "pt_gpu" or "lt_gpu" job for that scene fails with the reason.
+-- GPU job for that scene fails with the reason.
 --
 -- The constructors are wrapped in their module tables, so this must run before
 -- dort/dsl is first required (the DSL copies some of them, dsl.lua:69-163).
@@ -328,7 +328,7 @@ local render_make = dort.render.make
 
 function dort.render.make(scene, params)
   local kind = params and params.renderer
-  if kind ~= "pt_gpu" and kind ~= "lt_gpu" then
+  if kind ~= "pt_gpu" and kind ~= "lt_gpu" and kind ~= "bdpt_gpu" then
     return render_make(scene, params)
   end
   local r = scenes[scene]

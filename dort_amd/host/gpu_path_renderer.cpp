@@ -298,4 +298,83 @@ void GpuLightRenderer::render(dort::CtxG&, dort::Progress& progress) {
   this->film->splat_scale = 1.f / float(std::max(1u, opts_.iterations));
 }
 
+GpuBdptRenderer::GpuBdptRenderer(std::shared_ptr<dort::Scene> scene, std::shared_ptr<dort::Film> film,
+                                 std::shared_ptr<dort::Sampler> sampler, std::shared_ptr<dort::Camera> camera,
+                                 std::shared_ptr<GpuSceneDesc> gpu_scene, GpuBdptOptions opts)
+    : dort::Renderer(std::move(scene), std::move(film), std::move(sampler), std::move(camera)),
+      gpu_scene_(std::move(gpu_scene)),
+      opts_(opts) {
+  if (!gpu_scene_) throw std::invalid_argument("GpuBdptRenderer: no GPU scene description");
+  if (dgpu_abi_version() != DGPU_ABI_VERSION)
+    throw std::runtime_error("GpuBdptRenderer: libdortgpu.so has another ABI version than dortgpu.h");
+  if (opts_.iterations == 0) throw std::invalid_argument("GpuBdptRenderer: iterations must be positive");
+  // samples and splats land with the 1-pixel box filter only
+  const dort::SampledFilter& f = this->film->filter;
+  if (f.radius.x != 0.5f || f.radius.y != 0.5f || f.evaluate(dort::Vec2(0.f, 0.f)) != 1.f ||
+      f.evaluate(dort::Vec2(0.49f, 0.49f)) != 1.f)
+    throw std::invalid_argument("GpuBdptRenderer: only the 1-pixel box filter is supported");
+  int st = dgpu_create(opts_.device, &ctx_);
+  if (st != DGPU_OK) fail("dgpu_create", st);
+}
+
+GpuBdptRenderer::~GpuBdptRenderer() {
+  if (dev_scene_) dgpu_scene_destroy(dev_scene_);
+  if (ctx_) dgpu_destroy(ctx_);
+}
+
+void GpuBdptRenderer::fail(const char* what, int status) const {
+  throw std::runtime_error(std::string("GpuBdptRenderer: ") + what + " failed (status " +
+                           std::to_string(status) + "): " + dgpu_last_error());
+}
+
+void GpuBdptRenderer::render(dort::CtxG&, dort::Progress& progress) {
+  if (!dev_scene_) {
+    const dgpu_scene_desc& d = gpu_scene_->desc();
+    int st = dgpu_scene_create(ctx_, &d, &opts_.bvh, &dev_scene_);
+    if (st != DGPU_OK) fail("dgpu_scene_create", st);
+  }
+  {
+    std::unique_lock<std::mutex> lock(this->sampler_mutex);
+    uint64_t hi = this->sampler->rng.uniform_uint32(0xFFFFFFFFu);
+    uint64_t lo = this->sampler->rng.uniform_uint32(0xFFFFFFFFu);
+    seed_ = (hi << 32) | lo;
+  }
+  const uint32_t xr = (uint32_t)this->film->res.x, yr = (uint32_t)this->film->res.y;
+  const size_t npix = size_t(xr) * yr;
+  std::vector<float> rgb(npix * 3, 0.f), w(npix, 0.f), splat(npix * 3, 0.f);
+  dgpu_render_opts o{};
+  o.x_res = xr;
+  o.y_res = yr;
+  o.seed = seed_;
+  o.min_depth = opts_.min_depth;
+  o.max_depth = opts_.max_depth;
+  const uint32_t chunk = opts_.iterations_per_chunk
+                             ? opts_.iterations_per_chunk
+                             : std::max(1u, (opts_.iterations + 19u) / 20u);
+  stats_ = dgpu_stats{};
+  for (uint32_t begin = 0; begin < opts_.iterations; begin += chunk) {
+    if (progress.is_cancelled()) break;
+    o.sample_begin = begin;
+    o.sample_end = std::min(opts_.iterations, begin + chunk);
+    dgpu_stats st{};
+    int rc = dgpu_render_bdpt(dev_scene_, &o, opts_.use_t1_paths ? 1u : 0u, rgb.data(), w.data(), splat.data(),
+                              &st, nullptr);
+    if (rc != DGPU_OK) fail("dgpu_render_bdpt", rc);
+    stats_.paths += st.paths;
+    stats_.extension_rays += st.extension_rays;
+    stats_.shadow_rays += st.shadow_rays;
+    stats_.dropped_samples += st.dropped_samples;
+    stats_.kernel_ms += st.kernel_ms;
+    progress.set_percent_done(100.f * float(o.sample_end) / float(opts_.iterations));
+  }
+  std::unique_lock<std::mutex> film_lock(this->film_mutex);
+  for (size_t i = 0; i < npix; ++i) {
+    dort::Film::Pixel& px = this->film->pixels[i];
+    px.color = px.color + dort::Spectrum(rgb[3 * i], rgb[3 * i + 1], rgb[3 * i + 2]);
+    px.weight += w[i];
+    px.splat.add_relaxed(dort::Spectrum(splat[3 * i], splat[3 * i + 1], splat[3 * i + 2]));
+  }
+  this->film->splat_scale = 1.f / float(std::max(1u, opts_.iterations));
+}
+
 }  // namespace dort_amd

@@ -156,4 +156,45 @@ class GpuLightRenderer final : public dort::Renderer {
   uint64_t seed_ = 0;
 };
 
+// dort's "bdpt" renderer (BdptRenderer, include/dort/bdpt_renderer.hpp:14-40) backed by
+// dgpu_render_bdpt.  Options as lua_render.cpp:151-160 takes them (iterations, min_depth,
+// max_depth, use_t1_paths; debug_image_dir is not supported).  Camera-subpath samples go
+// to the film's colour sums and weights, light-subpath samples that reach the camera to
+// its splat sums; the film's splat_scale is 1 / iterations, as BdptRenderer::render
+// leaves it (bdpt_renderer.cpp:22-33).
+struct GpuBdptOptions {
+  uint32_t iterations = 1;
+  uint32_t min_depth = 0;
+  uint32_t max_depth = 5;
+  bool use_t1_paths = true;
+  dgpu_bvh_opts bvh = {};
+  int device = 0;
+  uint32_t iterations_per_chunk = 0;  // progress / cancellation granularity (0: about 20 chunks)
+};
+
+class GpuBdptRenderer final : public dort::Renderer {
+ public:
+  GpuBdptRenderer(std::shared_ptr<dort::Scene> scene, std::shared_ptr<dort::Film> film,
+                  std::shared_ptr<dort::Sampler> sampler, std::shared_ptr<dort::Camera> camera,
+                  std::shared_ptr<GpuSceneDesc> gpu_scene, GpuBdptOptions opts);
+  ~GpuBdptRenderer() override;
+  GpuBdptRenderer(const GpuBdptRenderer&) = delete;
+  GpuBdptRenderer& operator=(const GpuBdptRenderer&) = delete;
+
+  void render(dort::CtxG& ctx, dort::Progress& progress) override;
+
+  const dgpu_stats& last_stats() const { return stats_; }
+  uint64_t last_seed() const { return seed_; }
+
+ private:
+  [[noreturn]] void fail(const char* what, int status) const;
+
+  std::shared_ptr<GpuSceneDesc> gpu_scene_;
+  GpuBdptOptions opts_;
+  dgpu_ctx* ctx_ = nullptr;
+  dgpu_scene* dev_scene_ = nullptr;
+  dgpu_stats stats_{};
+  uint64_t seed_ = 0;
+};
+
 }  // namespace dort_amd

@@ -365,9 +365,10 @@ dgpu_bvh_opts read_bvh_opts(lua_State* l, int p) {
 // make_job(scene, recorder, params [, camera_transform, camera_fov]) -> RenderJob.
 // params as lua_render_make reads them (lua_render.cpp:104-150, 209): x_res (800),
 // y_res (600), filter (1-pixel box), sampler (random, split with seed 42), renderer
-// ("pt_gpu" or "lt_gpu"), iterations (1), camera (the scene's); "pt_gpu": min_depth (0),
-// max_depth (5), only_direct, sample_all_lights, direct_strategy ("mis"), and rr_depth
-// (2, dort's rule); "lt_gpu": min_depth, max_depth.  Both: device (0),
+// ("pt_gpu", "lt_gpu" or "bdpt_gpu"), iterations (1), camera (the scene's); "pt_gpu":
+// min_depth (0), max_depth (5), only_direct, sample_all_lights, direct_strategy ("mis"),
+// and rr_depth (2, dort's rule); "lt_gpu": min_depth, max_depth; "bdpt_gpu": min_depth,
+// max_depth, use_t1_paths (true, lua_render.cpp:151-160).  All: device (0),
 // iterations_per_chunk, bvh_width, bvh_max_leaf_size, bvh_sah_bins, stack_lds,
 // block_size, queue_tile.  Unknown params are an error, as in dort.  The GPU camera is
 // the recorded one, or camera_transform / camera_fov when given (the Lua side passes the
@@ -402,6 +403,7 @@ int lua_make_job(lua_State* l) {
 
   GpuPathOptions po;
   GpuLightOptions lo;
+  GpuBdptOptions bo;
   if (method == "pt_gpu") {
     po.iterations = iterations;
     po.min_depth = dort::lua_param_uint32_opt(l, p, "min_depth", 0);
@@ -424,6 +426,15 @@ int lua_make_job(lua_State* l) {
     po.bvh = read_bvh_opts(l, p);
     po.device = device;
     po.iterations_per_chunk = per_chunk;
+  } else if (method == "bdpt_gpu") {
+    // lua_render.cpp:151-160; debug_image_dir is not supported
+    bo.iterations = iterations;
+    bo.min_depth = dort::lua_param_uint32_opt(l, p, "min_depth", 0);
+    bo.max_depth = dort::lua_param_uint32_opt(l, p, "max_depth", 5);
+    bo.use_t1_paths = dort::lua_param_bool_opt(l, p, "use_t1_paths", true);
+    bo.bvh = read_bvh_opts(l, p);
+    bo.device = device;
+    bo.iterations_per_chunk = per_chunk;
   } else if (method == "lt_gpu") {
     lo.iterations = iterations;
     lo.min_depth = dort::lua_param_uint32_opt(l, p, "min_depth", 0);
@@ -432,7 +443,7 @@ int lua_make_job(lua_State* l) {
     lo.device = device;
     lo.iterations_per_chunk = per_chunk;
   } else {
-    lua_pushfstring(l, "dort_gpu: unknown renderer '%s' (pt_gpu, lt_gpu)", method.c_str());
+    lua_pushfstring(l, "dort_gpu: unknown renderer '%s' (pt_gpu, lt_gpu, bdpt_gpu)", method.c_str());
     method.clear();
     return lua_error(l);
   }
@@ -453,6 +464,8 @@ int lua_make_job(lua_State* l) {
       std::shared_ptr<dort::Renderer> renderer;
       if (method == "pt_gpu")
         renderer = std::make_shared<GpuPathRenderer>(scene, film, sampler, camera, gpu, po);
+      else if (method == "bdpt_gpu")
+        renderer = std::make_shared<GpuBdptRenderer>(scene, film, sampler, camera, gpu, bo);
       else
         renderer = std::make_shared<GpuLightRenderer>(scene, film, sampler, camera, gpu, lo);
       // the job exactly as lua_render_make assembles it (lua_render.cpp:211-219)
@@ -489,6 +502,8 @@ int lua_job_seed(lua_State* l) {
     seed = g->last_seed();
   } else if (auto g2 = std::dynamic_pointer_cast<GpuLightRenderer>(job->renderer)) {
     seed = g2->last_seed();
+  } else if (auto g3 = std::dynamic_pointer_cast<GpuBdptRenderer>(job->renderer)) {
+    seed = g3->last_seed();
   } else {
     job.reset();
     return luaL_error(l, "dort_gpu: not a GPU render job");
@@ -507,6 +522,8 @@ int lua_job_stats(lua_State* l) {
     st = g->last_stats();
   } else if (auto g2 = std::dynamic_pointer_cast<GpuLightRenderer>(job->renderer)) {
     st = g2->last_stats();
+  } else if (auto g3 = std::dynamic_pointer_cast<GpuBdptRenderer>(job->renderer)) {
+    st = g3->last_stats();
   } else {
     job.reset();
     return luaL_error(l, "dort_gpu: not a GPU render job");

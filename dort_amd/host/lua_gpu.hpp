@@ -9,14 +9,15 @@
 //     dort::Scene: meshes' triangles (with dort's own Transform::apply on the points),
 //     analytic cubes as 12 triangles, materials, area lights, the camera;
 //   * dort_gpu.make_job(scene, recorder, params [, camera_transform, fov]) -- a dort
-//     RenderJob (lua_render.hpp) whose renderer is GpuPathRenderer ("pt_gpu") or
-//     GpuLightRenderer ("lt_gpu"), reading the same params as lua_render_make's "pt" and
-//     "lt" branches (lua_render.cpp:104-150) plus rr_depth and the BVH / kernel knobs.
+//     RenderJob (lua_render.hpp) whose renderer is GpuPathRenderer ("pt_gpu"),
+//     GpuLightRenderer ("lt_gpu") or GpuBdptRenderer ("bdpt_gpu"), reading the same params
+//     as lua_render_make's "pt", "lt" and "bdpt" branches (lua_render.cpp:104-160) plus
+//     rr_depth and the BVH / kernel knobs.
 //     dort.render.render_sync / get_image then work on it unchanged.
 //
 // The Lua half (dort_gpu.lua, run once after dort's modules are registered) wraps the
 // dort.builder / material / shape / light / camera constructors so that every scene a
-// script builds is recorded, and adds "pt_gpu" / "lt_gpu" to dort.render.make.
+// script builds is recorded, and adds "pt_gpu", "lt_gpu" and "bdpt_gpu" to dort.render.make.
 #pragma once
 
 extern "C" {

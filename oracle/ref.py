@@ -235,6 +235,11 @@ def gpu_lib():
         L.refgpu_render.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(abi.RenderOpts), C.c_uint32, C.c_uint32,
                                     C.c_uint32, C.c_int, C.c_void_p, C.c_void_p, C.POINTER(C.c_uint64),
                                     C.POINTER(C.c_uint64), C.POINTER(C.c_float), C.c_char_p, C.c_size_t]
+        L.refgpu_render_bdpt.restype = C.c_int
+        L.refgpu_render_bdpt.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(abi.RenderOpts), C.c_uint32,
+                                         C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                         C.POINTER(C.c_float), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                         C.c_char_p, C.c_size_t]
         L.refgpu_render_light.restype = C.c_int
         L.refgpu_render_light.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(abi.RenderOpts), C.c_uint32,
                                           C.c_uint32, C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_uint64),
@@ -282,3 +287,19 @@ class GpuRendererInDort:
         if rc != 0:
             raise RuntimeError(err.value.decode())
         return rgb, {"splat_scale": scale.value, "seed": seed.value, "paths": paths.value}
+
+    def render_bdpt(self, opts, sampler_seed=42, chunk=0, use_t1_paths=True):
+        """Renderer::render of GpuBdptRenderer (dort's bdpt); returns (colour sums, weights,
+        splat sums, info dict)."""
+        rgb = np.zeros((opts.y_res, opts.x_res, 3), np.float32)
+        w = np.zeros((opts.y_res, opts.x_res), np.float32)
+        splat = np.zeros((opts.y_res, opts.x_res, 3), np.float32)
+        scale, seed, paths = C.c_float(), C.c_uint64(), C.c_uint64()
+        err = C.create_string_buffer(512)
+        rc = gpu_lib().refgpu_render_bdpt(self.h, C.addressof(self.packed.desc), C.byref(opts), sampler_seed, chunk,
+                                          1 if use_t1_paths else 0, rgb.ctypes.data, w.ctypes.data,
+                                          splat.ctypes.data, C.byref(scale), C.byref(seed), C.byref(paths), err,
+                                          len(err))
+        if rc != 0:
+            raise RuntimeError(err.value.decode())
+        return rgb, w, splat, {"seed": seed.value, "paths": paths.value, "splat_scale": scale.value}

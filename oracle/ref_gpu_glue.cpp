@@ -1,4 +1,4 @@
-// ref_gpu_glue.cpp -- drives dort_amd::GpuPathRenderer and GpuLightRenderer inside dort
+// ref_gpu_glue.cpp -- drives dort_amd::GpuPathRenderer, GpuLightRenderer and GpuBdptRenderer inside dort
 // (TEST INFRASTRUCTURE ONLY).
 //
 // Linked with dort's own objects and ref_driver.cpp into oracle/_ref/libdortref_gpu.so
@@ -138,6 +138,50 @@ int refgpu_render_light(void* ref_scene, const dgpu_scene_desc* d, const dgpu_re
       rgb[3 * i] = sp.red();
       rgb[3 * i + 1] = sp.green();
       rgb[3 * i + 2] = sp.blue();
+    }
+    *out_scale = film->splat_scale;
+    *out_seed = r.last_seed();
+    *out_paths = r.last_stats().paths;
+    return 0;
+  } catch (const std::exception& e) {
+    std::snprintf(err, err_len, "%s", e.what());
+    return 1;
+  }
+}
+
+// GpuBdptRenderer (dort's "bdpt") in the same seam: colour sums, weights, splat sums and
+// splat_scale of the film.
+int refgpu_render_bdpt(void* ref_scene, const dgpu_scene_desc* d, const dgpu_render_opts* o,
+                       uint32_t sampler_seed, uint32_t chunk, uint32_t use_t1, float* rgb, float* w, float* splat,
+                       float* out_scale, uint64_t* out_seed, uint64_t* out_paths, char* err, size_t err_len) {
+  try {
+    auto film = std::make_shared<dort::Film>(o->x_res, o->y_res,
+                                             std::make_shared<dort::BoxFilter>(dort::Vec2(0.5f, 0.5f)));
+    auto sampler = std::make_shared<dort::RandomSampler>(1, dort::Rng(sampler_seed));
+    dort_amd::GpuBdptOptions opts;
+    opts.iterations = o->sample_end;
+    opts.min_depth = o->min_depth;
+    opts.max_depth = o->max_depth;
+    opts.use_t1_paths = use_t1 != 0;
+    opts.iterations_per_chunk = chunk;
+    dort_amd::GpuBdptRenderer r(ref_scene_scene(ref_scene), film, sampler, ref_scene_camera(ref_scene),
+                                typed_scene(d), opts);
+    dort::CtxG ctx;
+    ctx.pool = std::make_shared<dort::ThreadPool>();
+    ctx.pool->start(1);
+    CountingProgress progress;
+    r.render(ctx, progress);
+    ctx.pool->stop();
+    for (size_t i = 0; i < film->pixels.size(); ++i) {
+      const dort::Film::Pixel& px = film->pixels[i];
+      dort::Spectrum sp = px.splat.load_relaxed();
+      rgb[3 * i] = px.color.red();
+      rgb[3 * i + 1] = px.color.green();
+      rgb[3 * i + 2] = px.color.blue();
+      w[i] = px.weight;
+      splat[3 * i] = sp.red();
+      splat[3 * i + 1] = sp.green();
+      splat[3 * i + 2] = sp.blue();
     }
     *out_scale = film->splat_scale;
     *out_seed = r.last_seed();

@@ -88,3 +88,21 @@ def test_light_renderer_seam_film_equals_c_abi_render(gpu_ctx):
     # the same light paths; splat sums differ only in the order of float atomics
     np.testing.assert_allclose(rgb, rgb2, rtol=1e-4, atol=1e-5)
     assert rgb.sum() > 0
+
+
+def test_bdpt_renderer_seam_film_equals_c_abi_render(gpu_ctx):
+    """GpuBdptRenderer (dort's bdpt) adds to the film's colour sums, weights and splat
+    sums what the C-ABI renders for its seed, and leaves splat_scale at 1 / iterations
+    as BdptRenderer::render does (bdpt_renderer.cpp:22-33)."""
+    sc = scenes.dort_test_box("area")
+    opts = abi.render_opts(64, 48, spp=12, seed=0, max_depth=3)
+    rgb, w, splat, info = ref.GpuRendererInDort(sc).render_bdpt(opts, sampler_seed=9, chunk=4)
+    assert info["paths"] == 64 * 48 * 12 and info["splat_scale"] == pytest.approx(1 / 12)
+    direct = abi.RenderOpts.from_buffer_copy(opts)
+    direct.seed = info["seed"]
+    rgb2, w2, splat2, _ = gpu_ctx.scene(sc).render_bdpt(direct)
+    np.testing.assert_array_equal(w, w2)
+    np.testing.assert_allclose(rgb, rgb2, rtol=1e-6, atol=1e-7)
+    # the same light subpaths; splat sums differ only in the order of float atomics
+    np.testing.assert_allclose(splat, splat2, rtol=1e-4, atol=1e-5)
+    assert rgb.sum() > 0 and splat.sum() > 0

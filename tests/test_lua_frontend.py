@@ -13,8 +13,9 @@ checkout):
 * scenes the GPU path cannot express are refused with the reason when a "pt_gpu" job is
   made; "pt" still renders with dort; without a GPU, a "pt_gpu" job fails loudly.
 
-GPU test: a "pt_gpu" job made and rendered through dort.render.make / render_sync /
-get_image gives the image the C-ABI renders for the same scene and seed.
+GPU test: a "pt_gpu", "lt_gpu" or "bdpt_gpu" job made and rendered through
+dort.render.make / render_sync / get_image gives the image the C-ABI renders for the same
+scene and seed.
 """
 import json
 import os
@@ -69,6 +70,29 @@ def _assert_records(d, sc):
     np.testing.assert_allclose(np.float32(d["camera"]["camera_to_world"]).reshape(3, 4),
                                np.asarray(sc.camera_to_world, np.float32), rtol=0, atol=5e-7)
     assert np.float32(d["camera"]["fov"]) == np.float32(sc.fov)
+
+
+def _scene_from_record(d):
+    """A scenes.Scene of a recorded GPU scene (write_json of the dort_gpu recorder)."""
+    pts = np.asarray(d["points"], np.float32).reshape(-1, 3)
+    mats = []
+    for m in d["materials"]:
+        mat = abi.Material()
+        mat.type = m["type"]
+        mat.albedo_tex = -1
+        mat.albedo = abi.f3(*m["albedo"])
+        mat.roughness = m["roughness"]
+        mat.ior_inside = m["ior_inside"]
+        mat.ior_outside = m["ior_outside"]
+        mat.reflect_tint = abi.f3(*m["reflect_tint"])
+        mat.transmit_tint = abi.f3(*m["transmit_tint"])
+        mats.append(mat)
+    return scenes.Scene(
+        positions=pts, indices=np.arange(len(pts), dtype=np.uint32).reshape(-1, 3),
+        tri_material=np.asarray(d["tri_material"], np.uint32), materials=mats,
+        area_lights=[(a["triangle"], tuple(a["radiance"])) for a in d["area_lights"]],
+        camera_to_world=np.float32(d["camera"]["camera_to_world"]).reshape(3, 4), fov=d["camera"]["fov"],
+        infinite=tuple(d["infinite"]) if d["infinite"] else None, name="lua_record")
 
 
 @needs_host
@@ -140,11 +164,11 @@ def test_pt_gpu_fails_loudly_without_a_gpu(tmp_path):
 
 @pytest.mark.gpu
 @needs_host
-@pytest.mark.parametrize("renderer", ["pt_gpu", "lt_gpu"])
+@pytest.mark.parametrize("renderer", ["pt_gpu", "lt_gpu", "bdpt_gpu"])
 def test_lua_job_renders_what_the_c_abi_renders(gpu_ctx, tmp_path, renderer):
-    """dort.render.make{renderer = "pt_gpu" | "lt_gpu"} -> render_sync -> get_image{hdr}
-    on the test-box scene built in Lua equals the C-ABI render of
-    scenes.dort_test_box("area") for the seed the job drew from its sampler."""
+    """dort.render.make{renderer = "pt_gpu" | "lt_gpu" | "bdpt_gpu"} -> render_sync -> get_image{hdr}
+    on the test-box scene built in Lua equals the C-ABI render of the scene it recorded
+    for the seed the job drew from its sampler."""
     x, y, spp, d0, d1 = 96, 64, 8, 0, 3
     npy, meta = tmp_path / "img.npy", tmp_path / "meta.json"
     print(f"lua {renderer}: {x}x{y}, {spp} iterations, depth {d0}..{d1}", flush=True)
@@ -152,14 +176,22 @@ def test_lua_job_renders_what_the_c_abi_renders(gpu_ctx, tmp_path, renderer):
     assert p.returncode == 0, p.stderr
     img, info = np.load(npy), json.loads(meta.read_text())
     opts = abi.render_opts(x, y, spp=spp, seed=int(info["seed"], 16), min_depth=d0, max_depth=d1)
-    gsc = gpu_ctx.scene(scenes.dort_test_box("area"))
+    # the scene exactly as the script recorded it (dort's float32 transforms place its points
+    # up to a few ulps from scenes.dort_test_box's)
+    rec = tmp_path / "scene.json"
+    q = ref.run_lua(BOX_SCRIPT, "record", rec)
+    assert q.returncode == 0, q.stderr
+    gsc = gpu_ctx.scene(_scene_from_record(json.loads(rec.read_text())))
     if renderer == "pt_gpu":
         assert info["paths"] == x * y * spp
         rgb, w, _ = gsc.render(opts)
         want = rgb / w[..., None]
-    else:
+    elif renderer == "lt_gpu":
         # LightRenderer's film: splat sums times 1 / iterations (light_renderer.cpp:38-44)
         want = gsc.render_light(opts)[0] / spp
+    else:
+        # BdptRenderer's film: colour / weight + splat sums / iterations
+        want = gsc.bdpt_image(opts)
     if renderer == "pt_gpu":
         # the job renders in chunks; per pixel the samples add up in the same order
         np.testing.assert_allclose(img, want, rtol=1e-5, atol=1e-6)
