@@ -113,6 +113,11 @@ __device__ __forceinline__ LtRay lt_light_ray(const DevScene& S, const DevLight&
     out.o = S.centroid + (-dir + off) * S.radius;
     out.d = dir;
   }
+  // The disk lies outside the scene's bounding sphere except at its centre; advance the
+  // origin along the ray to the sphere (|off| = dr): no geometry lies before it, and from
+  // there the trace kernels' fast box test holds, whose padding covers origins within
+  // the sphere but not the whole disk.
+  out.o = out.o + out.d * (S.radius - S.radius * sqrtf(fmaxf(0.f, 1.f - dr * dr)));
   out.nn = out.d;
   out.tmin = 0.f;
   out.pos_pdf = INV_PI_F / sqf(S.radius);
