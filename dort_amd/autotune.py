@@ -18,7 +18,7 @@ import time
 from . import abi
 from .gpu import DgpuError
 
-WIDTHS = (2, 4)
+WIDTHS = (2, 4, 8)
 LEAVES = (2, 4, 8)
 STACKS = (8, 16, 32)
 BLOCKS = (64, 128, 256)

@@ -690,7 +690,8 @@ hipError_t launch_bdpt(const WaveParams& P, const BdptArgsHost& H, uint32_t widt
   BdptArgs A;
   A.lv = H.lv; A.cv = H.cv; A.n = H.n; A.nv = H.nv; A.use_t1 = H.use_t1; A.splat = H.splat;
   uint32_t blocks = (P.n_chunk + BD_BLOCK - 1u) / BD_BLOCK;
-  if (width == 4) hipLaunchKernelGGL(k_bdpt<4>, dim3(blocks), dim3(BD_BLOCK), 0, st, P, A);
+  if (width == 8) hipLaunchKernelGGL(k_bdpt<8>, dim3(blocks), dim3(BD_BLOCK), 0, st, P, A);
+  else if (width == 4) hipLaunchKernelGGL(k_bdpt<4>, dim3(blocks), dim3(BD_BLOCK), 0, st, P, A);
   else hipLaunchKernelGGL(k_bdpt<2>, dim3(blocks), dim3(BD_BLOCK), 0, st, P, A);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;

@@ -7,7 +7,7 @@
 // GPU traversal stack (64 entries) can never overflow: below depth 32 every split
 // falls back to an object-median split, which halves the range.  The binary tree
 // is then flattened to BVH2 (child boxes stored in the parent) or collapsed to a
-// 4-wide BVH, and triangles are copied in leaf order as 3 x float4.
+// 4- or 8-wide BVH, and triangles are copied in leaf order as 3 x float4.
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -199,7 +199,7 @@ BuiltBvh build_bvh(const float* positions, uint32_t num_vertices, const uint32_t
   if (num_tris == 0) throw std::invalid_argument("scene has no triangles");
   if (num_tris >= (1u << 27)) throw std::invalid_argument("2^27 triangles or more");
   BuiltBvh out;
-  out.width = prm.width == 4 ? 4 : 2;
+  out.width = prm.width == 8 ? 8 : (prm.width == 4 ? 4 : 2);
   uint32_t max_leaf = std::min(std::max(prm.max_leaf_size, 1u), 16u);
   std::vector<float> cent(3 * (size_t)num_tris);
   std::vector<Aabb> pbox(num_tris);
@@ -277,18 +277,17 @@ BuiltBvh build_bvh(const float* positions, uint32_t num_vertices, const uint32_t
     out.num_nodes = (uint32_t)order.size();
     out.num_leaves = leaves;
   } else {
-    // collapse to 4-wide: open the child with the largest surface area until 4 children
-    struct W { int32_t src; std::vector<int32_t> kids; };
-    std::vector<W> wide;
-    std::vector<int32_t> todo{0};
+    // collapse to W-wide: open the child with the largest surface area until W children
+    const size_t W = out.width;
+    struct Wn { int32_t src; std::vector<int32_t> kids; };
+    std::vector<Wn> wide;
     std::vector<int32_t> wide_of(nodes.size(), -1);
-    // first pass: create wide nodes in BFS/DFS order
     std::vector<int32_t> st{0};
     while (!st.empty()) {
       int32_t c = st.back(); st.pop_back();
-      W w; w.src = c;
+      Wn w; w.src = c;
       std::vector<int32_t> kids{nodes[c].left, nodes[c].right};
-      while (kids.size() < 4) {
+      while (kids.size() < W) {
         int best = -1;
         float best_a = -1.f;
         for (size_t i = 0; i < kids.size(); ++i) {
@@ -308,23 +307,31 @@ BuiltBvh build_bvh(const float* positions, uint32_t num_vertices, const uint32_t
       for (int i = (int)kids.size() - 1; i >= 0; --i)
         if (nodes[kids[i]].left >= 0) st.push_back(kids[i]);
     }
-    // Compressed 64-byte nodes: the child boxes are quantized to 8 bits per plane in
-    // the frame of the node's own box (Ylitie, Karras, Laine 2017, "Efficient incoherent
-    // ray traversal on GPUs through compressed wide BVHs"), rounded outward, so that a
-    // visit loads 4 x 16 B instead of 7:
+    // Compressed nodes: the child boxes are quantized to 8 bits per plane in the frame
+    // of the node's own box (Ylitie, Karras, Laine 2017, "Efficient incoherent ray
+    // traversal on GPUs through compressed wide BVHs"), rounded outward.
+    // BVH4, 64 bytes (4 x 16 B):
     //   [0]  origin x, y, z (float), exponents ex | ey << 8 | ez << 16 (int8 each)
     //   [1]  lo.x[4] hi.x[4] lo.y[4] hi.y[4]   (one byte per child, 4 per word)
     //   [2]  lo.z[4] hi.z[4] child[0] child[1]
     //   [3]  child[2] child[3] 0 0
+    // BVH8, 96 bytes (6 x 16 B):
+    //   [0]  origin, exponents
+    //   [1]  lo.x[0..3] lo.x[4..7] hi.x[0..3] hi.x[4..7]
+    //   [2]  the same for y
+    //   [3]  the same for z
+    //   [4]  child[0..3]
+    //   [5]  child[4..7]
     // A plane is origin + q * 2^e.  The boxes are grown by box_pad before quantizing.
     // An empty slot has an inverted box (lo 255, hi 0) and empty_child_code.
-    out.nodes.assign(16 * wide.size(), 0.f);
+    const size_t NF = W == 8 ? 24 : 16;  // floats per node
+    out.nodes.assign(NF * wide.size(), 0.f);
     uint32_t leaves = 0;
     for (size_t i = 0; i < wide.size(); ++i) {
-      float* o = out.nodes.data() + 16 * i;
+      float* o = out.nodes.data() + NF * i;
       uint32_t* w = reinterpret_cast<uint32_t*>(o);
       const size_t nk = wide[i].kids.size();
-      double lo[4][3], hi[4][3], nlo[3] = {INFINITY, INFINITY, INFINITY}, nhi[3] = {-INFINITY, -INFINITY, -INFINITY};
+      double lo[8][3], hi[8][3], nlo[3] = {INFINITY, INFINITY, INFINITY}, nhi[3] = {-INFINITY, -INFINITY, -INFINITY};
       for (size_t c = 0; c < nk; ++c) {
         const BNode& n = nodes[wide[i].kids[c]];
         for (int a = 0; a < 3; ++a) {
@@ -347,10 +354,10 @@ BuiltBvh build_bvh(const float* positions, uint32_t num_vertices, const uint32_t
         exps |= (uint32_t)(uint8_t)(int8_t)e << (8 * a);
       }
       w[3] = exps;
-      uint32_t q[6] = {0, 0, 0, 0, 0, 0};  // lo.x hi.x lo.y hi.y lo.z hi.z, a byte per child
+      uint32_t ql_b[3][8], qh_b[3][8];  // quantized planes per axis and child
       const uint32_t empty = empty_child_code(num_tris);
-      uint32_t code[4] = {empty, empty, empty, empty};
-      for (size_t c = 0; c < 4; ++c) {
+      uint32_t code[8] = {empty, empty, empty, empty, empty, empty, empty, empty};
+      for (size_t c = 0; c < W; ++c) {
         for (int a = 0; a < 3; ++a) {
           uint32_t ql = 255, qh = 0;  // empty slot: inverted box
           if (c < nk) {
@@ -364,8 +371,8 @@ BuiltBvh build_bvh(const float* positions, uint32_t num_vertices, const uint32_t
             ql = (uint32_t)std::min(std::max(fl, 0.0), 255.0);
             qh = (uint32_t)std::min(std::max(ce, 0.0), 255.0);
           }
-          q[2 * a] |= ql << (8 * c);
-          q[2 * a + 1] |= qh << (8 * c);
+          ql_b[a][c] = ql;
+          qh_b[a][c] = qh;
         }
         if (c < nk) {
           int32_t k = wide[i].kids[c];
@@ -374,9 +381,20 @@ BuiltBvh build_bvh(const float* positions, uint32_t num_vertices, const uint32_t
           else code[c] = (uint32_t)wide_of[k];
         }
       }
-      w[4] = q[0]; w[5] = q[1]; w[6] = q[2]; w[7] = q[3];
-      w[8] = q[4]; w[9] = q[5]; w[10] = code[0]; w[11] = code[1];
-      w[12] = code[2]; w[13] = code[3]; w[14] = 0; w[15] = 0;
+      auto pack4 = [](const uint32_t* b) { return b[0] | (b[1] << 8) | (b[2] << 16) | (b[3] << 24); };
+      if (W == 4) {
+        w[4] = pack4(ql_b[0]); w[5] = pack4(qh_b[0]); w[6] = pack4(ql_b[1]); w[7] = pack4(qh_b[1]);
+        w[8] = pack4(ql_b[2]); w[9] = pack4(qh_b[2]); w[10] = code[0]; w[11] = code[1];
+        w[12] = code[2]; w[13] = code[3]; w[14] = 0; w[15] = 0;
+      } else {
+        for (int a = 0; a < 3; ++a) {
+          w[4 + 4 * a] = pack4(ql_b[a]);
+          w[5 + 4 * a] = pack4(ql_b[a] + 4);
+          w[6 + 4 * a] = pack4(qh_b[a]);
+          w[7 + 4 * a] = pack4(qh_b[a] + 4);
+        }
+        for (int c = 0; c < 8; ++c) w[16 + c] = code[c];
+      }
     }
     out.num_nodes = (uint32_t)wide.size();
     out.num_leaves = leaves;

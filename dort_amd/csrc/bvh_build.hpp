@@ -6,7 +6,7 @@
 namespace dgpu_host {
 
 struct BvhParams {
-  uint32_t width = 2;          // 2 or 4
+  uint32_t width = 2;          // 2, 4 or 8
   uint32_t max_leaf_size = 4;  // 1..16
   uint32_t sah_bins = 16;
   float node_cost = 1.f;       // SAH: cost of one node visit relative to one triangle test
@@ -14,8 +14,8 @@ struct BvhParams {
 };
 
 struct BuiltBvh {
-  std::vector<float> nodes;          // 16 floats (64 B) per node: BVH2, or compressed BVH4
-  std::vector<float> tris;           // 12 floats per triangle, leaf order (BVH4: + the empty-slot triangle)
+  std::vector<float> nodes;          // per node: 16 floats (64 B) for BVH2 and compressed BVH4, 24 (96 B) for BVH8
+  std::vector<float> tris;           // 12 floats per triangle, leaf order (BVH4/8: + the empty-slot triangle)
   std::vector<uint32_t> tri_leaf_of; // scene triangle id -> leaf-order slot
   uint32_t width = 2;
   uint32_t depth = 0;
@@ -29,7 +29,7 @@ inline uint32_t leaf_code(uint32_t first, uint32_t count) {
   return 0x80000000u | ((count - 1u) << 27) | first;
 }
 
-// Child code of an empty slot of a 4-wide node: a leaf holding only triangle slot
+// Child code of an empty slot of a wide node: a leaf holding only triangle slot
 // num_tris, a degenerate triangle (all vertices at the origin) that the builder
 // appends and that no ray hits.  The slot's box is inverted, so a ray visits it only
 // through rounding, and then harmlessly.

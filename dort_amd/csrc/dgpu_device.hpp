@@ -119,7 +119,7 @@ struct DevScene {
   // acceleration structure
   const float4* nodes;
   const float4* tris;        // leaf order, 3 per triangle
-  uint32_t width;            // 2 or 4
+  uint32_t width;            // 2, 4 or 8
   uint32_t num_tris;
   uint32_t stack_cap;        // traversal stack entries per lane (LDS + spill)
   // per-triangle attributes in leaf order (the leaf-order triangle record holds the

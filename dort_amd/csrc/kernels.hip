@@ -68,8 +68,13 @@ __device__ __forceinline__ bool trace_one(BvhRef b, const TraceRay& r, LaneStack
 
 // Trace kernels ask for 8 waves per SIMD (at most 64 VGPRs): traversal is bound by
 // instruction issue and memory latency, and the eighth wave measured 2-3% faster than
-// seven, also where a few registers spill.
-#define DG_TRACE_WAVES __attribute__((amdgpu_waves_per_eu(8)))
+// seven, also where a few registers spill.  BVH8 traversal keeps twice the boxes and a
+// longer sort in registers: at 8 waves it spilled 80 bytes a lane and ran 20% slower on
+// the 2M-triangle terrain than at 6 waves (80 VGPRs, no spill); 5 waves was no faster.
+#ifndef DGPU_TRACE_WAVES_W8
+#define DGPU_TRACE_WAVES_W8 6
+#endif
+#define DG_TRACE_WAVES(W) __attribute__((amdgpu_waves_per_eu((W) == 8 ? DGPU_TRACE_WAVES_W8 : 8)))
 
 // The wave's batch, traced 64 rays at a time: each part of the batch runs until its
 // longest ray ends.
@@ -194,7 +199,7 @@ struct ApiIO {
 };
 
 template <bool ANY, int WIDTH, int LDS, int BLOCK, bool FAST>
-__global__ __launch_bounds__(BLOCK) DG_TRACE_WAVES void k_trace_api(ApiIO io, uint32_t n, uint32_t* fetch,
+__global__ __launch_bounds__(BLOCK) DG_TRACE_WAVES(WIDTH) void k_trace_api(ApiIO io, uint32_t n, uint32_t* fetch,
                                                      uint32_t* spill, uint32_t stack_cap) {
   trace_loop<ANY, WIDTH, LDS, BLOCK, FAST>(io, n, fetch, spill, stack_cap, ANY ? DGPU_REFILL_ANY : 0u);
 }
@@ -294,7 +299,7 @@ __device__ __forceinline__ SegTable load_seg_table(const BatchEntry* batch, cons
 }
 
 template <int WIDTH, int LDS, int BLOCK, bool BATCH>
-__global__ __launch_bounds__(BLOCK) DG_TRACE_WAVES void k_extend(ExtendIO<BATCH> io, const uint32_t* count,
+__global__ __launch_bounds__(BLOCK) DG_TRACE_WAVES(WIDTH) void k_extend(ExtendIO<BATCH> io, const uint32_t* count,
                                                   uint32_t* fetch, uint32_t* spill, uint32_t stack_cap,
                                                   const BatchEntry* batch, const ChunkDesc* chunks,
                                                   const uint32_t* chunk_cur, uint32_t refill) {
@@ -305,7 +310,7 @@ __global__ __launch_bounds__(BLOCK) DG_TRACE_WAVES void k_extend(ExtendIO<BATCH>
 }
 
 template <int WIDTH, int LDS, int BLOCK, bool BATCH>
-__global__ __launch_bounds__(BLOCK) DG_TRACE_WAVES void k_connect(ConnectIO<BATCH> io, const uint32_t* count,
+__global__ __launch_bounds__(BLOCK) DG_TRACE_WAVES(WIDTH) void k_connect(ConnectIO<BATCH> io, const uint32_t* count,
                                                    uint32_t* fetch, uint32_t* spill, uint32_t stack_cap,
                                                    const BatchEntry* batch, const ChunkDesc* chunks,
                                                    const uint32_t* chunk_cur) {
@@ -1022,7 +1027,9 @@ __global__ __launch_bounds__(ADJ_BLOCK) void k_adjoint(WaveParams P, AdjArgs A) 
 #define DG_VARIANTS(X)                                                              \
   X(2, 8, 64) X(2, 16, 64) X(2, 32, 64) X(2, 8, 128) X(2, 16, 128) X(2, 32, 128)      \
   X(2, 8, 256) X(2, 16, 256) X(2, 32, 256) X(4, 8, 64) X(4, 16, 64) X(4, 32, 64)     \
-  X(4, 8, 128) X(4, 16, 128) X(4, 32, 128) X(4, 8, 256) X(4, 16, 256) X(4, 32, 256)
+  X(4, 8, 128) X(4, 16, 128) X(4, 32, 128) X(4, 8, 256) X(4, 16, 256) X(4, 32, 256)  \
+  X(8, 8, 64) X(8, 16, 64) X(8, 32, 64) X(8, 8, 128) X(8, 16, 128) X(8, 32, 128)      \
+  X(8, 8, 256) X(8, 16, 256) X(8, 32, 256)
 
 bool variant_supported(uint32_t width, uint32_t lds, uint32_t block) {
 #define X(W, L, B) if (width == W && lds == L && block == B) return true;
@@ -1359,7 +1366,8 @@ hipError_t launch_advance(uint32_t* cur, hipStream_t st) {
 
 hipError_t launch_adjoint(const WaveParams& P, const AdjArgs& A, uint32_t width, hipStream_t st) {
   uint32_t blocks = (P.n_chunk + ADJ_BLOCK - 1u) / ADJ_BLOCK;
-  if (width == 4) hipLaunchKernelGGL(k_adjoint<4>, dim3(blocks), dim3(ADJ_BLOCK), 0, st, P, A);
+  if (width == 8) hipLaunchKernelGGL(k_adjoint<8>, dim3(blocks), dim3(ADJ_BLOCK), 0, st, P, A);
+  else if (width == 4) hipLaunchKernelGGL(k_adjoint<4>, dim3(blocks), dim3(ADJ_BLOCK), 0, st, P, A);
   else hipLaunchKernelGGL(k_adjoint<2>, dim3(blocks), dim3(ADJ_BLOCK), 0, st, P, A);
   return hipGetLastError();
 }

@@ -271,6 +271,44 @@ DG_INLINE void decode_boxes(uint4 A, uint4 Q, uint4 R, const TravState& s, float
   d0 = d[0]; d1 = d[1]; d2 = d[2]; d3 = d[3];
 }
 
+// Entry distances of the eight children of a compressed BVH8 node (A: origin +
+// exponents; X, Y, Z: per axis lo[0..3] lo[4..7] hi[0..3] hi[4..7] bytes), +inf for a
+// child the ray misses; the planes as in decode_boxes.
+template <bool FAST>
+DG_INLINE void decode_boxes8(uint4 A, uint4 X, uint4 Y, uint4 Z, const TravState& s, float d[8]) {
+  V3 org = v3(__uint_as_float(A.x), __uint_as_float(A.y), __uint_as_float(A.z));
+  bool nx = s.inv.x < 0.f, ny = s.inv.y < 0.f, nz = s.inv.z < 0.f;
+  const uint32_t near_x[2] = {nx ? X.z : X.x, nx ? X.w : X.y}, far_x[2] = {nx ? X.x : X.z, nx ? X.y : X.w};
+  const uint32_t near_y[2] = {ny ? Y.z : Y.x, ny ? Y.w : Y.y}, far_y[2] = {ny ? Y.x : Y.z, ny ? Y.y : Y.w};
+  const uint32_t near_z[2] = {nz ? Z.z : Z.x, nz ? Z.w : Z.y}, far_z[2] = {nz ? Z.x : Z.z, nz ? Z.y : Z.w};
+  if (FAST) {
+    V3 si = v3(ldexpf(s.inv.x, exp_i8(A.w, 0)), ldexpf(s.inv.y, exp_i8(A.w, 1)), ldexpf(s.inv.z, exp_i8(A.w, 2)));
+    V3 of = v3(fmaf(org.x, s.inv.x, s.o.x), fmaf(org.y, s.inv.y, s.o.y), fmaf(org.z, s.inv.z, s.o.z));
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const int h = c >> 2, k = c & 3;
+      float tn = fmaxf(fmaxf(fmaf(byte_f(near_x[h], k), si.x, of.x), fmaf(byte_f(near_y[h], k), si.y, of.y)),
+                       fmaxf(fmaf(byte_f(near_z[h], k), si.z, of.z), s.tmin));
+      float tf = fminf(fminf(fmaf(byte_f(far_x[h], k), si.x, of.x), fmaf(byte_f(far_y[h], k), si.y, of.y)),
+                       fminf(fmaf(byte_f(far_z[h], k), si.z, of.z), s.tmax));
+      d[c] = tn <= tf ? tn : INFINITY;
+    }
+  } else {
+    V3 sc = v3(ldexpf(1.f, exp_i8(A.w, 0)), ldexpf(1.f, exp_i8(A.w, 1)), ldexpf(1.f, exp_i8(A.w, 2)));
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const int h = c >> 2, k = c & 3;
+      float tn = fmaxf(fmaxf((fmaf(byte_f(near_x[h], k), sc.x, org.x) - s.o.x) * s.inv.x,
+                             (fmaf(byte_f(near_y[h], k), sc.y, org.y) - s.o.y) * s.inv.y),
+                       fmaxf((fmaf(byte_f(near_z[h], k), sc.z, org.z) - s.o.z) * s.inv.z, s.tmin));
+      float tf = fminf(fminf((fmaf(byte_f(far_x[h], k), sc.x, org.x) - s.o.x) * s.inv.x,
+                             (fmaf(byte_f(far_y[h], k), sc.y, org.y) - s.o.y) * s.inv.y),
+                       fminf((fmaf(byte_f(far_z[h], k), sc.z, org.z) - s.o.z) * s.inv.z, s.tmax));
+      d[c] = tn <= tf * ROBUST_FAR ? tn : INFINITY;
+    }
+  }
+}
+
 // Orders (distance, child) pairs a and b by distance with selects, no branches.
 DG_INLINE void order_pair(float& da, uint32_t& ca, float& db, uint32_t& cb) {
   bool sw = db < da;
@@ -298,8 +336,47 @@ DG_INLINE float4 bvh_word(const float4* base, size_t i) {
 
 template <int WIDTH, bool ANY, bool FAST>
 DG_INLINE void trav_inner(const float4* __restrict__ nodes, TravState& s, LaneStack& st, uint4 A, uint4 Q,
-                          uint4 R, uint4 K) {
-  if (WIDTH == 4) {
+                          uint4 R, uint4 K, uint4 C0 = make_uint4(0, 0, 0, 0), uint4 C1 = make_uint4(0, 0, 0, 0)) {
+  if (WIDTH == 8) {
+    // compressed 96 B node: A origin, Q / R / K the x / y / z planes, C0 / C1 the child codes
+    float d[8];
+    decode_boxes8<FAST>(A, Q, R, K, s, d);
+    uint32_t c[8] = {C0.x, C0.y, C0.z, C0.w, C1.x, C1.y, C1.z, C1.w};
+    // any-hit rays: the nearest child by a tournament; closest-hit rays: a 19-comparator
+    // sorting network (Knuth), so the children are pushed far to near
+    order_pair(d[0], c[0], d[1], c[1]);
+    order_pair(d[2], c[2], d[3], c[3]);
+    order_pair(d[4], c[4], d[5], c[5]);
+    order_pair(d[6], c[6], d[7], c[7]);
+    order_pair(d[0], c[0], d[2], c[2]);
+    order_pair(d[4], c[4], d[6], c[6]);
+    order_pair(d[0], c[0], d[4], c[4]);
+    if (!ANY) {
+      // 15 more comparators complete the sort (22 in all; checked on every 0-1 input)
+      order_pair(d[1], c[1], d[3], c[3]);
+      order_pair(d[5], c[5], d[7], c[7]);
+      order_pair(d[1], c[1], d[5], c[5]);
+      order_pair(d[2], c[2], d[6], c[6]);
+      order_pair(d[3], c[3], d[7], c[7]);
+      order_pair(d[2], c[2], d[3], c[3]);
+      order_pair(d[4], c[4], d[5], c[5]);
+      order_pair(d[6], c[6], d[7], c[7]);
+      order_pair(d[2], c[2], d[4], c[4]);
+      order_pair(d[3], c[3], d[5], c[5]);
+      order_pair(d[1], c[1], d[4], c[4]);
+      order_pair(d[3], c[3], d[6], c[6]);
+      order_pair(d[1], c[1], d[2], c[2]);
+      order_pair(d[3], c[3], d[4], c[4]);
+      order_pair(d[5], c[5], d[6], c[6]);
+    }
+    if (d[0] != INFINITY) {
+#pragma unroll
+      for (int k = 7; k >= 1; --k)
+        if (d[k] != INFINITY) st.push(c[k]);
+      s.node = c[0];
+      return;
+    }
+  } else if (WIDTH == 4) {
     // compressed node (64 B): origin, per-axis scale 2^e, 8-bit child planes
     float d0, d1, d2, d3;
     decode_boxes<FAST>(A, Q, R, s, d0, d1, d2, d3);
@@ -366,7 +443,18 @@ DG_INLINE void trav_step(const float4* __restrict__ nodes, const float4* __restr
     return;
   }
   bool leaf = (s.node & LEAF_BIT) != 0u;
-  float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = a, c = a, k = a;
+  float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = a, c = a, k = a, e = a, f = a;
+  if (WIDTH == 8) {
+    // node: 6 x 16 B at 6 * node, the first three shared with a triangle's 3 x 16 B at
+    // 3 * first; the node's z planes and child codes (words 3-5) loaded alongside, the
+    // leaf lanes reading the root's
+    size_t idx = s.node & 0x07FFFFFFu;
+    const float4* p = (leaf ? tris : nodes) + (leaf ? 3 * idx : 6 * idx);
+    a = bvh_word(p, 0); b = bvh_word(p, 1); c = bvh_word(p, 2);
+    const size_t nb = leaf ? 0u : 6 * idx;
+    k = bvh_word(nodes, nb + 3u); e = bvh_word(nodes, nb + 4u); f = bvh_word(nodes, nb + 5u);
+    asm volatile("" : : "v"(a.w), "v"(b.w), "v"(c.w), "v"(k.x), "v"(k.y), "v"(e.x), "v"(f.x));
+  }
   if (WIDTH == 4) {
     // node: 4 x 16 B at 4 * node; triangle: 3 x 16 B at 3 * first (both indices < 2^27)
     size_t idx = s.node & 0x07FFFFFFu;
@@ -379,10 +467,11 @@ DG_INLINE void trav_step(const float4* __restrict__ nodes, const float4* __restr
     asm volatile("" : : "v"(a.w), "v"(b.w), "v"(c.w), "v"(k.x), "v"(k.y));
   }
   if (!leaf) {
-    trav_inner<WIDTH, ANY, FAST>(nodes, s, st, f4_bits(a), f4_bits(b), f4_bits(c), f4_bits(k));
+    trav_inner<WIDTH, ANY, FAST>(nodes, s, st, f4_bits(a), f4_bits(b), f4_bits(c), f4_bits(k), f4_bits(e),
+                                 f4_bits(f));
     return;
   }
-  if (WIDTH != 4) {
+  if (WIDTH == 2) {
     uint32_t ti = leaf_first(s.node);
     a = bvh_word(tris, 3 * (size_t)ti); b = bvh_word(tris, 3 * (size_t)ti + 1); c = bvh_word(tris, 3 * (size_t)ti + 2);
   }

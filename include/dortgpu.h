@@ -134,7 +134,7 @@ typedef struct {
 
 /* BVH / kernel knobs (the autotuner's search space). Zero fields take defaults. */
 typedef struct {
-  uint32_t width;          /* 2 or 4 (default 4) */
+  uint32_t width;          /* 2, 4 or 8 (default 4) */
   uint32_t max_leaf_size;  /* triangles per leaf, 1..8 (default 4) */
   uint32_t sah_bins;       /* binned-SAH bins (default 16) */
   uint32_t stack_lds;      /* traversal stack entries kept in LDS per lane (default 16) */

@@ -113,7 +113,7 @@ struct Checker {
       for (uint32_t k = 0; k < c; ++k) out.push_back(f + k);
       return;
     }
-    uint32_t ch[4];
+    uint32_t ch[8];
     int nc = children(code, ch, nullptr);
     for (int k = 0; k < nc; ++k) collect(ch[k], out, depth + 1);
   }
@@ -131,31 +131,42 @@ struct Checker {
       }
       return 2;
     }
-    const uint32_t* w = reinterpret_cast<const uint32_t*>(&b.nodes[16 * (size_t)node]);
+    const int W = (int)b.width;
+    const uint32_t* w = node_words(node);
     float org[3];
     std::memcpy(org, w, 12);
     int e[3] = {(int8_t)(w[3] & 0xFF), (int8_t)((w[3] >> 8) & 0xFF), (int8_t)((w[3] >> 16) & 0xFF)};
-    uint32_t code[4] = {w[10], w[11], w[12], w[13]};
     int n = 0;
-    for (int c = 0; c < 4; ++c) {
-      if (code[c] == empty_child_code(s.n)) {
+    for (int c = 0; c < W; ++c) {
+      uint32_t code = child_code(w, c);
+      if (code == empty_child_code(s.n)) {
         // an empty slot: inverted box on every axis
         for (int a = 0; a < 3; ++a)
-          if (((w[4 + 2 * a] >> (8 * c)) & 0xFF) != 255 || ((w[5 + 2 * a] >> (8 * c)) & 0xFF) != 0)
-            fail("empty slot without an inverted box");
+          if (plane(w, a, c, false) != 255 || plane(w, a, c, true) != 0) fail("empty slot without an inverted box");
         continue;
       }
-      ch[n] = code[c];
+      ch[n] = code;
       if (boxes)
         for (int a = 0; a < 3; ++a) {
-          boxes[n].lo[a] = org[a] + std::ldexp((double)((w[4 + 2 * a] >> (8 * c)) & 0xFF), e[a]);
-          boxes[n].hi[a] = org[a] + std::ldexp((double)((w[5 + 2 * a] >> (8 * c)) & 0xFF), e[a]);
+          boxes[n].lo[a] = org[a] + std::ldexp((double)plane(w, a, c, false), e[a]);
+          boxes[n].hi[a] = org[a] + std::ldexp((double)plane(w, a, c, true), e[a]);
         }
       ++n;
     }
-    if (n == 0) fail("BVH4 node without children");
+    if (n == 0) fail("wide node without children");
     return n;
   }
+
+  // the words of wide node `node` (16 floats per BVH4 node, 24 per BVH8 node)
+  const uint32_t* node_words(uint32_t node) const {
+    return reinterpret_cast<const uint32_t*>(&b.nodes[(b.width == 8 ? 24 : 16) * (size_t)node]);
+  }
+  // quantized plane of child c on axis a (hi: the upper plane)
+  uint32_t plane(const uint32_t* w, int a, int c, bool hi) const {
+    if (b.width == 4) return (w[4 + 2 * a + (hi ? 1 : 0)] >> (8 * c)) & 0xFFu;
+    return (w[4 + 4 * a + (hi ? 2 : 0) + (c >> 2)] >> (8 * (c & 3))) & 0xFFu;
+  }
+  uint32_t child_code(const uint32_t* w, int c) const { return b.width == 4 ? w[10 + c] : w[16 + c]; }
 
   void walk(uint32_t code, uint32_t depth) {
     if (!err.empty()) return;
@@ -171,24 +182,24 @@ struct Checker {
     }
     node_seen[code]++;
     max_depth = std::max(max_depth, depth);
-    uint32_t ch[4];
-    Box bx[4];
+    uint32_t ch[8];
+    Box bx[8];
     int n = children(code, ch, bx);
     for (int k = 0; k < n; ++k) {
-      check_box(bx[k], ch[k], b.width == 2 ? "BVH2" : "BVH4");
+      check_box(bx[k], ch[k], b.width == 2 ? "BVH2" : (b.width == 4 ? "BVH4" : "BVH8"));
       walk(ch[k], depth + 1);
     }
   }
 
-  // quantized BVH4 boxes: no looser than the padded box of the triangles below plus
-  // one quantum per side (the builder's outward rounding)
+  // quantized BVH4 / BVH8 boxes: no looser than the padded box of the triangles below
+  // plus one quantum per side (the builder's outward rounding)
   void check_tightness() {
-    if (b.width != 4) return;
+    if (b.width == 2) return;
     for (uint32_t node = 0; node < b.num_nodes && err.empty(); ++node) {
-      const uint32_t* w = reinterpret_cast<const uint32_t*>(&b.nodes[16 * (size_t)node]);
+      const uint32_t* w = node_words(node);
       int e[3] = {(int8_t)(w[3] & 0xFF), (int8_t)((w[3] >> 8) & 0xFF), (int8_t)((w[3] >> 16) & 0xFF)};
-      uint32_t ch[4];
-      Box bx[4];
+      uint32_t ch[8];
+      Box bx[8];
       int n = children(node, ch, bx);
       for (int k = 0; k < n; ++k) {
         std::vector<uint32_t> slots;
@@ -215,10 +226,10 @@ struct Checker {
     slot_seen.assign(s.n, 0);
     node_seen.assign(b.num_nodes, 0);
     // ids and vertices of the leaf-order records
-    // BVH4: one more record, the degenerate triangle that empty slots name
-    size_t records = s.n + (b.width == 4 ? 1 : 0);
+    // BVH4 / BVH8: one more record, the degenerate triangle that empty slots name
+    size_t records = s.n + (b.width != 2 ? 1 : 0);
     if (b.tris.size() != 12 * records || b.tri_leaf_of.size() != s.n) fail("triangle arrays have the wrong size");
-    if (b.width == 4 && err.empty())
+    if (b.width != 2 && err.empty())
       for (int v = 0; v < 3; ++v)
         for (int a = 0; a < 3; ++a)
           if (vert(s.n, v)[a] != 0.f) fail("the empty-slot triangle is not degenerate");
@@ -236,7 +247,7 @@ struct Checker {
       if (slot_seen[sl] != 1 && s.n > 1) fail("a triangle slot is not reached exactly once");
     for (uint32_t nd = 0; nd < b.num_nodes && err.empty(); ++nd)
       if (node_seen[nd] != 1) fail("a node is not reached exactly once");
-    if (b.width == 4 && err.empty() && max_depth != b.depth) fail("reported depth differs from the depth reached");
+    if (b.width != 2 && err.empty() && max_depth != b.depth) fail("reported depth differs from the depth reached");
     check_tightness();
     return err.empty();
   }
@@ -273,7 +284,7 @@ int main(int argc, char** argv) {
   int n = 0;
   for (auto& c : cases) {
     if (threaded_only && c.s.n <= 100000) continue;
-    for (uint32_t width : {2u, 4u})
+    for (uint32_t width : {2u, 4u, 8u})
       for (uint32_t leaf : {1u, 4u, 8u, 16u}) {
         if (c.s.n > 100000 && leaf != 4u) continue;
         BvhParams p;

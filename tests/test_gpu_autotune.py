@@ -17,11 +17,11 @@ def _parity(a, b):
 def test_autotune_one_scene(gpu_ctx):
     sc = scenes.terrain_scene(quads=60, seed=4, sky_res=(128, 64))
     opts = abi.render_opts(48, 48, spp=2, seed=3, max_depth=6, rr_depth=3)
-    best, table = autotune.autotune(gpu_ctx, sc, opts, shape_space=[(2, 4), (4, 2), (4, 8)],
+    best, table = autotune.autotune(gpu_ctx, sc, opts, shape_space=[(2, 4), (4, 2), (4, 8), (8, 4)],
                                     launch_space=[("stack_lds", (8, 32)), ("block_size", (64, 256)),
                                                   ("queue_tile", (1000,))])
     print("best", best, "of", [(k, round(ms, 3)) for k, ms in table])
-    assert len(table) == 9
+    assert len(table) == 10
     assert all(np.isfinite(ms) and ms > 0 for _, ms in table)
     assert best in [k for k, _ in table]
     assert min(ms for _, ms in table) == dict((str(k), ms) for k, ms in table)[str(best)]

@@ -8,7 +8,7 @@ distances within 1e-5 relative.
 import numpy as np
 import pytest
 
-from dort_amd import scenes
+from dort_amd import abi, scenes
 from oracle import oracle
 from tests.helpers import classify_hits
 
@@ -24,7 +24,7 @@ def sphere():
     return scenes.icosphere_scene(freq=32, seed=11)  # 20,480 triangles
 
 
-@pytest.mark.parametrize("width", [2, 4])
+@pytest.mark.parametrize("width", [2, 4, 8])
 def test_closest_hit_parity_icosphere(gpu_ctx, sphere, width):
     rays = _workload(sphere)
     ref = oracle.OracleScene(sphere).trace(rays)
@@ -37,7 +37,7 @@ def test_closest_hit_parity_icosphere(gpu_ctx, sphere, width):
     assert c["max_rel_t"] <= 1e-5
 
 
-@pytest.mark.parametrize("width", [2, 4])
+@pytest.mark.parametrize("width", [2, 4, 8])
 def test_any_hit_parity_icosphere(gpu_ctx, sphere, width):
     rays = _workload(sphere)
     # finite segments: half of each closest-hit distance, and twice
@@ -114,7 +114,7 @@ def _bounded_rays(scene, n, seed):
     return {k: np.concatenate([rays[k], r2[k]]) for k in rays}
 
 
-@pytest.mark.parametrize("width", [2, 4])
+@pytest.mark.parametrize("width", [2, 4, 8])
 def test_bounded_box_test_parity(gpu_ctx, sphere, width):
     """The padded FMA box test (DGPU_TRACE_BOUNDED, the path tracer's) against the oracle
     and against the exact box test on rays that start inside the scene's range."""
@@ -155,3 +155,19 @@ def test_device_buffers(gpu_ctx, sphere):
     occ_d = gpu_ctx.device_array(np.zeros(n, np.uint32))
     sc.trace_any(dev_rays, out=occ_d, device=True)
     assert np.array_equal(occ_d.numpy(), occ_h)
+
+
+def test_bvh_widths_render_the_same_paths(gpu_ctx):
+    """BVH2, BVH4 and BVH8 give the same hits, so the same paths: equal film weights and
+    radiance equal except where a path parts on an FP tie."""
+    sc = scenes.terrain_scene(quads=80, seed=3, sky_res=(128, 64))
+    opts = abi.render_opts(64, 64, spp=4, seed=9, max_depth=6, rr_depth=3)
+    ref_rgb, ref_w, _ = gpu_ctx.scene(sc, width=4).render(opts)
+    ref = ref_rgb / ref_w[..., None]
+    for width in (2, 8):
+        rgb, w, st = gpu_ctx.scene(sc, width=width).render(opts)
+        np.testing.assert_array_equal(w, ref_w)
+        img = rgb / w[..., None]
+        close = (np.abs(img - ref) <= 1e-3 * np.abs(ref) + 1e-4).all(axis=-1).mean()
+        print(f"width {width}: {close:.4f} of pixels equal to BVH4's")
+        assert close >= 0.99
