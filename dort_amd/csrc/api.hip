@@ -763,7 +763,7 @@ dgpu_status dgpu_render(dgpu_scene* s, const dgpu_render_opts* o, float* film_rg
   if (o->sample_end < o->sample_begin) return fail(DGPU_EINVAL, "sample_end < sample_begin");
   if (o->max_depth + 2 >= MAX_BOUNCES) return fail(DGPU_EINVAL, "max_depth must be < %u", MAX_BOUNCES - 2);
   if (o->direct_strategy > DGPU_DIRECT_LIGHT) return fail(DGPU_EINVAL, "bad direct_strategy");
-  if ((uint64_t)o->x_res * o->y_res > 0xFFFFFFFFull) return fail(DGPU_EINVAL, "film too large");
+  if ((uint64_t)o->x_res * o->y_res > 0xFFFFFFFFull || o->x_res >= (1u << 28)) return fail(DGPU_EINVAL, "film too large");
   if (o->sample_all_lights && s->S.num_lights > MAX_SLOT_LIGHTS)
     return fail(DGPU_EUNSUPPORTED, "sample_all_lights supports at most %u lights", MAX_SLOT_LIGHTS);
   if (out_stats) memset(out_stats, 0, sizeof *out_stats);
@@ -776,7 +776,9 @@ dgpu_status dgpu_render(dgpu_scene* s, const dgpu_render_opts* o, float* film_rg
   const uint64_t total = (uint64_t)npix * (o->sample_end - o->sample_begin);
   uint32_t slots = o->sample_all_lights ? 2 * std::max(1u, s->S.num_lights) : 2;
   uint64_t tile = s->queue_tile ? s->queue_tile : ctx->default_tile(slots);
-  uint32_t C = (uint32_t)std::min<uint64_t>(std::min<uint64_t>(total ? total : 1, tile), 0xFFFFFFFFull / slots);
+  // q0 + n_chunk must stay below 2^32 (the kernels' 32-bit path positions)
+  uint32_t C = (uint32_t)std::min<uint64_t>(std::min<uint64_t>(total ? total : 1, tile),
+                                            std::min<uint64_t>(0xFFFFFFFFull / slots, std::max<uint64_t>(1, 0xFFFFFFFFull - npix)));
   const uint32_t shade_grid = ctx->shade_blocks(false, o->sample_all_lights != 0, s->S.has_rough != 0);
 
   KernelCfg ce = s->cfg, cc = s->cfg;
@@ -787,6 +789,7 @@ dgpu_status dgpu_render(dgpu_scene* s, const dgpu_render_opts* o, float* film_rg
   memset(&P, 0, sizeof P);
   P.S = s->S;
   P.xr = o->x_res; P.yr = o->y_res; P.npix = npix;
+  P.ord = make_path_order(o->x_res, o->y_res);
   P.min_depth = o->min_depth; P.max_depth = o->max_depth; P.rr_depth = o->rr_depth;
   P.only_direct = o->only_direct; P.sample_all = o->sample_all_lights; P.strategy = o->direct_strategy;
   P.slots = slots;
@@ -843,6 +846,8 @@ dgpu_status dgpu_render(dgpu_scene* s, const dgpu_render_opts* o, float* film_rg
   for (uint64_t g0 = 0; g0 < total && !cancelled; g0 += C) {
     P.g0 = g0;
     P.n_chunk = (uint32_t)std::min<uint64_t>(C, total - g0);
+    P.q0 = (uint32_t)(g0 % npix);
+    P.s0 = o->sample_begin + (uint32_t)(g0 / npix);
     HIP_TRY(timed(3, [&] { return launch_raygen(P, P.n_chunk, false, stream); }));
     for (uint32_t b = 0; b <= o->max_depth; ++b) {
       if (cancel && *cancel) { cancelled = true; break; }
@@ -903,7 +908,7 @@ dgpu_status dgpu_render_light(dgpu_scene* s, const dgpu_render_opts* o, float* f
   if (o->sample_end < o->sample_begin) return fail(DGPU_EINVAL, "sample_end < sample_begin");
   if (o->max_depth + 2 >= MAX_BOUNCES) return fail(DGPU_EINVAL, "max_depth must be < %u", MAX_BOUNCES - 2);
   if (o->min_depth > o->max_depth) return fail(DGPU_EINVAL, "min_depth > max_depth");
-  if ((uint64_t)o->x_res * o->y_res > 0xFFFFFFFFull) return fail(DGPU_EINVAL, "film too large");
+  if ((uint64_t)o->x_res * o->y_res > 0xFFFFFFFFull || o->x_res >= (1u << 28)) return fail(DGPU_EINVAL, "film too large");
   if (out_stats) memset(out_stats, 0, sizeof *out_stats);
   dgpu_ctx* ctx = s->ctx;
   std::lock_guard<std::mutex> lk(ctx->mu);
@@ -1066,7 +1071,7 @@ dgpu_status dgpu_render_bdpt(dgpu_scene* s, const dgpu_render_opts* o, uint32_t 
   if (o->sample_end < o->sample_begin) return fail(DGPU_EINVAL, "sample_end < sample_begin");
   if (o->max_depth >= 30) return fail(DGPU_EINVAL, "max_depth must be < 30");
   if (o->min_depth > o->max_depth) return fail(DGPU_EINVAL, "min_depth > max_depth");
-  if ((uint64_t)o->x_res * o->y_res > 0xFFFFFFFFull) return fail(DGPU_EINVAL, "film too large");
+  if ((uint64_t)o->x_res * o->y_res > 0xFFFFFFFFull || o->x_res >= (1u << 28)) return fail(DGPU_EINVAL, "film too large");
   if (out_stats) memset(out_stats, 0, sizeof *out_stats);
   dgpu_ctx* ctx = s->ctx;
   std::lock_guard<std::mutex> lk(ctx->mu);
@@ -1249,7 +1254,8 @@ dgpu_status dgpu_render_batch(dgpu_ctx* ctx, const dgpu_batch_item* items, uint3
     if (!it.scene || it.scene->ctx != ctx) return fail(DGPU_EINVAL, "item %u: scene of another context", i);
     if (!it.film_rgb_sum || !it.film_weight) return fail(DGPU_EINVAL, "item %u: null film", i);
     if (o.x_res == 0 || o.y_res == 0) return fail(DGPU_EINVAL, "item %u: empty film", i);
-    if ((uint64_t)o.x_res * o.y_res > (1u << 30)) return fail(DGPU_EINVAL, "item %u: film too large", i);
+    if ((uint64_t)o.x_res * o.y_res > (1u << 30) || o.x_res >= (1u << 28))
+      return fail(DGPU_EINVAL, "item %u: film too large", i);
     if (o.sample_end < o.sample_begin) return fail(DGPU_EINVAL, "item %u: sample_end < sample_begin", i);
     if (o.min_depth != o0.min_depth || o.max_depth != o0.max_depth || o.rr_depth != o0.rr_depth ||
         o.only_direct != o0.only_direct || o.direct_strategy != o0.direct_strategy)
@@ -1280,6 +1286,7 @@ dgpu_status dgpu_render_batch(dgpu_ctx* ctx, const dgpu_batch_item* items, uint3
     memset(&E, 0, sizeof E);
     E.S = items[i].scene->S;
     E.xr = o.x_res; E.yr = o.y_res; E.npix = o.x_res * o.y_res;
+    E.ord = make_path_order(o.x_res, o.y_res);
     E.sample_begin = o.sample_begin;
     dgpu_rng_key(o.seed, o.scene_id, &E.k0, &E.k1);
     paths[i] = (uint64_t)E.npix * (o.sample_end - o.sample_begin);
@@ -1333,6 +1340,9 @@ dgpu_status dgpu_render_batch(dgpu_ctx* ctx, const dgpu_batch_item* items, uint3
   uint64_t total = 0;
   for (uint64_t p : paths) total += p;
   uint64_t C = std::min<uint64_t>(s0->queue_tile ? s0->queue_tile : ctx->batch_tile(2), 0x7FFFFFFFull);
+  uint32_t max_npix = 0;
+  for (uint32_t i = 0; i < n; ++i) max_npix = std::max(max_npix, tab[i].npix);
+  C = std::min<uint64_t>(C, 0xFFFFFFFFull - max_npix);  // q_begin + slots below 2^32
   if (total > C) {
     uint64_t nch = (total + C - 1) / C;
     C = (total + nch - 1) / nch;
@@ -1353,6 +1363,8 @@ dgpu_status dgpu_render_batch(dgpu_ctx* ctx, const dgpu_batch_item* items, uint3
         sg.pair_begin = cd.n_pairs;
         sg.local_begin = local;
         sg.local_end = local + take;
+        sg.q_begin = (uint32_t)(local % tab[si].npix);
+        sg.s_begin = tab[si].sample_begin + (uint32_t)(local / tab[si].npix);
         sg.n_pairs = (uint32_t)std::min<uint64_t>(tab[si].npix, take);
         cd.n_chunk += (uint32_t)take;
         cd.n_pairs += sg.n_pairs;

@@ -515,7 +515,8 @@ __global__ __launch_bounds__(BD_BLOCK) void k_bdpt(WaveParams P, BdptArgs A) {
   if (i >= P.n_chunk) return;
   const DevScene& S = P.S;
   const uint64_t g = P.g0 + i;
-  const uint32_t sample = P.sample_begin + (uint32_t)(g / P.npix), pixel = (uint32_t)(g % P.npix);
+  const uint32_t sample = P.sample_begin + (uint32_t)(g / P.npix);
+  const uint32_t pixel = tile_pixel((uint32_t)(g % P.npix), P.xr, P.yr);  // 8x8 tiles per wave
   float r0[4];
   bd_rng(P, pixel, sample, 0, 0, r0);
   // film_pos = pixel + random_2d (bdpt_renderer.cpp:24-27); a jitter of exactly 0 puts the
@@ -666,7 +667,7 @@ __global__ __launch_bounds__(256) void k_bdpt_film(WaveParams P) {
   if (blockIdx.x == 0 && threadIdx.x == 0) P.stats[0] += P.n_chunk;
   if (px >= P.npix) return;
   uint64_t g_end = P.g0 + P.n_chunk;
-  uint64_t g = P.g0 + ((px + P.npix - (uint32_t)(P.g0 % P.npix)) % P.npix);
+  uint64_t g = P.g0 + (((uint64_t)tile_pos(px, P.xr, P.yr) + P.npix - (uint32_t)(P.g0 % P.npix)) % P.npix);
   if (g >= g_end) return;
   float fr = P.film_rgb[3 * (size_t)px], fg = P.film_rgb[3 * (size_t)px + 1], fb = P.film_rgb[3 * (size_t)px + 2];
   float fw = P.film_w[px];

@@ -19,6 +19,7 @@
 
 #include "../../include/dgpu_rng.h"
 #include "../../include/dortgpu.h"
+#include "path_order.hpp"
 
 #define DG_INLINE __host__ __device__ __forceinline__
 

@@ -54,16 +54,19 @@ struct BatchEntry {
   DevScene S;
   uint32_t xr, yr, npix, sample_begin;
   uint32_t k0, k1, pad0, pad1;
+  PathOrder ord;   // the scene's pixel order
   float* film_rgb;
   float* film_w;
 };
 
 // The part of one scene's paths that a batch chunk holds: scene-local path indices
 // [local_begin, local_end) at chunk slots [slot_begin, ...); film pairs (one per
-// distinct pixel) start at pair_begin.
+// distinct pixel) start at pair_begin.  The segment's first path is at position
+// q_begin (< npix) of sample s_begin.
 struct ChunkSeg {
   uint32_t scene, slot_begin, pair_begin, n_pairs;
   uint64_t local_begin, local_end;
+  uint32_t q_begin, s_begin;
 };
 
 struct ChunkDesc {
@@ -82,6 +85,9 @@ struct WaveParams {
   uint32_t sample_begin;
   uint32_t n_chunk;        // paths in this chunk
   uint64_t g0;             // first global path index of the chunk
+  PathOrder ord;           // pixel order of the image
+  uint32_t q0, s0;         // the chunk's first path: position q0 (< npix) of sample s0;
+                           // chunks are cut so that q0 + n_chunk < 2^32
   float4* thr;             // throughput rgb + flags
   float4* rad;             // radiance accumulated so far
   float4* pend;            // pending NEE contributions [path * slots + k]; w = 1 while visible

@@ -360,15 +360,11 @@ __global__ void k_raygen(WaveParams P) {
     const ChunkSeg& sg = find_seg_slot(cd, i);
     scene = sg.scene;
     const BatchEntry& E = P.batch[scene];
-    uint64_t local = sg.local_begin + (i - sg.slot_begin);
-    sample = E.sample_begin + (uint32_t)(local / E.npix);
-    pixel = (uint32_t)(local % E.npix);
+    order_sample_pixel(E.ord, sg.s_begin, sg.q_begin + (i - sg.slot_begin), sample, pixel);
     xr = E.xr; yr = E.yr; k0 = E.k0; k1 = E.k1;
     S = &E.S;
   } else {
-    uint64_t g = P.g0 + i;
-    sample = P.sample_begin + (uint32_t)(g / P.npix);
-    pixel = (uint32_t)(g % P.npix);
+    order_sample_pixel(P.ord, P.s0, P.q0 + i, sample, pixel);
     xr = P.xr; yr = P.yr; k0 = P.k0; k1 = P.k1;
     S = &P.S;
   }
@@ -540,8 +536,8 @@ __device__ __forceinline__ void emit_shadow_batch(const WaveParams& P, const Nee
 template <bool BATCH, bool ALL, bool ROUGH>
 __device__ __forceinline__ void shade_body(const WaveParams& P, const DevScene& S, uint32_t bounce,
                                            bool active, uint32_t i, uint32_t p, uint32_t scene,
-                                           uint32_t pixel, uint32_t sample, uint32_t k0, uint32_t k1,
-                                           uint32_t* sh) {
+                                           const PathOrder& ord, uint32_t s0, uint32_t t, uint32_t k0,
+                                           uint32_t k1, uint32_t* sh) {
   const uint32_t cur = bounce & 1u, nxt = cur ^ 1u;
   const uint32_t SL = P.slots;
   uint32_t flags = 0, bounces = 0;
@@ -553,6 +549,9 @@ __device__ __forceinline__ void shade_body(const WaveParams& P, const DevScene& 
   // random blocks are drawn where they are first needed (they are pure functions of
   // the path, bounce and block), so fewer of them are live across the NEE code
   float rb0[4] = {0.f, 0.f, 0.f, 0.f}, rb1[4] = {0.f, 0.f, 0.f, 0.f}, rb2[4] = {0.f, 0.f, 0.f, 0.f};
+  // the path's sample and pixel (position t from the start of sample s0), needed only by
+  // a path that scatters here
+  uint32_t sample = 0, pixel = 0;
   if (active) {
     if (bounce == 0u) {
       // the path starts here (k_raygen writes only the queue)
@@ -593,6 +592,7 @@ __device__ __forceinline__ void shade_body(const WaveParams& P, const DevScene& 
       shading = true;
       wo = normalize(-rd);
       bsdf = make_bsdf(S, g);
+      order_sample_pixel(ord, s0, t, sample, pixel);
       dgpu_rng_block(k0, k1, pixel, sample, bounce, 0, rb0);
     }
   }
@@ -709,7 +709,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) __attribute__((amdgpu_waves_per_eu(DGP
 void k_shade(WaveParams P, uint32_t bounce) {
   __shared__ uint32_t sh[SHADE_BLOCK / 64 + 1];
   __shared__ uint32_t seg_begin[BATCH ? MAX_CHUNK_SEGS : 1], seg_scene[BATCH ? MAX_CHUNK_SEGS : 1];
-  __shared__ uint64_t seg_local[BATCH ? MAX_CHUNK_SEGS : 1];
+  __shared__ uint32_t seg_q[BATCH ? MAX_CHUNK_SEGS : 1], seg_s[BATCH ? MAX_CHUNK_SEGS : 1];
   const uint32_t cur = bounce & 1u;
   const uint32_t n = P.counters[CNT_QUEUE + bounce];
   uint32_t nsegs = 0;
@@ -719,7 +719,8 @@ void k_shade(WaveParams P, uint32_t bounce) {
     for (uint32_t k = threadIdx.x; k < nsegs; k += SHADE_BLOCK) {
       seg_begin[k] = cd.segs[k].slot_begin;
       seg_scene[k] = cd.segs[k].scene;
-      seg_local[k] = cd.segs[k].local_begin;
+      seg_q[k] = cd.segs[k].q_begin;
+      seg_s[k] = cd.segs[k].s_begin;
     }
     __syncthreads();
   }
@@ -739,17 +740,12 @@ void k_shade(WaveParams P, uint32_t bounce) {
       using CEntry = const __attribute__((address_space(4))) BatchEntry;
       const CEntry& E = ((CEntry*)P.batch)[scene];
       // the path's pixel and sample, as k_raygen derived them from its segment
-      uint64_t local = seg_local[sg] + (p - seg_begin[sg]);
-      uint32_t sample = E.sample_begin + (uint32_t)(local / E.npix);
-      uint32_t pixel = (uint32_t)(local % E.npix);
-      shade_body<true, false, ROUGH>(P, *(const DevScene*)&E.S, bounce, active, i, p, scene, pixel, sample, E.k0, E.k1,
-                              sh);
+      shade_body<true, false, ROUGH>(P, *(const DevScene*)&E.S, bounce, active, i, p, scene,
+                                     *(const PathOrder*)&E.ord, seg_s[sg], seg_q[sg] + (p - seg_begin[sg]), E.k0,
+                                     E.k1, sh);
     } else {
       uint32_t p = active ? P.qpath[cur][i] : 0u;
-      uint64_t gidx = P.g0 + p;
-      uint32_t sample = P.sample_begin + (uint32_t)(gidx / P.npix);
-      uint32_t pixel = (uint32_t)(gidx % P.npix);
-      shade_body<false, ALL, ROUGH>(P, P.S, bounce, active, i, p, 0u, pixel, sample, P.k0, P.k1, sh);
+      shade_body<false, ALL, ROUGH>(P, P.S, bounce, active, i, p, 0u, P.ord, P.s0, P.q0 + p, P.k0, P.k1, sh);
     }
   }
 }
@@ -824,7 +820,7 @@ __global__ __launch_bounds__(256) void k_finalize(WaveParams P) {
     const CEntry& E = ((CEntry*)P.batch)[sg.scene];  // constant: the film pointers are global
     uint32_t q = t - sg.pair_begin;
     uint64_t local = sg.local_begin + q;
-    uint32_t px = (uint32_t)(local % E.npix);
+    uint32_t px = tile_pixel((uint32_t)(local % E.npix), E.xr, E.yr);
     film_accumulate(P, E.film_rgb, E.film_w, px, sg.slot_begin + q, E.npix, local, sg.local_end, E.sample_begin,
                     E.k0, E.k1);
   } else {
@@ -832,7 +828,9 @@ __global__ __launch_bounds__(256) void k_finalize(WaveParams P) {
     uint32_t px = t;
     if (px >= P.npix) return;
     uint64_t g_end = P.g0 + P.n_chunk;
-    uint64_t g = P.g0 + ((px + P.npix - (uint32_t)(P.g0 % P.npix)) % P.npix);
+    // the chunk's first path on pixel px (paths of a sample visit pixels in tile order)
+    uint32_t q = tile_pos(px, P.xr, P.yr);
+    uint64_t g = P.g0 + (((uint64_t)q + P.npix - (uint32_t)(P.g0 % P.npix)) % P.npix);
     if (g >= g_end) return;
     film_accumulate(P, P.film_rgb, P.film_w, px, (uint32_t)(g - P.g0), P.npix, g, g_end, P.sample_begin, P.k0,
                     P.k1);
