@@ -237,27 +237,28 @@ def batch_bench(ctx, args):
     """64 independent scenes, triangle counts log-uniform in [1k, 500k] (displaced
     geodesic spheres and boxes on a ground plane), resolutions uniform in {256^2, 512^2,
     768^2}, 16 spp, max depth 6, each lit by a 1024x512 sky.  Both legs use the knobs the
-    autotuner picks for the batch.  Batched wavefront (one
-    hipGraph replayed per chunk) against one dgpu_render launch sequence per scene; each
+    autotuner picks per triangle-count class.  Batched wavefront (every chunk in one
+    hipGraph launch) against one dgpu_render launch sequence per scene; each
     leg has its own films in HBM, and before timing the batched films are checked
     against the per-scene ones (weights bitwise, radiance to rtol 1e-5)."""
     from dort_amd import abi, autotune, gpu, scenes
     t0 = time.time()
     batch = scenes.batch_scenes(64, seed=3)
-    # the knobs the batch shares (BVH width, stack entries in LDS, trace workgroup),
-    # picked by the autotuner from batched renders of every eighth scene at the workload's
-    # 16 spp (at 1 spp the launches are too small for occupancy to show, and the tuner
-    # picked a deeper LDS stack that made the full batch 15% slower)
+    # the knobs per size class (BVH width, stack entries in LDS, trace workgroup), picked
+    # by the autotuner from batched renders of up to 8 scenes of each class at the
+    # workload's 16 spp (at 1 spp the launches are too small for occupancy to show, and
+    # the tuner picked a deeper LDS stack that made the full batch 15% slower);
+    # dgpu_render_batch traces each class's chunks with its own kernel variant
     t_tune = time.time()
-    sample = list(range(0, 64, 8))
-    knobs, table = autotune.autotune_batch(
-        ctx, [batch[k][0] for k in sample],
-        [abi.render_opts(batch[k][1], batch[k][1], spp=16, seed=500 + k, max_depth=6, rr_depth=3, scene_id=k)
-         for k in sample],
-        shape_space=[(2, 4), (4, 4)],
+    all_opts = [abi.render_opts(res, res, spp=16, seed=500 + k, max_depth=6, rr_depth=3, scene_id=k)
+                for k, (_, res) in enumerate(batch)]
+    scene_knobs, classes = autotune.autotune_batch_classes(
+        ctx, [sc for sc, _ in batch], all_opts, bounds=(20000, 150000), per_class=8,
+        shape_space=[(4, 4), (8, 4), (2, 4)],
         launch_space=[("stack_lds", (8, 16, 32)), ("block_size", (64, 128, 256))])
     t_tune = time.time() - t_tune
-    log(f"batch autotune: {knobs} ({len(table)} candidates, {t_tune:.1f}s)")
+    for (lo, hi), idx, best, table in classes:
+        log(f"batch autotune, {len(idx)} scenes of {lo}-{hi} triangles: {best} ({len(table)} candidates)")
     items, films_p, films_b = [], [], []
 
     def zeroed(n):
@@ -266,8 +267,8 @@ def batch_bench(ctx, args):
         return a
 
     for k, (sc, res) in enumerate(batch):
-        gsc = ctx.scene(sc, **knobs)
-        o = abi.render_opts(res, res, spp=16, seed=500 + k, max_depth=6, rr_depth=3, scene_id=k)
+        gsc = ctx.scene(sc, **scene_knobs[k])
+        o = all_opts[k]
         items.append((gsc, o))
         films_p.append((zeroed(res * res * 3), zeroed(res * res)))
         films_b.append((zeroed(res * res * 3), zeroed(res * res)))
@@ -303,9 +304,12 @@ def batch_bench(ctx, args):
             "paths": paths, "per_scene_Mpaths_s": paths / t_p / 1e6, "batched_Mpaths_s": paths / t_b / 1e6,
             "speedup": t_p / t_b, "per_scene_ms": t_p * 1e3, "batched_ms": t_b * 1e3,
             "films_equal_per_scene": not mismatched,
-            "knobs": knobs, "autotune": {"candidates": [[k, round(ms, 3)] for k, ms in table],
-                                         "seconds": round(t_tune, 1),
-                                         "sample": "every eighth scene, 16 spp, batched"}}
+            "graph_launches": 1,
+            "autotune": {"classes": [{"triangles": [lo, hi if hi != float("inf") else None], "scenes": len(idx),
+                                      "knobs": best, "candidates": [[k, round(ms, 3)] for k, ms in table]}
+                                     for (lo, hi), idx, best, table in classes],
+                         "seconds": round(t_tune, 1),
+                         "sample": "up to 8 scenes of each triangle-count class, 16 spp, batched"}}
 
 
 def cpu_baseline(sc, args):

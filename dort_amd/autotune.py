@@ -8,10 +8,14 @@ best BVH.  Results are cached per workload in memory.
 
     best, table = autotune(ctx, scene, opts)                 # one scene, opts with few spp
     gsc = ctx.scene(scene, **best)
-    best, table = autotune_batch(ctx, scenes, opts_list)     # knobs a whole batch shares
+    best, table = autotune_batch(ctx, scenes, opts_list)     # knobs for a whole batch
+    knobs, classes = autotune_batch_classes(ctx, scenes, opts_list)  # knobs per size class
 
-A batch's scenes share one BVH width and kernel variant (dgpu_render_batch), so the
-batch tuner times dgpu_render_batch over the batch (or a sample of it) per candidate.
+dgpu_render_batch groups a batch's items by their scenes' knobs and traces each group
+with its own kernel variant, all in one graph launch.  The batch tuner times
+dgpu_render_batch over a set of scenes (or a sample of it) per candidate; the class
+tuner splits the batch by triangle count and tunes each class on a sample of its own
+scenes, so small and large scenes can take different BVH widths and stacks.
 """
 import time
 
@@ -139,3 +143,26 @@ def autotune_batch(ctx, scene_list, opts_list, repeats=1, log=None, shape_space=
             f.free()
             w.free()
     return _cache[key]
+
+
+def autotune_batch_classes(ctx, scene_list, opts_list, bounds=(20000, 150000), per_class=8, repeats=1, log=None,
+                           shape_space=None, launch_space=None):
+    """Knobs per scene of a batch, tuned per size class: the scenes are split by triangle
+    count at `bounds` (class k holds counts in [bounds[k-1], bounds[k])), and each class is
+    tuned with autotune_batch on at most `per_class` of its scenes (evenly spaced).
+    Returns ([knobs of scene i], [(class bounds, scene indices, best knobs, table), ...])."""
+    edges = (0,) + tuple(bounds) + (float("inf"),)
+    knobs = [None] * len(scene_list)
+    classes = []
+    for lo, hi in zip(edges[:-1], edges[1:]):
+        idx = [i for i, s in enumerate(scene_list) if lo <= s.num_triangles < hi]
+        if not idx:
+            continue
+        step = max(1, len(idx) // per_class)
+        sample = idx[::step][:per_class]
+        best, table = autotune_batch(ctx, [scene_list[i] for i in sample], [opts_list[i] for i in sample],
+                                     repeats=repeats, log=log, shape_space=shape_space, launch_space=launch_space)
+        for i in idx:
+            knobs[i] = dict(best)
+        classes.append(((lo, hi), idx, best, table))
+    return knobs, classes

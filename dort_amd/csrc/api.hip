@@ -1261,9 +1261,6 @@ dgpu_status dgpu_render_batch(dgpu_ctx* ctx, const dgpu_batch_item* items, uint3
         o.only_direct != o0.only_direct || o.direct_strategy != o0.direct_strategy)
       return fail(DGPU_EINVAL, "item %u: depth / strategy options differ from item 0", i);
     if (o.sample_all_lights) return fail(DGPU_EUNSUPPORTED, "batches do not support sample_all_lights");
-    const KernelCfg& c = it.scene->cfg;
-    if (c.width != s0->cfg.width || c.lds != s0->cfg.lds || c.block != s0->cfg.block)
-      return fail(DGPU_EINVAL, "item %u: BVH width / kernel knobs differ from item 0", i);
   }
   if (o0.max_depth + 2 >= MAX_BOUNCES) return fail(DGPU_EINVAL, "max_depth must be < %u", MAX_BOUNCES - 2);
   if (o0.direct_strategy > DGPU_DIRECT_LIGHT) return fail(DGPU_EINVAL, "bad direct_strategy");
@@ -1330,32 +1327,55 @@ dgpu_status dgpu_render_batch(dgpu_ctx* ctx, const dgpu_batch_item* items, uint3
     }
   }
 
-  // chunks over the scene-major concatenation of every scene's paths.  Queue words hold
-  // chunk path indices (NEE slot 2 * path + k), and the kernels find a path's scene by a
-  // search over the chunk's segment starts, so a chunk can hold the whole batch when
-  // memory allows.  Larger chunks mean fewer bounce launches and shorter tails (64
-  // scenes: 2^23-path chunks rendered the batch workload in 319 ms, 2^25 in 227 ms,
-  // 2^26 in 212 ms, and chunks of a third of the batch, 110M paths, in 202 ms; 2^28 and
-  // one 331M-path chunk took 207 ms).  Chunks are cut to equal sizes.
+  // Items are grouped by their scene's kernel knobs (BVH width, LDS stack entries,
+  // trace workgroup); each group's paths are cut into chunks of its own, so a chunk is
+  // traced by one kernel variant.  Chunks run over the scene-major concatenation of the
+  // group's paths.  Queue words hold chunk path indices (NEE slot 2 * path + k), and the
+  // kernels find a path's scene by a search over the chunk's segment starts, so a chunk
+  // can hold the whole group when memory allows.  Larger chunks mean fewer bounce
+  // launches and shorter tails (64 scenes: 2^23-path chunks rendered the batch workload
+  // in 319 ms, 2^25 in 227 ms, 2^26 in 212 ms, and chunks of a third of the batch, 110M
+  // paths, in 202 ms; 2^28 and one 331M-path chunk took 207 ms).  A group's chunks are
+  // cut to equal sizes.
   uint64_t total = 0;
   for (uint64_t p : paths) total += p;
-  uint64_t C = std::min<uint64_t>(s0->queue_tile ? s0->queue_tile : ctx->batch_tile(2), 0x7FFFFFFFull);
   uint32_t max_npix = 0;
   for (uint32_t i = 0; i < n; ++i) max_npix = std::max(max_npix, tab[i].npix);
-  C = std::min<uint64_t>(C, 0xFFFFFFFFull - max_npix);  // q_begin + slots below 2^32
-  if (total > C) {
-    uint64_t nch = (total + C - 1) / C;
-    C = (total + nch - 1) / nch;
+  std::vector<KernelCfg> group_cfg;        // knobs of each group
+  std::vector<std::vector<uint32_t>> group_items;
+  for (uint32_t i = 0; i < n; ++i) {
+    const KernelCfg& c = items[i].scene->cfg;
+    size_t g = 0;
+    while (g < group_cfg.size() &&
+           !(group_cfg[g].width == c.width && group_cfg[g].lds == c.lds && group_cfg[g].block == c.block))
+      ++g;
+    if (g == group_cfg.size()) {
+      group_cfg.push_back(c);
+      group_items.emplace_back();
+    }
+    group_items[g].push_back(i);
   }
   std::vector<ChunkDesc> chunks;
-  {
-    uint32_t si = 0;
+  std::vector<uint32_t> chunk_group;  // group of each chunk
+  for (size_t g = 0; g < group_items.size(); ++g) {
+    const std::vector<uint32_t>& gi = group_items[g];
+    uint64_t gtotal = 0;
+    for (uint32_t i : gi) gtotal += paths[i];
+    const dgpu_scene* sg0 = items[gi[0]].scene;
+    uint64_t C = std::min<uint64_t>(sg0->queue_tile ? sg0->queue_tile : ctx->batch_tile(2), 0x7FFFFFFFull);
+    C = std::min<uint64_t>(C, 0xFFFFFFFFull - max_npix);  // q_begin + slots below 2^32
+    if (gtotal > C) {
+      uint64_t nch = (gtotal + C - 1) / C;
+      C = (gtotal + nch - 1) / nch;
+    }
+    size_t k = 0;
     uint64_t local = 0;
-    while (si < n && paths[si] == 0) ++si;
-    while (si < n) {
+    while (k < gi.size() && paths[gi[k]] == 0) ++k;
+    while (k < gi.size()) {
       ChunkDesc cd;
       memset(&cd, 0, sizeof cd);
-      while (si < n && cd.n_chunk < C && cd.n_segs < MAX_CHUNK_SEGS) {
+      while (k < gi.size() && cd.n_chunk < C && cd.n_segs < MAX_CHUNK_SEGS) {
+        const uint32_t si = gi[k];
         uint64_t take = std::min<uint64_t>(paths[si] - local, C - cd.n_chunk);
         ChunkSeg& sg = cd.segs[cd.n_segs++];
         sg.scene = si;
@@ -1370,12 +1390,13 @@ dgpu_status dgpu_render_batch(dgpu_ctx* ctx, const dgpu_batch_item* items, uint3
         cd.n_pairs += sg.n_pairs;
         local += take;
         if (local == paths[si]) {
-          ++si;
+          ++k;
           local = 0;
-          while (si < n && paths[si] == 0) ++si;
+          while (k < gi.size() && paths[gi[k]] == 0) ++k;
         }
       }
       chunks.push_back(cd);
+      chunk_group.push_back((uint32_t)g);
     }
   }
   if (chunks.empty()) return DGPU_OK;
@@ -1385,10 +1406,15 @@ dgpu_status dgpu_render_batch(dgpu_ctx* ctx, const dgpu_batch_item* items, uint3
   for (uint32_t i = 0; i < n; ++i) any_rough |= items[i].scene->S.has_rough;
   const uint32_t shade_grid = ctx->shade_blocks(true, false, any_rough != 0);
 
-  KernelCfg ce = s0->cfg, cc = s0->cfg;
-  ce.grid = ctx->grid_for(4, ce);
-  cc.grid = ctx->grid_for(5, cc);
-  uint32_t maxgrid = std::max(ce.grid, cc.grid);
+  // per group: the extend and connect launch shapes; the spill area fits the largest
+  std::vector<KernelCfg> group_ce(group_cfg), group_cc(group_cfg);
+  size_t spill_threads = 0;
+  for (size_t g = 0; g < group_cfg.size(); ++g) {
+    group_ce[g].grid = ctx->grid_for(4, group_ce[g]);
+    group_cc[g].grid = ctx->grid_for(5, group_cc[g]);
+    spill_threads = std::max<size_t>(spill_threads,
+                                     (size_t)std::max(group_ce[g].grid, group_cc[g].grid) * group_cfg[g].block);
+  }
   WaveParams P;
   memset(&P, 0, sizeof P);
   P.S = s0->S;
@@ -1411,7 +1437,7 @@ dgpu_status dgpu_render_batch(dgpu_ctx* ctx, const dgpu_batch_item* items, uint3
   P.smeta = (uint2*)ctx->buf("smeta", (size_t)cap * 2 * 8, &st);
   P.counters = (uint32_t*)ctx->buf("counters", NUM_COUNTERS * 4, &st);
   P.stats = (unsigned long long*)ctx->buf("stats", STATS_WORDS * 8, &st);
-  P.spill = (uint32_t*)ctx->buf("spill", (size_t)maxgrid * s0->cfg.block * stack_cap * 4, &st);
+  P.spill = (uint32_t*)ctx->buf("spill", spill_threads * stack_cap * 4, &st);
   P.stack_cap = stack_cap;
   BatchEntry* dtab = (BatchEntry*)ctx->buf("batch_tab", n * sizeof(BatchEntry), &st);
   ChunkDesc* dchunks = (ChunkDesc*)ctx->buf("batch_chunks", chunks.size() * sizeof(ChunkDesc), &st);
@@ -1428,8 +1454,11 @@ dgpu_status dgpu_render_batch(dgpu_ctx* ctx, const dgpu_batch_item* items, uint3
   // tab and chunks must stay alive until the copies above have run
   HIP_TRY(hipStreamSynchronize(stream));
 
-  // one chunk: raygen -> (extend -> shade -> connect) x (max_depth + 1) -> finalize -> advance
-  auto chunk_sequence = [&]() -> hipError_t {
+  // chunk c: raygen -> (extend -> shade -> connect) x (max_depth + 1) -> finalize -> advance,
+  // with its group's trace kernels; the kernels read the chunk index from the device
+  auto chunk_sequence = [&](size_t c) -> hipError_t {
+    const KernelCfg& ce = group_ce[chunk_group[c]];
+    const KernelCfg& cc = group_cc[chunk_group[c]];
     hipError_t e = launch_raygen(P, cap, true, stream);
     for (uint32_t b = 0; e == hipSuccess && b <= o0.max_depth; ++b) {
       e = launch_extend(P, ce, b, true, stream);
@@ -1442,10 +1471,12 @@ dgpu_status dgpu_render_batch(dgpu_ctx* ctx, const dgpu_batch_item* items, uint3
   };
   HIP_TRY(hipEventRecord(ctx->ev0, stream));
   if (use_graph) {
+    // every chunk of every group in one graph: the whole batch is one graph launch
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-    hipError_t ce_ = chunk_sequence();
+    hipError_t ce_ = hipSuccess;
+    for (size_t c = 0; c < chunks.size() && ce_ == hipSuccess; ++c) ce_ = chunk_sequence(c);
     hipError_t ee = hipStreamEndCapture(stream, &graph);
     if (ce_ != hipSuccess || ee != hipSuccess) {
       if (graph) (void)hipGraphDestroy(graph);
@@ -1456,14 +1487,13 @@ dgpu_status dgpu_render_batch(dgpu_ctx* ctx, const dgpu_batch_item* items, uint3
       (void)hipGraphDestroy(graph);
       return fail(DGPU_EDEVICE, "hipGraphInstantiate failed: %s", hipGetErrorString(ie));
     }
-    hipError_t le = hipSuccess;
-    for (size_t c = 0; c < chunks.size() && le == hipSuccess; ++c) le = hipGraphLaunch(exec, stream);
+    hipError_t le = hipGraphLaunch(exec, stream);
     (void)hipStreamSynchronize(stream);
     (void)hipGraphExecDestroy(exec);
     (void)hipGraphDestroy(graph);
     if (le != hipSuccess) return fail(DGPU_EDEVICE, "hipGraphLaunch failed: %s", hipGetErrorString(le));
   } else {
-    for (size_t c = 0; c < chunks.size(); ++c) HIP_TRY(chunk_sequence());
+    for (size_t c = 0; c < chunks.size(); ++c) HIP_TRY(chunk_sequence(c));
   }
   HIP_TRY(hipEventRecord(ctx->ev1, stream));
   unsigned long long hstats[STATS_WORDS] = {};

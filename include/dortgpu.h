@@ -309,15 +309,17 @@ dgpu_status dgpu_render_bdpt(dgpu_scene* scene, const dgpu_render_opts* opts, ui
                              dgpu_stats* out_stats, const volatile int* cancel);
 
 /* ---- batched rendering (no dort counterpart) -------------------------------
- * Renders many scenes in one wavefront: every scene's paths go into shared queues,
- * the per-chunk bounce loop (raygen, extend/shade/connect per bounce, film) is
- * captured once in a hipGraph and replayed per chunk; the chunk index advances on
- * the device.  Scenes may differ in triangles, resolution, sample range and seed;
- * they must share the depth / strategy options, BVH width and kernel knobs, and a
- * context.  Each item's film is accumulated like dgpu_render's, so batch results
- * equal per-scene renders.  n is in [1, 255].  Chunks hold at most item 0's scene
- * queue_tile paths, or by default equal shares of the batch of at most 32 GiB of path
- * state each (about 148M paths); a chunk spans at most 64 scenes.
+ * Renders many scenes in one wavefront: every scene's paths go into shared queues, and
+ * the bounce loop of every chunk (raygen, extend/shade/connect per bounce, film) is
+ * captured in one hipGraph, launched once; the chunk index advances on the device.
+ * Scenes may differ in triangles, resolution, sample range, seed, BVH width and kernel
+ * knobs (dgpu_scene_create's width, stack_lds and block_size): items whose scenes share
+ * the knobs form a group, and each group's paths are cut into chunks of their own,
+ * traced by that group's kernel variant.  Items must share the depth / strategy options
+ * and a context.  Each item's film is accumulated like dgpu_render's, so batch results
+ * equal per-scene renders.  n is in [1, 255].  A group's chunks hold at most its first
+ * scene's queue_tile paths, or by default equal shares of the group of at most 32 GiB of
+ * path state each (about 148M paths); a chunk spans at most 64 scenes.
  * flags: DGPU_BUF_DEVICE (films are device pointers), DGPU_BATCH_NO_GRAPH.  Device films
  * are added to without atomics, so no two of them (nor an item's colour and weight
  * arrays) may overlap: such a batch fails with DGPU_EINVAL.  Host films may be shared

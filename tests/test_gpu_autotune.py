@@ -45,3 +45,27 @@ def test_autotune_batch_knobs_keep_batch_equal_to_per_scene(gpu_ctx):
         ref_rgb, ref_w, _ = g.render(o)
         assert np.array_equal(w, ref_w)
         np.testing.assert_allclose(rgb, ref_rgb, rtol=1e-5, atol=1e-6)
+
+
+def test_autotune_batch_classes(gpu_ctx):
+    """Per-class knobs: scenes split by triangle count, each class tuned on its own
+    sample; the batch rendered with every scene's class knobs equals the per-scene renders."""
+    batch = scenes.batch_scenes(6, seed=8, min_tris=1000, max_tris=30000, sky_res=(64, 32))
+    sc_list = [s for s, _ in batch]
+    opts = [abi.render_opts(20, 20, spp=2, seed=70 + k, max_depth=5, rr_depth=3, scene_id=k) for k in range(6)]
+    cut = sorted(s.num_triangles for s in sc_list)[3]
+    knobs, classes = autotune.autotune_batch_classes(
+        gpu_ctx, sc_list, opts, bounds=(cut,), per_class=2, shape_space=[(8, 4)],
+        launch_space=[("stack_lds", (32,))])
+    print("classes", [(b, idx, best) for b, idx, best, _ in classes])
+    assert sorted(i for _, idx, _, _ in classes for i in idx) == list(range(6))
+    for (lo, hi), idx, best, table in classes:
+        assert all(lo <= sc_list[i].num_triangles < hi for i in idx)
+        assert len(table) == 3 and all(np.isfinite(ms) for _, ms in table)
+        assert all(knobs[i] == best for i in idx)
+    items = [(gpu_ctx.scene(s, **k), o) for s, k, o in zip(sc_list, knobs, opts)]
+    films, _ = gpu_ctx.render_batch(items)
+    for (g, o), (rgb, w) in zip(items, films):
+        ref_rgb, ref_w, _ = g.render(o)
+        assert np.array_equal(w, ref_w)
+        np.testing.assert_allclose(rgb, ref_rgb, rtol=1e-5, atol=1e-6)

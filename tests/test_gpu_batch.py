@@ -1,5 +1,6 @@
 """Batched rendering (dgpu_render_batch): many scenes of uneven size in one batched
-wavefront with the bounce loop captured in a hipGraph, from a few small scenes up to
+wavefront with the bounce loops of every chunk captured in one hipGraph, scenes of
+different kernel knobs in groups, from a few small scenes up to
 the 64-scene batch workload itself.  Each scene's film must equal
 its own dgpu_render: the same counter-based random numbers give the same paths and
 the same per-pixel order; the batched and single-scene kernels are separate
@@ -42,6 +43,25 @@ def test_batch_graph_equals_direct_launches(gpu_ctx):
     b, _ = gpu_ctx.render_batch(items, graph=False)
     for (ra, wa), (rb, wb) in zip(a, b):
         assert np.array_equal(ra, rb) and np.array_equal(wa, wb)
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_batch_with_mixed_knobs_equals_per_scene(gpu_ctx, graph):
+    """Items whose scenes differ in BVH width, LDS stack and trace workgroup form groups
+    traced by their own kernel variants (one graph launch for every chunk of every group);
+    each film equals the scene's own render with the same knobs.  Small chunks make some
+    groups span several chunks."""
+    knobs = [dict(width=8, stack_lds=32, block_size=64, queue_tile=900),
+             dict(width=4, queue_tile=900), dict(width=2, stack_lds=8, block_size=256, queue_tile=900),
+             dict(width=8, stack_lds=32, block_size=64, queue_tile=900), dict(width=4, queue_tile=900),
+             dict(width=8, stack_lds=16, block_size=128, queue_tile=900)]
+    items = [(gpu_ctx.scene(sc, **k), o) for (sc, o), k in zip(_mixed_scenes(), knobs)]
+    films, st = gpu_ctx.render_batch(items, graph=graph)
+    assert st["paths"] == sum(o.x_res * o.y_res * (o.sample_end - o.sample_begin) for _, o in items)
+    for (gsc, o), (rgb, w) in zip(items, films):
+        ref_rgb, ref_w, _ = gsc.render(o)
+        assert np.array_equal(w, ref_w)
+        np.testing.assert_allclose(rgb, ref_rgb, rtol=1e-5, atol=1e-6)
 
 
 def test_batch_rejects_mismatched_options(gpu_ctx):
