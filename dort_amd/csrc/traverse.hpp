@@ -57,21 +57,19 @@ struct LaneStack {
   }
 };
 
-// Per-ray part of the watertight test: the ray-space transform of Woop et al. as three
-// row vectors, x = rx . (v - o), y = ry . (v - o), z = rz . (v - o), with
-// rx = e_kx - sx e_kz, ry = e_ky - sy e_kz, rz = sz e_kz (kz the dominant axis of the
-// direction).  Applying the rows with fused multiply-adds replaces the per-vertex
-// component permutation by three FMAs per coordinate; the zero entries add exact
-// zeros, so x = fma(-sx, A[kz], A[kx]) etc.  Every vertex is transformed the same way
-// in every triangle that shares it, which is all that watertightness needs.
+// Per-ray part of the watertight test (Woop et al.): the dominant axis kz of the
+// direction, the axes kx, ky that follow it (swapped when d[kz] < 0, which keeps the
+// winding), and the shear sx = d[kx] / d[kz], sy = d[ky] / d[kz], sz = 1 / d[kz].  A vertex
+// v goes to ray space as x = fma(-sx, A[kz], A[kx]), y = fma(-sy, A[kz], A[ky]),
+// z = sz A[kz] with A = v - o.  Every vertex is transformed the same way in every
+// triangle that shares it, which is all that watertightness needs.  The three axis
+// indices are packed in one word (2 bits each), so the per-ray state is the origin, three
+// factors and one word: eight registers where three transform rows took twelve.
 struct WtRay {
   V3 o;
-  V3 rx, ry, rz;
+  float sx, sy, sz;
+  uint32_t k;  // kx | ky << 2 | kz << 4
 };
-
-DG_INLINE V3 axis_scaled(int k, float v) {
-  return v3(k == 0 ? v : 0.f, k == 1 ? v : 0.f, k == 2 ? v : 0.f);
-}
 
 DG_INLINE WtRay wt_setup(V3 o, V3 d) {
   WtRay w;
@@ -81,15 +79,16 @@ DG_INLINE WtRay wt_setup(V3 o, V3 d) {
   int ky = kx + 1 == 3 ? 0 : kx + 1;
   float dz = comp(d, kz);
   if (dz < 0.f) { int t = kx; kx = ky; ky = t; }
-  float sx = comp(d, kx) / dz, sy = comp(d, ky) / dz, sz = 1.0f / dz;
+  w.sx = comp(d, kx) / dz;
+  w.sy = comp(d, ky) / dz;
+  w.sz = 1.0f / dz;
   w.o = o;
-  w.rx = axis_scaled(kx, 1.f) + axis_scaled(kz, -sx);
-  w.ry = axis_scaled(ky, 1.f) + axis_scaled(kz, -sy);
-  w.rz = axis_scaled(kz, sz);
+  w.k = (uint32_t)kx | ((uint32_t)ky << 2) | ((uint32_t)kz << 4);
   return w;
 }
 
-DG_INLINE float row_dot(V3 r, V3 a) { return fmaf(r.z, a.z, fmaf(r.y, a.y, r.x * a.x)); }
+// Component k (0, 1, 2) of a, by two selects.
+DG_INLINE float comp_sel(V3 a, uint32_t k) { return k == 0u ? a.x : (k == 1u ? a.y : a.z); }
 
 // Watertight test; on a hit inside [tmin, tmax] returns true with t, b1, b2.
 // Contraction into FMA is off here: an edge shared by two triangles must give edge
@@ -101,9 +100,11 @@ DG_INLINE bool wt_hit(const WtRay& w, float4 va, float4 vb, float4 vc, float tmi
                       float& t_out, float& b1_out, float& b2_out) {
 #pragma clang fp contract(off)
   V3 A = xyz(va) - w.o, B = xyz(vb) - w.o, C = xyz(vc) - w.o;
-  float Ax = row_dot(w.rx, A), Ay = row_dot(w.ry, A);
-  float Bx = row_dot(w.rx, B), By = row_dot(w.ry, B);
-  float Cx = row_dot(w.rx, C), Cy = row_dot(w.ry, C);
+  const uint32_t kx = w.k & 3u, ky = (w.k >> 2) & 3u, kz = w.k >> 4;
+  const float Akz = comp_sel(A, kz), Bkz = comp_sel(B, kz), Ckz = comp_sel(C, kz);
+  float Ax = fmaf(-w.sx, Akz, comp_sel(A, kx)), Ay = fmaf(-w.sy, Akz, comp_sel(A, ky));
+  float Bx = fmaf(-w.sx, Bkz, comp_sel(B, kx)), By = fmaf(-w.sy, Bkz, comp_sel(B, ky));
+  float Cx = fmaf(-w.sx, Ckz, comp_sel(C, kx)), Cy = fmaf(-w.sy, Ckz, comp_sel(C, ky));
   float U = Cx * By - Cy * Bx;
   float V = Ax * Cy - Ay * Cx;
   float W = Bx * Ay - By * Ax;
@@ -115,7 +116,7 @@ DG_INLINE bool wt_hit(const WtRay& w, float4 va, float4 vb, float4 vc, float tmi
   if (fminf(fminf(U, V), W) < 0.f && fmaxf(fmaxf(U, V), W) > 0.f) return false;  // mixed signs
   float det = U + V + W;
   if (det == 0.f) return false;
-  float Az = row_dot(w.rz, A), Bz = row_dot(w.rz, B), Cz = row_dot(w.rz, C);
+  float Az = w.sz * Akz, Bz = w.sz * Bkz, Cz = w.sz * Ckz;
   float T = fmaf(U, Az, fmaf(V, Bz, W * Cz));
   uint32_t sgn = __float_as_uint(det) & 0x80000000u;
   float Tn = __uint_as_float(__float_as_uint(T) ^ sgn), dn = fabsf(det);
