@@ -738,11 +738,26 @@ void k_shade(WaveParams P, uint32_t bounce) {
       // from it address global memory (else every material, light and texture load is
       // a flat load) and that the shade's stores do not change it
       using CEntry = const __attribute__((address_space(4))) BatchEntry;
-      const CEntry& E = ((CEntry*)P.batch)[scene];
-      // the path's pixel and sample, as k_raygen derived them from its segment
-      shade_body<true, false, ROUGH>(P, *(const DevScene*)&E.S, bounce, active, i, p, scene,
-                                     *(const PathOrder*)&E.ord, seg_s[sg], seg_q[sg] + (p - seg_begin[sg]), E.k0,
-                                     E.k1, sh);
+      // Queues keep the chunk's scene-major order, so nearly every wave holds paths of
+      // one segment.  Such a wave reads its scene's record with scalar loads (a
+      // wave-uniform index into constant memory); a wave that straddles segments reads
+      // each lane's record with vector loads.  Both copies run the same queue appends,
+      // so every wave of the workgroup meets the same barriers.
+      const uint32_t sg0 = __builtin_amdgcn_readfirstlane(sg);
+      if (__ballot(active && sg != sg0) == 0ull) {
+        const uint32_t sc0 = __builtin_amdgcn_readfirstlane(seg_scene[sg0]);
+        const CEntry& E = ((CEntry*)P.batch)[sc0];
+        const uint32_t s_begin = __builtin_amdgcn_readfirstlane(seg_s[sg0]);
+        const uint32_t q_first = __builtin_amdgcn_readfirstlane(seg_q[sg0] - seg_begin[sg0]);
+        shade_body<true, false, ROUGH>(P, *(const DevScene*)&E.S, bounce, active, i, p, sc0,
+                                       *(const PathOrder*)&E.ord, s_begin, q_first + p, E.k0, E.k1, sh);
+      } else {
+        const CEntry& E = ((CEntry*)P.batch)[scene];
+        // the path's pixel and sample, as k_raygen derived them from its segment
+        shade_body<true, false, ROUGH>(P, *(const DevScene*)&E.S, bounce, active, i, p, scene,
+                                       *(const PathOrder*)&E.ord, seg_s[sg], seg_q[sg] + (p - seg_begin[sg]), E.k0,
+                                       E.k1, sh);
+      }
     } else {
       uint32_t p = active ? P.qpath[cur][i] : 0u;
       shade_body<false, ALL, ROUGH>(P, P.S, bounce, active, i, p, 0u, P.ord, P.s0, P.q0 + p, P.k0, P.k1, sh);
