@@ -53,6 +53,8 @@ def parse():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-light", action="store_true")
+    ap.add_argument("--no-batch-tune", action="store_true",
+                    help="batch leg: the default knobs (BVH4, stack 16, block 256) instead of the autotuner's")
     ap.add_argument("--calib", action="store_true",
                     help="also run a 256 MiB device-to-device copy (counter calibration for rocprofv3 --pmc)")
     ap.add_argument("--width", type=int, default=4)
@@ -252,10 +254,14 @@ def batch_bench(ctx, args):
     t_tune = time.time()
     all_opts = [abi.render_opts(res, res, spp=16, seed=500 + k, max_depth=6, rr_depth=3, scene_id=k)
                 for k, (_, res) in enumerate(batch)]
-    scene_knobs, classes = autotune.autotune_batch_classes(
-        ctx, [sc for sc, _ in batch], all_opts, bounds=(20000, 150000), per_class=8,
-        shape_space=[(4, 4), (8, 4), (2, 4)],
-        launch_space=[("stack_lds", (8, 16, 32)), ("block_size", (64, 128, 256))])
+    if args.no_batch_tune:
+        fixed = dict(autotune.DEFAULTS, block_size=256)
+        scene_knobs, classes = [fixed] * len(batch), [((0, float("inf")), list(range(len(batch))), fixed, [])]
+    else:
+        scene_knobs, classes = autotune.autotune_batch_classes(
+            ctx, [sc for sc, _ in batch], all_opts, bounds=(20000, 150000), per_class=8,
+            shape_space=[(4, 4), (8, 4), (2, 4)],
+            launch_space=[("stack_lds", (8, 16, 32)), ("block_size", (64, 128, 256))])
     t_tune = time.time() - t_tune
     for (lo, hi), idx, best, table in classes:
         log(f"batch autotune, {len(idx)} scenes of {lo}-{hi} triangles: {best} ({len(table)} candidates)")
