@@ -16,6 +16,7 @@ DIRECT_MIS, DIRECT_BSDF, DIRECT_LIGHT = range(3)
 BUF_DEVICE = 1
 RENDER_PROFILE = 2
 BATCH_NO_GRAPH = 4
+ADJOINT_MEGAKERNEL = 16
 TRACE_BOUNDED = 8
 
 f3 = C.c_float * 3

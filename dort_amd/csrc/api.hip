@@ -134,6 +134,21 @@ struct dgpu_ctx {
   }
   int shade_grid_cache[2][2][2] = {};  // [batch][sample_all][rough]
 
+  // Workgroups of the adjoint replay's strided shade kernel.
+  uint32_t shade_blocks_adjoint(bool all, bool rough) {
+    int& cached = shade_grid_adj_cache[all][rough];
+    if (cached == 0) {
+      int nb = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, shade_kernel_ptr(false, all, rough, true), SHADE_BLOCK,
+                                                       0) != hipSuccess ||
+          nb <= 0)
+        nb = 2;
+      cached = nb * num_cus * 4;
+    }
+    return (uint32_t)cached;
+  }
+  int shade_grid_adj_cache[2][2] = {};  // [sample_all][rough]
+
   uint32_t grid_for(int which, const KernelCfg& c) {
     uint64_t key = ((uint64_t)which << 48) | ((uint64_t)c.width << 32) | ((uint64_t)c.lds << 16) | c.block;
     auto it = grid_cache.find(key);
@@ -1168,6 +1183,7 @@ dgpu_status dgpu_adjoint(dgpu_scene* s, const dgpu_render_opts* o, const float* 
   if (o->sample_end <= o->sample_begin) return fail(DGPU_EINVAL, "empty sample range");
   if (o->max_depth + 2 >= MAX_BOUNCES) return fail(DGPU_EINVAL, "max_depth must be < %u", MAX_BOUNCES - 2);
   if (o->direct_strategy > DGPU_DIRECT_LIGHT) return fail(DGPU_EINVAL, "bad direct_strategy");
+  if ((uint64_t)o->x_res * o->y_res > 0xFFFFFFFFull || o->x_res >= (1u << 28)) return fail(DGPU_EINVAL, "film too large");
   if (o->sample_all_lights && s->S.num_lights > MAX_SLOT_LIGHTS)
     return fail(DGPU_EUNSUPPORTED, "sample_all_lights supports at most %u lights", MAX_SLOT_LIGHTS);
   if (d_tex && s->tex_info_host.empty()) return fail(DGPU_EINVAL, "scene has no texture 0");
@@ -1182,24 +1198,13 @@ dgpu_status dgpu_adjoint(dgpu_scene* s, const dgpu_render_opts* o, const float* 
   const uint64_t total = (uint64_t)npix * spp;
   const size_t texels = s->tex_info_host.empty() ? 0 : (size_t)s->tex_info_host[0].x * s->tex_info_host[0].y;
   const uint32_t nm = s->S.num_mats;
-  bool dev = o->flags & DGPU_BUF_DEVICE;
-  WaveParams P;
-  memset(&P, 0, sizeof P);
-  P.S = s->S;
-  P.xr = o->x_res; P.yr = o->y_res; P.npix = npix;
-  P.min_depth = o->min_depth; P.max_depth = o->max_depth; P.rr_depth = o->rr_depth;
-  P.only_direct = o->only_direct; P.sample_all = o->sample_all_lights; P.strategy = o->direct_strategy;
-  P.slots = 2;
-  dgpu_rng_key(o->seed, o->scene_id, &P.k0, &P.k1);
-  P.sample_begin = o->sample_begin;
-  uint32_t chunk = (uint32_t)std::min<uint64_t>(total, ADJ_CHUNK);
-  size_t threads = ((size_t)chunk + 63) / 64 * 64;
-  P.spill = (uint32_t*)ctx->buf("adj_spill", threads * s->stack_cap * 4, &st);
-  AdjArgs A;
-  memset(&A, 0, sizeof A);
-  A.inv_spp = 1.f / float(spp);
+  const bool dev = o->flags & DGPU_BUF_DEVICE;
+  const bool mega = o->flags & DGPU_ADJOINT_MEGAKERNEL;
+  // gradient, dL/dI and image buffers on the device (the caller's, or staging)
+  const float* a_dldi;
+  float *a_tex, *a_rough, *a_img;
   if (dev) {
-    A.dldi = dldi; A.g_tex = d_tex; A.g_rough = d_rough; A.image = image_out;
+    a_dldi = dldi; a_tex = d_tex; a_rough = d_rough; a_img = image_out;
   } else {
     float* b_dldi = (float*)ctx->buf("adj_dldi", (size_t)npix * 12, &st);
     float* b_tex = d_tex ? (float*)ctx->buf("adj_gtex", texels * 12, &st) : nullptr;
@@ -1210,21 +1215,113 @@ dgpu_status dgpu_adjoint(dgpu_scene* s, const dgpu_render_opts* o, const float* 
     if (b_tex) HIP_TRY(hipMemsetAsync(b_tex, 0, texels * 12, stream));
     if (b_rough) HIP_TRY(hipMemsetAsync(b_rough, 0, (size_t)nm * 4, stream));
     if (b_img) HIP_TRY(hipMemsetAsync(b_img, 0, (size_t)npix * 12, stream));
-    A.dldi = b_dldi; A.g_tex = b_tex; A.g_rough = b_rough; A.image = b_img;
+    a_dldi = b_dldi; a_tex = b_tex; a_rough = b_rough; a_img = b_img;
   }
-  if (st != DGPU_OK) return st;
-  HIP_TRY(hipEventRecord(ctx->ev0, stream));
-  for (uint64_t g0 = 0; g0 < total; g0 += chunk) {
-    P.g0 = g0;
-    P.n_chunk = (uint32_t)std::min<uint64_t>(chunk, total - g0);
-    HIP_TRY(launch_adjoint(P, A, s->S.width, stream));
+  WaveParams P;
+  memset(&P, 0, sizeof P);
+  P.S = s->S;
+  P.xr = o->x_res; P.yr = o->y_res; P.npix = npix;
+  P.ord = make_path_order(o->x_res, o->y_res);
+  P.min_depth = o->min_depth; P.max_depth = o->max_depth; P.rr_depth = o->rr_depth;
+  P.only_direct = o->only_direct; P.sample_all = o->sample_all_lights; P.strategy = o->direct_strategy;
+  P.rough = s->S.has_rough;
+  dgpu_rng_key(o->seed, o->scene_id, &P.k0, &P.k1);
+  P.sample_begin = o->sample_begin;
+  if (mega) {
+    // one thread per path: trace it, then replay it with the deposits (k_adjoint)
+    P.slots = 2;
+    uint32_t chunk = (uint32_t)std::min<uint64_t>(total, ADJ_CHUNK);
+    size_t threads = ((size_t)chunk + 63) / 64 * 64;
+    P.spill = (uint32_t*)ctx->buf("adj_spill", threads * s->stack_cap * 4, &st);
+    if (st != DGPU_OK) return st;
+    AdjArgs A;
+    memset(&A, 0, sizeof A);
+    A.inv_spp = 1.f / float(spp);
+    A.dldi = a_dldi; A.g_tex = a_tex; A.g_rough = a_rough; A.image = a_img;
+    HIP_TRY(hipEventRecord(ctx->ev0, stream));
+    for (uint64_t g0 = 0; g0 < total; g0 += chunk) {
+      P.g0 = g0;
+      P.n_chunk = (uint32_t)std::min<uint64_t>(chunk, total - g0);
+      HIP_TRY(launch_adjoint(P, A, s->S.width, stream));
+    }
+    HIP_TRY(hipEventRecord(ctx->ev1, stream));
+  } else {
+    // Wavefront replay.  Per chunk, pass 1 renders the paths (the path tracer's kernels)
+    // and keeps each path's radiance; pass 2 renders the same paths again (the same
+    // counter-based random numbers) with the adjoint shade, whose lobes leave gradient
+    // payloads that are deposited once the radiance they carry is known: a shadow ray's
+    // at the next bounce (or after the last), a continuation's at the next bounce as the
+    // first-pass radiance less what the path has gathered by then.
+    const uint32_t slots = o->sample_all_lights ? 2 * std::max(1u, s->S.num_lights) : 2;
+    P.slots = slots;
+    const uint64_t per_path = 120ull + 56ull * slots + 32ull + 16ull * slots;  // + first-pass radiance, payloads
+    uint64_t tile = s->queue_tile;
+    if (!tile) {
+      size_t free_b = 0, total_b = 0;
+      if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
+      tile = std::max<uint64_t>(1ull << 20, std::min<uint64_t>(16ull << 30, free_b / 4) / per_path);
+    }
+    const uint32_t C = (uint32_t)std::min<uint64_t>(
+        std::min<uint64_t>(total, tile),
+        std::min<uint64_t>(0xFFFFFFFFull / slots, std::max<uint64_t>(1, 0xFFFFFFFFull - npix)));
+    const uint32_t shade_grid = ctx->shade_blocks(false, o->sample_all_lights != 0, s->S.has_rough != 0);
+    const uint32_t shade_grid_adj = ctx->shade_blocks_adjoint(o->sample_all_lights != 0, s->S.has_rough != 0);
+    KernelCfg ce = s->cfg, cc = s->cfg;
+    ce.grid = ctx->grid_for(2, ce);
+    cc.grid = ctx->grid_for(3, cc);
+    uint32_t maxgrid = std::max(ce.grid, cc.grid);
+    P.thr = (float4*)ctx->buf("thr", (size_t)C * 16, &st);
+    P.rad = (float4*)ctx->buf("rad", (size_t)C * 16, &st);
+    P.pend = (float4*)ctx->buf("pend", (size_t)C * slots * 16, &st);
+    P.qo[0] = (float4*)ctx->buf("qo0", (size_t)C * 16, &st);
+    P.qo[1] = (float4*)ctx->buf("qo1", (size_t)C * 16, &st);
+    P.qd[0] = (float4*)ctx->buf("qd0", (size_t)C * 16, &st);
+    P.qd[1] = (float4*)ctx->buf("qd1", (size_t)C * 16, &st);
+    P.qpath[0] = (uint32_t*)ctx->buf("qp0", (size_t)C * 4, &st);
+    P.qpath[1] = (uint32_t*)ctx->buf("qp1", (size_t)C * 4, &st);
+    P.hit = (float4*)ctx->buf("hit", (size_t)C * 16, &st);
+    P.so = (float4*)ctx->buf("so", (size_t)C * slots * 16, &st);
+    P.sd = (float4*)ctx->buf("sd", (size_t)C * slots * 16, &st);
+    P.smeta = (uint2*)ctx->buf("smeta", (size_t)C * slots * 8, &st);
+    P.counters = (uint32_t*)ctx->buf("counters", NUM_COUNTERS * 4, &st);
+    P.stats = (unsigned long long*)ctx->buf("stats", STATS_WORDS * 8, &st);
+    P.spill = (uint32_t*)ctx->buf("spill", (size_t)maxgrid * s->cfg.block * s->stack_cap * 4, &st);
+    P.stack_cap = s->stack_cap;
+    P.adj_L = (float4*)ctx->buf("adj_L", (size_t)C * 16, &st);
+    P.adj_pend = (float4*)ctx->buf("adj_pend", (size_t)C * slots * 16, &st);
+    P.adj_cont = (float4*)ctx->buf("adj_cont", (size_t)C * 16, &st);
+    if (st != DGPU_OK) return st;
+    P.adj_dldi = a_dldi;
+    P.adj_g_tex = a_tex;
+    P.adj_g_rough = a_rough;
+    P.adj_image = a_img;
+    P.adj_inv_spp = 1.f / float(spp);
+    HIP_TRY(hipMemsetAsync(P.stats, 0, STATS_WORDS * 8, stream));
+    HIP_TRY(hipEventRecord(ctx->ev0, stream));
+    for (uint64_t g0 = 0; g0 < total; g0 += C) {
+      P.g0 = g0;
+      P.n_chunk = (uint32_t)std::min<uint64_t>(C, total - g0);
+      P.q0 = (uint32_t)(g0 % npix);
+      P.s0 = o->sample_begin + (uint32_t)(g0 / npix);
+      for (int pass = 0; pass < 2; ++pass) {
+        HIP_TRY(launch_raygen(P, P.n_chunk, false, stream));
+        for (uint32_t b = 0; b <= o->max_depth; ++b) {
+          HIP_TRY(launch_extend(P, ce, b, false, stream));
+          if (pass == 0) HIP_TRY(launch_shade(P, P.n_chunk, b, false, stream, shade_grid));
+          else HIP_TRY(launch_shade_adjoint(P, P.n_chunk, b, stream, shade_grid_adj));
+          HIP_TRY(launch_connect(P, cc, b, false, stream));
+        }
+        if (pass == 0) HIP_TRY(launch_adj_total(P, stream));
+        else HIP_TRY(launch_adj_final(P, stream));
+      }
+    }
+    HIP_TRY(hipEventRecord(ctx->ev1, stream));
   }
-  HIP_TRY(hipEventRecord(ctx->ev1, stream));
   std::vector<float> htex, hrough, himg;
   if (!dev) {
-    if (d_tex) { htex.resize(texels * 3); HIP_TRY(hipMemcpyAsync(htex.data(), A.g_tex, texels * 12, hipMemcpyDeviceToHost, stream)); }
-    if (d_rough) { hrough.resize(nm); HIP_TRY(hipMemcpyAsync(hrough.data(), A.g_rough, (size_t)nm * 4, hipMemcpyDeviceToHost, stream)); }
-    if (image_out) { himg.resize((size_t)npix * 3); HIP_TRY(hipMemcpyAsync(himg.data(), A.image, (size_t)npix * 12, hipMemcpyDeviceToHost, stream)); }
+    if (d_tex) { htex.resize(texels * 3); HIP_TRY(hipMemcpyAsync(htex.data(), a_tex, texels * 12, hipMemcpyDeviceToHost, stream)); }
+    if (d_rough) { hrough.resize(nm); HIP_TRY(hipMemcpyAsync(hrough.data(), a_rough, (size_t)nm * 4, hipMemcpyDeviceToHost, stream)); }
+    if (image_out) { himg.resize((size_t)npix * 3); HIP_TRY(hipMemcpyAsync(himg.data(), a_img, (size_t)npix * 12, hipMemcpyDeviceToHost, stream)); }
   }
   HIP_TRY(hipStreamSynchronize(stream));
   if (!dev) {

@@ -112,6 +112,15 @@ struct WaveParams {
   // light tracing (dgpu_render_light)
   uint32_t min_len, max_len;  // path lengths in vertices (LightRenderer min_length / max_length)
   float2* fpos;            // film position of each camera-connection shadow ray
+  // adjoint replay (dgpu_adjoint)
+  float4* adj_L;           // per path: the first pass's radiance; w = 1 if the sample counts
+  float4* adj_pend;        // per NEE slot: gradient payload of the pending shadow ray
+  float4* adj_cont;        // per path: gradient payload of the last continuation
+  const float* adj_dldi;   // dL/dI per pixel (rgb)
+  float* adj_g_tex;        // texture-0 gradient, texels * 3 (or null)
+  float* adj_g_rough;      // per-material roughness gradient (or null)
+  float* adj_image;        // += per-pixel mean radiance of the first pass (or null)
+  float adj_inv_spp;
 };
 
 struct AdjArgs {
@@ -140,7 +149,7 @@ bool variant_supported(uint32_t width, uint32_t lds, uint32_t block);
 // which: 0 closest-hit API, 1 any-hit API, 2 extend, 3 connect, 4/5 batched extend/connect
 // (the bounded API variants share the occupancy of 0/1)
 const void* trace_kernel_ptr(int which, uint32_t width, uint32_t lds, uint32_t block);
-const void* shade_kernel_ptr(bool batch, bool all, bool rough);
+const void* shade_kernel_ptr(bool batch, bool all, bool rough, bool adj = false);
 
 // bounded: the ray origins lie within the range the box padding covers (fast box test)
 hipError_t launch_trace_api(const DevScene& S, const KernelCfg& cfg, const dgpu_rays& rays,
@@ -154,6 +163,12 @@ hipError_t launch_extend(const WaveParams& P, const KernelCfg& cfg, uint32_t bou
 // max_blocks: the shade grid (workgroups of 256 that stride over the queue)
 hipError_t launch_shade(const WaveParams& P, uint32_t capacity, uint32_t bounce, bool batch, hipStream_t st,
                         uint32_t max_blocks);
+// The adjoint replay's shade (k_shade with gradient deposits), the first pass's per-path
+// radiance (k_adj_total, per pixel) and the last bounce's pending deposits (k_adj_final).
+hipError_t launch_shade_adjoint(const WaveParams& P, uint32_t capacity, uint32_t bounce, hipStream_t st,
+                                uint32_t max_blocks);
+hipError_t launch_adj_total(const WaveParams& P, hipStream_t st);
+hipError_t launch_adj_final(const WaveParams& P, hipStream_t st);
 hipError_t launch_connect(const WaveParams& P, const KernelCfg& cfg, uint32_t bounce, bool batch,
                           hipStream_t st);
 hipError_t launch_finalize(const WaveParams& P, uint32_t capacity, bool batch, hipStream_t st);

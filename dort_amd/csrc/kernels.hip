@@ -529,15 +529,61 @@ __device__ __forceinline__ void emit_shadow_batch(const WaveParams& P, const Nee
   P.smeta[qi] = make_uint2(path_word, (s.ignore & 0x7FFFFFFFu) | (k << 31));
 }
 
+// ----------------------------------------------------------------- adjoint payloads
+
+// Gradient deposits of the adjoint replay (dgpu_adjoint), per lobe where a path scatters:
+//   d loss / d theta += dldi . (d ln f / d theta) * (radiance carried by f)
+// for the albedo texels of texture 0 (Lambert: d ln f_c / d albedo_c = 1 / albedo_c) and
+// a GGX conductor's roughness.  A payload is a 16-byte record (x, y, z, word) with word =
+// kind << 30 | index: ADJ_TEXEL: xyz = 1 / albedo per channel (0 where the albedo is 0),
+// index the texel; ADJ_ROUGH: x = d ln f / d alpha at (wi, wo), index the material.  The
+// radiance a lobe carries is known only later (a shadow ray's visibility, the rest of the
+// path), so payloads wait in the path's records until it is.
+constexpr uint32_t ADJ_NONE = 0u, ADJ_TEXEL = 1u, ADJ_ROUGH = 2u;
+constexpr uint32_t ADJ_LDS_MATS = 64;  // roughness partial sums per workgroup in LDS
+
+__device__ __forceinline__ float4 adj_payload(const WaveParams& P, const Bsdf& b, int32_t texel, uint32_t mat,
+                                              V3 wi, V3 wo) {
+  if (b.kind == DGPU_MAT_LAMBERT && texel >= 0 && P.adj_g_tex) {
+    V3 a = b.albedo();
+    return make_float4(a.x > 0.f ? 1.f / a.x : 0.f, a.y > 0.f ? 1.f / a.y : 0.f, a.z > 0.f ? 1.f / a.z : 0.f,
+                       __uint_as_float((ADJ_TEXEL << 30) | (uint32_t)texel));
+  }
+  if (b.kind == DGPU_MAT_GGX_CONDUCTOR && P.adj_g_rough) {
+    float r = conductor_dlnf_dalpha(b, b.to_local(wi), b.to_local(wo));
+    return make_float4(r, 0.f, 0.f, __uint_as_float((ADJ_ROUGH << 30) | mat));
+  }
+  return make_float4(0.f, 0.f, 0.f, __uint_as_float(ADJ_NONE << 30));
+}
+
+// Deposits payload pl for carried radiance c at pixel weight dl (dL/dI of the path's pixel
+// divided by the sample count).  Roughness sums of the first ADJ_LDS_MATS materials go to
+// the workgroup's LDS partials s_rough.
+__device__ __forceinline__ void adj_apply(const WaveParams& P, float4 pl, V3 c, const float dl[3], float* s_rough) {
+  const uint32_t word = __float_as_uint(pl.w), kind = word >> 30, idx = word & 0x3FFFFFFFu;
+  if (kind == ADJ_TEXEL) {
+    float* g = P.adj_g_tex + 3 * (size_t)idx;
+    if (pl.x != 0.f) atomicAdd(g + 0, dl[0] * c.x * pl.x);
+    if (pl.y != 0.f) atomicAdd(g + 1, dl[1] * c.y * pl.y);
+    if (pl.z != 0.f) atomicAdd(g + 2, dl[2] * c.z * pl.z);
+  } else if (kind == ADJ_ROUGH) {
+    float v = pl.x * (dl[0] * c.x + dl[1] * c.y + dl[2] * c.z);
+    if (idx < ADJ_LDS_MATS) atomicAdd(s_rough + idx, v);
+    else atomicAdd(P.adj_g_rough + idx, v);
+  }
+}
+
 // One bounce of PathRenderer::sample for every path in the current queue.
 // Every lane of a wave runs the same number of queue appends (inactive lanes
 // append nothing), so the ballot-based compaction sees the whole wave.
 // sh: the workgroup's LDS words for block_append.
-template <bool BATCH, bool ALL, bool ROUGH>
+// ADJ: the adjoint replay's second pass (non-batched): the same path, plus the gradient
+// payloads of its lobes and their deposits once the radiance they carry is known.
+template <bool BATCH, bool ALL, bool ROUGH, bool ADJ = false>
 __device__ __forceinline__ void shade_body(const WaveParams& P, const DevScene& S, uint32_t bounce,
                                            bool active, uint32_t i, uint32_t p, uint32_t scene,
                                            const PathOrder& ord, uint32_t s0, uint32_t t, uint32_t k0,
-                                           uint32_t k1, uint32_t* sh) {
+                                           uint32_t k1, uint32_t* sh, float* s_rough = nullptr) {
   const uint32_t cur = bounce & 1u, nxt = cur ^ 1u;
   const uint32_t SL = P.slots;
   uint32_t flags = 0, bounces = 0;
@@ -550,8 +596,20 @@ __device__ __forceinline__ void shade_body(const WaveParams& P, const DevScene& 
   // the path, bounce and block), so fewer of them are live across the NEE code
   float rb0[4] = {0.f, 0.f, 0.f, 0.f}, rb1[4] = {0.f, 0.f, 0.f, 0.f}, rb2[4] = {0.f, 0.f, 0.f, 0.f};
   // the path's sample and pixel (position t from the start of sample s0), needed only by
-  // a path that scatters here
+  // a path that scatters here (and by the adjoint's deposits)
   uint32_t sample = 0, pixel = 0;
+  // adjoint: the path's first-pass radiance, whether its sample counts, and its pixel weight
+  float4 adj_l = make_float4(0.f, 0.f, 0.f, 0.f);
+  float dl[3] = {0.f, 0.f, 0.f};
+  int32_t texel = -1;
+  if (ADJ && active) {
+    order_sample_pixel(ord, s0, t, sample, pixel);
+    adj_l = P.adj_L[p];
+    dl[0] = P.adj_dldi[3 * (size_t)pixel] * P.adj_inv_spp;
+    dl[1] = P.adj_dldi[3 * (size_t)pixel + 1] * P.adj_inv_spp;
+    dl[2] = P.adj_dldi[3 * (size_t)pixel + 2] * P.adj_inv_spp;
+  }
+  const bool adj_on = ADJ && adj_l.w != 0.f;
   if (active) {
     if (bounce == 0u) {
       // the path starts here (k_raygen writes only the queue)
@@ -568,8 +626,15 @@ __device__ __forceinline__ void shade_body(const WaveParams& P, const DevScene& 
       for (uint32_t k = 0; k < SL; ++k) {
         size_t s = (size_t)p * SL + k;
         float4 pe = P.pend[s];
-        if (pe.w != 0.f) { rad = rad + xyz(pe); P.pend[s].w = 0.f; }
+        if (pe.w != 0.f) {
+          rad = rad + xyz(pe);
+          P.pend[s].w = 0.f;
+          if (adj_on) adj_apply(P, P.adj_pend[s], xyz(pe), dl, s_rough);
+        }
       }
+      // adjoint: the previous vertex's continuation carried everything the path gathers
+      // from here on: its first-pass radiance less what it has gathered so far
+      if (adj_on) adj_apply(P, P.adj_cont[p], xyz(adj_l) - rad, dl, s_rough);
     }
     bounces = flags & F_BOUNCES;
     bool last_delta = (flags & F_DELTA) != 0u;
@@ -591,8 +656,8 @@ __device__ __forceinline__ void shade_body(const WaveParams& P, const DevScene& 
     if (is && bounces <= P.max_depth) {
       shading = true;
       wo = normalize(-rd);
-      bsdf = make_bsdf(S, g);
-      order_sample_pixel(ord, s0, t, sample, pixel);
+      bsdf = make_bsdf(S, g, ADJ ? &texel : nullptr);
+      if (!ADJ) order_sample_pixel(ord, s0, t, sample, pixel);
       dgpu_rng_block(k0, k1, pixel, sample, bounce, 0, rb0);
     }
   }
@@ -632,6 +697,7 @@ __device__ __forceinline__ void shade_body(const WaveParams& P, const DevScene& 
       uint32_t qb = block_append(&P.counters[CNT_SHADOW + bounce], nb.emit, sh);
       if (nb.emit) {
         P.pend[slot_b] = make_float4(nb.contrib.x, nb.contrib.y, nb.contrib.z, 1.f);
+        if (ADJ) P.adj_pend[slot_b] = adj_payload(P, bsdf, texel, g.mat, nb.d, wo);
         if (BATCH) emit_shadow_batch(P, nb, p, 0u, qb);
         else emit_shadow(P, nb, slot_b, qb);
       } else if (active && bounce == 0u) {
@@ -648,6 +714,7 @@ __device__ __forceinline__ void shade_body(const WaveParams& P, const DevScene& 
       uint32_t ql = block_append(&P.counters[CNT_SHADOW + bounce], nl.emit, sh);
       if (nl.emit) {
         P.pend[slot_l] = make_float4(nl.contrib.x, nl.contrib.y, nl.contrib.z, 1.f);
+        if (ADJ) P.adj_pend[slot_l] = adj_payload(P, bsdf, texel, g.mat, nl.d, wo);
         if (BATCH) emit_shadow_batch(P, nl, p, 1u, ql);
         else emit_shadow(P, nl, slot_l, ql);
       } else if (active && bounce == 0u) {
@@ -681,6 +748,10 @@ __device__ __forceinline__ void shade_body(const WaveParams& P, const DevScene& 
         cont = true;
         next_o = g.p; next_d = wi; next_eps = g.eps;
         flags = (flags & ~(F_BOUNCES | F_DELTA)) | bounces | (delta ? F_DELTA : 0u);
+        // adjoint: a delta lobe has no parameter here; a path that ends here carries
+        // nothing further, so only continuing paths keep a payload
+        if (ADJ) P.adj_cont[p] = delta ? make_float4(0.f, 0.f, 0.f, __uint_as_float(ADJ_NONE << 30))
+                                       : adj_payload(P, bsdf, texel, g.mat, wi, wo);
       }
     }
   }
@@ -704,10 +775,15 @@ __device__ __forceinline__ void shade_body(const WaveParams& P, const DevScene& 
 #endif
 // ALL: sample_all_lights (one NEE per light and vertex) instead of one light picked by power.
 // ROUGH: with the rough dielectric's lobe (scenes that have one).
-template <bool BATCH, bool ALL, bool ROUGH>
+template <bool BATCH, bool ALL, bool ROUGH, bool ADJ = false>
 __global__ __launch_bounds__(SHADE_BLOCK) __attribute__((amdgpu_waves_per_eu(DGPU_SHADE_WAVES)))
 void k_shade(WaveParams P, uint32_t bounce) {
   __shared__ uint32_t sh[SHADE_BLOCK / 64 + 1];
+  __shared__ float s_rough[ADJ ? ADJ_LDS_MATS : 1];
+  if (ADJ) {
+    for (uint32_t k = threadIdx.x; k < ADJ_LDS_MATS; k += SHADE_BLOCK) s_rough[k] = 0.f;
+    __syncthreads();
+  }
   __shared__ uint32_t seg_begin[BATCH ? MAX_CHUNK_SEGS : 1], seg_scene[BATCH ? MAX_CHUNK_SEGS : 1];
   __shared__ uint32_t seg_q[BATCH ? MAX_CHUNK_SEGS : 1], seg_s[BATCH ? MAX_CHUNK_SEGS : 1];
   const uint32_t cur = bounce & 1u;
@@ -760,9 +836,69 @@ void k_shade(WaveParams P, uint32_t bounce) {
       }
     } else {
       uint32_t p = active ? P.qpath[cur][i] : 0u;
-      shade_body<false, ALL, ROUGH>(P, P.S, bounce, active, i, p, 0u, P.ord, P.s0, P.q0 + p, P.k0, P.k1, sh);
+      shade_body<false, ALL, ROUGH, ADJ>(P, P.S, bounce, active, i, p, 0u, P.ord, P.s0, P.q0 + p, P.k0, P.k1, sh,
+                                         s_rough);
     }
   }
+  if (ADJ) {
+    __syncthreads();
+    if (P.adj_g_rough)
+      for (uint32_t k = threadIdx.x; k < ADJ_LDS_MATS && k < P.S.num_mats; k += SHADE_BLOCK)
+        if (s_rough[k] != 0.f) atomicAdd(P.adj_g_rough + k, s_rough[k]);
+  }
+}
+
+// Adjoint, first pass: each path's radiance as the film would receive it (rad plus its
+// visible pending NEE), with w = 1 when the sample counts (finite, non-negative, not on a
+// pixel's lower edge); per pixel in sample order, which also gives the optional image
+// (the per-pixel mean) without atomics.
+__global__ __launch_bounds__(256) void k_adj_total(WaveParams P) {
+  uint32_t px = blockIdx.x * blockDim.x + threadIdx.x;
+  if (px >= P.npix) return;
+  uint64_t g_end = P.g0 + P.n_chunk;
+  uint64_t g = P.g0 + (((uint64_t)tile_pos(px, P.xr, P.yr) + P.npix - (uint32_t)(P.g0 % P.npix)) % P.npix);
+  V3 sum = v3(0.f, 0.f, 0.f);
+  for (; g < g_end; g += P.npix) {
+    uint32_t slot = (uint32_t)(g - P.g0);
+    V3 rad = xyz(P.rad[slot]);
+    for (uint32_t k = 0; k < P.slots; ++k) {
+      float4 pe = P.pend[(size_t)slot * P.slots + k];
+      if (pe.w != 0.f) rad = rad + xyz(pe);
+    }
+    bool ok = isfinite(rad.x) && isfinite(rad.y) && isfinite(rad.z) && rad.x >= 0.f && rad.y >= 0.f &&
+              rad.z >= 0.f;
+    bool counts = ok && !jitter_on_edge(P.k0, P.k1, px, P.sample_begin + (uint32_t)(g / P.npix));
+    P.adj_L[slot] = make_float4(rad.x, rad.y, rad.z, counts ? 1.f : 0.f);
+    if (counts) sum = sum + rad;
+  }
+  if (P.adj_image) {
+    P.adj_image[3 * (size_t)px] += sum.x * P.adj_inv_spp;
+    P.adj_image[3 * (size_t)px + 1] += sum.y * P.adj_inv_spp;
+    P.adj_image[3 * (size_t)px + 2] += sum.z * P.adj_inv_spp;
+  }
+}
+
+// Adjoint, second pass: the deposits of the NEE still pending after the last bounce.
+__global__ __launch_bounds__(256) void k_adj_final(WaveParams P) {
+  __shared__ float s_rough[ADJ_LDS_MATS];
+  for (uint32_t k = threadIdx.x; k < ADJ_LDS_MATS; k += 256) s_rough[k] = 0.f;
+  __syncthreads();
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < P.n_chunk && P.adj_L[i].w != 0.f) {
+    uint32_t sample, pixel;
+    order_sample_pixel(P.ord, P.s0, P.q0 + i, sample, pixel);
+    float dl[3] = {P.adj_dldi[3 * (size_t)pixel] * P.adj_inv_spp, P.adj_dldi[3 * (size_t)pixel + 1] * P.adj_inv_spp,
+                   P.adj_dldi[3 * (size_t)pixel + 2] * P.adj_inv_spp};
+    for (uint32_t k = 0; k < P.slots; ++k) {
+      size_t s = (size_t)i * P.slots + k;
+      float4 pe = P.pend[s];
+      if (pe.w != 0.f) adj_apply(P, P.adj_pend[s], xyz(pe), dl, s_rough);
+    }
+  }
+  __syncthreads();
+  if (P.adj_g_rough)
+    for (uint32_t k = threadIdx.x; k < ADJ_LDS_MATS && k < P.S.num_mats; k += 256)
+      if (s_rough[k] != 0.f) atomicAdd(P.adj_g_rough + k, s_rough[k]);
 }
 
 // Film::add_sample with the r = 0.5 box filter: each sample lands on one pixel with
@@ -872,7 +1008,6 @@ struct AdjSink {
   uint32_t num_mats;
 };
 
-constexpr uint32_t ADJ_LDS_MATS = 64;
 constexpr uint32_t ADJ_BLOCK = 64;
 
 __device__ __forceinline__ void adj_deposit(const AdjSink& k, const Bsdf& b, int32_t texel, uint32_t mat,
@@ -1343,7 +1478,11 @@ hipError_t launch_raygen(const WaveParams& P, uint32_t capacity, bool batch, hip
   return hipGetLastError();
 }
 
-const void* shade_kernel_ptr(bool batch, bool all, bool rough) {
+const void* shade_kernel_ptr(bool batch, bool all, bool rough, bool adj) {
+  if (adj) {
+    if (all) return rough ? (const void*)&k_shade<false, true, true, true> : (const void*)&k_shade<false, true, false, true>;
+    return rough ? (const void*)&k_shade<false, false, true, true> : (const void*)&k_shade<false, false, false, true>;
+  }
   if (batch) return rough ? (const void*)&k_shade<true, false, true> : (const void*)&k_shade<true, false, false>;
   if (all) return rough ? (const void*)&k_shade<false, true, true> : (const void*)&k_shade<false, true, false>;
   return rough ? (const void*)&k_shade<false, false, true> : (const void*)&k_shade<false, false, false>;
@@ -1358,6 +1497,27 @@ hipError_t launch_shade(const WaveParams& P, uint32_t capacity, uint32_t bounce,
   else if (P.sample_all) { if (rough) DG_SHADE(false, true, true); else DG_SHADE(false, true, false); }
   else { if (rough) DG_SHADE(false, false, true); else DG_SHADE(false, false, false); }
 #undef DG_SHADE
+  return hipGetLastError();
+}
+
+hipError_t launch_shade_adjoint(const WaveParams& P, uint32_t capacity, uint32_t bounce, hipStream_t st,
+                                uint32_t max_blocks) {
+  uint32_t blocks = std::min((capacity + SHADE_BLOCK - 1u) / SHADE_BLOCK, std::max(1u, max_blocks));
+#define DG_SHADE(A, R) hipLaunchKernelGGL((k_shade<false, A, R, true>), dim3(blocks), dim3(SHADE_BLOCK), 0, st, P, bounce)
+  const bool rough = P.rough != 0u;
+  if (P.sample_all) { if (rough) DG_SHADE(true, true); else DG_SHADE(true, false); }
+  else { if (rough) DG_SHADE(false, true); else DG_SHADE(false, false); }
+#undef DG_SHADE
+  return hipGetLastError();
+}
+
+hipError_t launch_adj_total(const WaveParams& P, hipStream_t st) {
+  hipLaunchKernelGGL(k_adj_total, dim3((P.npix + 255u) / 256u), dim3(256), 0, st, P);
+  return hipGetLastError();
+}
+
+hipError_t launch_adj_final(const WaveParams& P, hipStream_t st) {
+  hipLaunchKernelGGL(k_adj_final, dim3((P.n_chunk + 255u) / 256u), dim3(256), 0, st, P);
   return hipGetLastError();
 }
 

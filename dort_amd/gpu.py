@@ -387,9 +387,13 @@ class Scene:
     def update_material(self, index, material):
         check(lib().dgpu_scene_update_material(self.h, index, C.byref(material)))
 
-    def adjoint(self, opts, dldi, want_texture=True, want_roughness=True, want_image=False):
-        """Gradient of sum(dldi * image) w.r.t. texture-0 texels and material roughness.
+    def adjoint(self, opts, dldi, want_texture=True, want_roughness=True, want_image=False, megakernel=False):
+        """Gradient of sum(dldi * image) w.r.t. texture-0 texels and material roughness, by
+        the two wavefront passes (or one thread per path with megakernel=True).
         Returns (g_tex (h, w, 3) or None, g_rough (num_materials,) or None, image or None, stats)."""
+        if megakernel:
+            opts = abi.RenderOpts.from_buffer_copy(opts)
+            opts.flags |= abi.ADJOINT_MEGAKERNEL
         dldi = np.ascontiguousarray(dldi, dtype=np.float32)
         assert dldi.shape == (opts.y_res, opts.x_res, 3)
         g_tex = None

@@ -162,9 +162,11 @@ enum {
   DGPU_BUF_DEVICE = 1u,     /* buffers are device pointers */
   DGPU_RENDER_PROFILE = 2u, /* dgpu_render: time every kernel launch with HIP events */
   DGPU_BATCH_NO_GRAPH = 4u, /* dgpu_render_batch: launch kernels directly, no hipGraph */
-  DGPU_TRACE_BOUNDED = 8u   /* dgpu_trace_*: every ray origin lies within the scene's range (no
+  DGPU_TRACE_BOUNDED = 8u,  /* dgpu_trace_*: every ray origin lies within the scene's range (no
                                coordinate beyond max |vertex or camera coordinate|), which
                                allows the path tracer's faster box test */
+  DGPU_ADJOINT_MEGAKERNEL = 16u /* dgpu_adjoint: one thread per path traces and replays it in
+                                   one kernel, instead of the two wavefront passes */
 };
 
 /* ---- rendering ----------------------------------------------------------- */
@@ -340,7 +342,9 @@ dgpu_status dgpu_render_batch(dgpu_ctx* ctx, const dgpu_batch_item* items, uint3
  * texels of texture 0 (d_albedo_texels: width*height*3, may be NULL) and every
  * material's GGX roughness (d_roughness: num_materials, may be NULL).  Each path is
  * traced twice from its counter-based random numbers (forward, then replay with
- * gradient deposits); memory stays O(pixels).  Outputs are accumulated (+=).
+ * gradient deposits): by default as two wavefront passes over a chunk of paths (the
+ * first keeps each path's radiance, 16 B per path), with DGPU_ADJOINT_MEGAKERNEL in
+ * opts->flags by one thread per path; nothing is taped.  Outputs are accumulated (+=).
  * With DGPU_BUF_DEVICE in opts->flags all three buffers are device pointers.
  * image_out (optional, same memory kind) receives += the per-pixel mean radiance of
  * the replayed paths. */

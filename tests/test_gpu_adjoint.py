@@ -115,3 +115,34 @@ def test_adjoint_linear_in_dldi(gpu_ctx, scene):
     # the replayed paths are the forward render's paths
     fwd = sc.image(opts)
     assert np.abs(img - fwd).max() <= 1e-3 * max(1.0, np.abs(fwd).max())
+
+
+@pytest.mark.parametrize("sample_all", [0, 1])
+def test_wavefront_adjoint_equals_megakernel(gpu_ctx, scene, sample_all):
+    """The two wavefront passes and the one-thread-per-path kernel replay the same paths
+    and make the same deposits (in another order, so sums agree to float rounding), and
+    the wavefront's image is the per-pixel mean of the first pass."""
+    opts = abi.render_opts(40, 40, spp=6, seed=21, max_depth=6, rr_depth=3)
+    opts.sample_all_lights = sample_all
+    sc = gpu_ctx.scene(scene)
+    d = _dldi(40, 7)
+    w_t, w_r, w_img, _ = sc.adjoint(opts, d, want_image=True)
+    m_t, m_r, m_img, _ = sc.adjoint(opts, d, want_image=True, megakernel=True)
+    print("texel rel", np.linalg.norm(w_t - m_t) / np.linalg.norm(m_t),
+          "rough", np.abs(w_r - m_r).max() / np.abs(m_r).max())
+    assert np.linalg.norm(m_t) > 0 and np.abs(m_r).max() > 0
+    np.testing.assert_allclose(w_t, m_t, rtol=1e-3, atol=1e-6 * np.abs(m_t).max())
+    np.testing.assert_allclose(w_r, m_r, rtol=1e-3, atol=1e-6 * np.abs(m_r).max())
+    np.testing.assert_allclose(w_img, m_img, rtol=1e-4, atol=1e-6)
+    fwd = sc.image(opts)
+    np.testing.assert_allclose(w_img, fwd, rtol=1e-4, atol=1e-6)
+
+
+def test_wavefront_adjoint_chunks(gpu_ctx, scene):
+    """Gradients do not depend on how the paths are cut into wavefront chunks."""
+    opts = abi.render_opts(24, 20, spp=5, seed=33, max_depth=6, rr_depth=3)
+    d = _dldi(24, 8)[:20]
+    a_t, a_r, _, _ = gpu_ctx.scene(scene).adjoint(opts, d)
+    b_t, b_r, _, _ = gpu_ctx.scene(scene, queue_tile=777).adjoint(opts, d)
+    np.testing.assert_allclose(b_t, a_t, rtol=1e-4, atol=1e-6 * np.abs(a_t).max())
+    np.testing.assert_allclose(b_r, a_r, rtol=1e-4, atol=1e-6 * np.abs(a_r).max())
