@@ -144,11 +144,13 @@ __device__ __forceinline__ void trace_batch_refill(IO& io, uint32_t n, uint32_t 
 // are compacted in order), so the lanes of a wave fetch largely the same nodes and
 // triangles.  Any-hit rays refill idle lanes from the wave's batch (DGPU_REFILL_ANY):
 // shadow rays end at their first hit, so a wave's rays differ most in length, and
-// refilling cut the terrain render's connect time by a fifth.  Closest-hit rays are
-// refilled after the first bounce (DGPU_REFILL_LATE), which cut extend by an eighth;
-// the camera rays of bounce 0 trace each 64-ray part to its end (refilled, they are
-// coherent enough that mixing traversal stages cost more than the idle lanes).  The
-// threshold is a kernel argument.  Refilling from the global queue instead (Aila & Laine 2009, dynamic fetch)
+// refilling cut the terrain render's connect time by a fifth.  The path tracer's
+// closest-hit rays are refilled after the first bounce (DGPU_REFILL_LATE), which cut
+// extend by an eighth; the camera rays of bounce 0 trace each 64-ray part to its end
+// (refilled, they are coherent enough that mixing traversal stages cost more than the
+// idle lanes).  The intersection API's closest-hit rays refill too (DGPU_REFILL_API):
+// +14% on its 16M-ray workload, half of whose rays start anywhere in random directions.
+// The threshold is a kernel argument.  Refilling from the global queue instead (Aila & Laine 2009, dynamic fetch)
 // was 1.2-3.8x slower: the refilled lanes hold rays from distant parts of the queue and
 // that coherence is lost.  Giving each XCD (blockIdx % 8) its own contiguous eighth of
 // the queue was also slower (7%): with one queue front the whole chip works on one
@@ -201,7 +203,7 @@ struct ApiIO {
 template <bool ANY, int WIDTH, int LDS, int BLOCK, bool FAST>
 __global__ __launch_bounds__(BLOCK) DG_TRACE_WAVES(WIDTH) void k_trace_api(ApiIO io, uint32_t n, uint32_t* fetch,
                                                      uint32_t* spill, uint32_t stack_cap) {
-  trace_loop<ANY, WIDTH, LDS, BLOCK, FAST>(io, n, fetch, spill, stack_cap, ANY ? DGPU_REFILL_ANY : 0u);
+  trace_loop<ANY, WIDTH, LDS, BLOCK, FAST>(io, n, fetch, spill, stack_cap, ANY ? DGPU_REFILL_ANY : DGPU_REFILL_API);
 }
 
 // Batched queues carry chunk path indices, like a single render's: a path's scene is the
