@@ -53,6 +53,7 @@ def parse():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-light", action="store_true")
+    ap.add_argument("--no-bdpt", action="store_true")
     ap.add_argument("--no-batch-tune", action="store_true",
                     help="batch leg: the default knobs (BVH4, stack 16, block 256) instead of the autotuner's")
     ap.add_argument("--calib", action="store_true",
@@ -141,6 +142,36 @@ def light_bench(gsc, film, args):
             "Mpaths_s": paths / t / 1e6, "ms": t * 1e3,
             "extension_rays_per_path": st["extension_rays"] / st["paths"],
             "shadow_rays_per_path": st["shadow_rays"] / st["paths"]}
+
+
+def bdpt_bench(ctx, gsc, film, wfilm, args):
+    """dort's bidirectional path tracer (dgpu_render_bdpt) in its two forms, the wavefront
+    stages and the one-thread-per-path kernel: camera-subpath samples per second on the
+    headline terrain (1024x1024 x 4 iterations, depth 8) and on dort's test box with its
+    area light (512x512 x 16 iterations, depth 5).  Films resident in HBM."""
+    from dort_amd import abi, gpu, scenes
+    npix = args.res * args.res
+    splat = gpu.DeviceArray(ctx, npix * 3, np.float32)
+    splat.copy_from(np.zeros(npix * 3, np.float32))
+    box = ctx.scene(scenes.dort_test_box("area"))
+    out = {}
+    for name, sc, res, its, depth in (("terrain", gsc, args.res, 4, 8), ("test_box", box, 512, 16, 5)):
+        opts = abi.render_opts(res, res, spp=its, seed=123, max_depth=depth)
+        row = {"workload": f"{name}, {res}x{res} x {its} iterations, max depth {depth}"}
+        for form, wave in (("wavefront", True), ("megakernel", False)):
+            sc.render_bdpt(opts, films=(film, wfilm, splat), device=True, wavefront=wave)
+            ts, st = [], None
+            for _ in range(3):
+                t0 = time.perf_counter()
+                *_, st = sc.render_bdpt(opts, films=(film, wfilm, splat), device=True, wavefront=wave)
+                ts.append(time.perf_counter() - t0)
+            t = float(np.median(ts))
+            row[form] = {"Mpaths_s": res * res * its / t / 1e6, "ms": t * 1e3,
+                         "extension_rays_per_path": st["extension_rays"] / st["paths"],
+                         "shadow_rays_per_path": st["shadow_rays"] / st["paths"]}
+        out[name] = row
+    splat.free()
+    return out
 
 
 def adjoint_errors(ctx, gsc, guess):
@@ -433,6 +464,12 @@ def main():
         lt = secondary["light_tracing"]
         log(f"light tracing: {lt['Mpaths_s']:.0f} Mpaths/s ({lt['ms']:.1f} ms), "
             f"{lt['extension_rays_per_path']:.2f} extension + {lt['shadow_rays_per_path']:.2f} shadow rays per path")
+    if not args.no_bdpt:
+        secondary["bdpt"] = bdpt_bench(ctx, gsc, film, wfilm, args)
+        for name, row in secondary["bdpt"].items():
+            log(f"bdpt {name}: wavefront {row['wavefront']['Mpaths_s']:.0f} Mpaths/s "
+                f"({row['wavefront']['ms']:.1f} ms), one thread per path {row['megakernel']['Mpaths_s']:.0f} "
+                f"({row['megakernel']['ms']:.1f} ms)")
     if not args.no_intersect:
         film.free()
         wfilm.free()

@@ -18,9 +18,12 @@
 // ray.  Stage 1 + b: block 0 light bounce b (BSDF uvc, Russian roulette), block 1 camera
 // bounce b (Russian roulette, BSDF uvc), block 2 its background-light pick, block 3 the
 // fresh light vertex of the strategies s = 1 at camera vertex b (pick, position).
+#include <algorithm>
+
 #include "dgpu_internal.hpp"
 #include "light_paths.hpp"
 #include "traverse.hpp"
+#include "wave_ops.hpp"
 
 namespace dgpu {
 
@@ -161,7 +164,7 @@ __device__ __forceinline__ BdPivot bd_sample_pivot(const DevScene& S, const DevL
       sincos_small(TWO_PI_F * v, sp, cp);
       o.wi = v3(cp * r, sp * r, z);
       o.pdf = INV_FOUR_PI_F;
-      o.rad = S.inf_rad;
+      o.rad = xyz(L.radiance);
     }
     o.shadow.d = normalize(o.wi);
     o.shadow.tmax = INFINITY;
@@ -498,6 +501,140 @@ __device__ __forceinline__ void bd_splat(const WaveParams& P, const BdptArgs& A,
     }
 }
 
+// One strategy (s, t) of sample_path (bdpt_renderer.cpp:43-116, path_contrib :275-466)
+// for path i, whose camera vertex t - 1 is zc with Bsdf bc: its unweighted contribution c
+// (black when the strategy gives nothing) and, for the strategies that need one, the
+// shadow ray whose visibility decides it (the caller traces or queues it).  Sets X's
+// fresh vertices, the light li of the path weight, bl (the last light vertex's Bsdf,
+// s >= 2) and fp (the film position of a t = 1 splat).
+struct BdStrategy {
+  V3 c;
+  bool ray;          // c counts only if `shadow` is unoccluded
+  TraceRay shadow;
+  uint32_t li;
+  float2 fp;
+};
+
+__device__ __forceinline__ bool bd_strategy(const WaveParams& P, const BdptArgs& A, BdPath& X, uint32_t pixel,
+                                            uint32_t sample, uint32_t light, const BdV& zc, const Bsdf& bc,
+                                            Bsdf& bl, BdStrategy& o) {
+  const DevScene& S = P.S;
+  const uint32_t i = X.i, s = X.s, t = X.t;
+  const bool z_bg = bv_kind(zc) == BV_BACKGROUND;
+  const bool z_surf = bv_kind(zc) == BV_SURFACE;
+  const V3 zero = v3(0.f, 0.f, 0.f);
+  o.c = zero;
+  o.ray = false;
+  o.shadow.o = o.shadow.d = zero;
+  o.shadow.tmin = o.shadow.tmax = 0.f;
+  o.shadow.ignore = 0xFFFFFFFFu;
+  o.li = light;
+  o.fp = make_float2(0.f, 0.f);
+  if (t == 1u && !A.use_t1) return false;
+  if (s + t < P.min_depth + 2u || s + t > P.max_depth + 2u) return false;
+  if (s > 0u && z_bg) return false;  // the virtual vertex connects to nothing
+  // s = 1, t = 1 joins the camera to a fresh light point; path_weight gives that
+  // strategy 0 for area and background lights (bdpt_renderer.cpp:494-503)
+  if (s == 1u && t == 1u) return false;
+  if (s == 0u) {
+    // the camera subpath ends on an emitter (bdpt_renderer.cpp:319-339)
+    if (t < 2u) return false;
+    BdV zp = bd_load(A.cv, A.n, i, t - 2u);
+    if (z_bg) {
+      o.li = zc.info >> 8;
+      V3 d = zc.p - zp.p;
+      o.c = (o.li == (uint32_t)S.env_light ? env_background(S, normalize(d)) : xyz(S.lights[o.li].radiance)) * zc.alpha;
+    } else {
+      uint32_t e = zc.info >> 8;
+      if (e == 0u) return false;
+      o.li = e - 1u;
+      if (dot(zc.nn, zp.p - zc.p) >= 0.f) o.c = xyz(S.lights[o.li].radiance) * zc.alpha;
+    }
+  } else if (s == 1u) {
+    // a fresh light vertex for the camera subpath (bdpt_renderer.cpp:342-388)
+    if (!z_surf) return false;
+    float rc[4];
+    bd_rng(P, pixel, sample, t, 3, rc);
+    float pick;
+    o.li = lt_pick_light(S, rc[0], pick);
+    if (pick == 0.f) return false;
+    const DevLight L = S.lights[o.li];
+    BdPivot lp = bd_sample_pivot(S, L, zc.p, zc.eps, rc[1], rc[2]);
+    if (is_black(lp.rad) || lp.pdf == 0.f) return false;
+    BdV zp = bd_load(A.cv, A.n, i, t - 2u);
+    V3 bwo = normalize(zp.p - zc.p);
+    V3 f = bsdf_eval(bc, lp.wi, bwo, BSDF_ALL);
+    if (is_black(f)) return false;
+    o.ray = true;
+    o.shadow = lp.shadow;
+    BdV fl;
+    if (L.kind != LK_AREA) {
+      fl.p = zc.p + lp.wi;  // the incident direction, stored as a point
+      fl.nn = zero;
+      fl.eps = 0.f;
+      fl.fwd = lp.pdf;
+    } else {
+      fl.p = lp.p;
+      fl.nn = lp.nn;
+      fl.eps = lp.eps;
+      fl.fwd = lp.pdf * fabsf(dot(lp.nn, lp.wi)) / length_sq(lp.p - zc.p);
+    }
+    fl.alpha = zero;
+    fl.bwd = bsdf_pdf(bc, lp.wi, bwo, BSDF_ALL);  // light_f_pdf(wi, bsdf_wo)
+    fl.leaf = 0xFFFFFFFFu;
+    fl.b1 = fl.b2 = 0.f;
+    fl.info = BV_LIGHT << 1;
+    X.first_light = fl;
+    o.c = zc.alpha * f * lp.rad * (fabsf(dot(zc.nn, lp.wi)) / (pick * lp.pdf));
+  } else {
+    const BdV yl = bd_load(A.lv, A.n, i, s - 1u);
+    const BdV yp = bd_load(A.lv, A.n, i, s - 2u);
+    bl = bd_bsdf(S, yl);
+    if (t == 1u) {
+      // the light subpath to the camera (bdpt_renderer.cpp:390-422)
+      const V3 cam = v3(S.c2w[3], S.c2w[7], S.c2w[11]);
+      float imp;
+      if (!lt_importance(P, lt_cam_point(S, yl.p), o.fp, imp)) return false;
+      V3 wi = normalize(yp.p - yl.p), wo = normalize(cam - yl.p);
+      V3 f = bsdf_eval(bl, wi, wo, BSDF_ALL);
+      if (is_black(f)) return false;
+      float dist = length(cam - yl.p);
+      o.ray = true;
+      o.shadow.o = yl.p; o.shadow.d = (cam - yl.p) / dist; o.shadow.tmin = yl.eps; o.shadow.tmax = dist;
+      o.shadow.ignore = 0xFFFFFFFFu;
+      BdV fc;
+      fc.p = cam; fc.eps = 0.f; fc.nn = zero; fc.fwd = 1.f; fc.bwd = 0.f;
+      fc.alpha = v3(imp, imp, imp);
+      fc.leaf = 0xFFFFFFFFu; fc.b1 = fc.b2 = 0.f;
+      fc.info = BV_CAMERA << 1;
+      X.first_camera = fc;
+      o.c = yl.alpha * f * imp * (fabsf(dot(yl.nn, wo)) / length_sq(cam - yl.p));
+    } else {
+      // the two subpaths joined (bdpt_renderer.cpp:423-443)
+      if (!z_surf) return false;
+      BdV zp = bd_load(A.cv, A.n, i, t - 2u);
+      V3 lwi = normalize(yp.p - yl.p), cwo = normalize(zp.p - zc.p), wo = normalize(zc.p - yl.p);
+      V3 lf = bsdf_eval(bl, lwi, wo, BSDF_ALL);
+      if (is_black(lf)) return false;
+      V3 cf = bsdf_eval(bc, -wo, cwo, BSDF_ALL);
+      if (is_black(cf)) return false;
+      float dist = length(zc.p - yl.p);
+      o.ray = true;
+      o.shadow.o = yl.p; o.shadow.d = (zc.p - yl.p) / dist; o.shadow.tmin = yl.eps; o.shadow.tmax = dist - zc.eps;
+      o.shadow.ignore = 0xFFFFFFFFu;
+      float geom = fabsf(dot(wo, yl.nn)) * fabsf(dot(wo, zc.nn)) / length_sq(yl.p - zc.p);
+      o.c = yl.alpha * zc.alpha * lf * cf * geom;
+    }
+  }
+  return !is_black(o.c);
+}
+
+// The strategy's MIS weight (path_weight; s + t = 2 has only the s = 0 strategy).
+__device__ __forceinline__ float bd_strategy_weight(const WaveParams& P, const BdPath& X, uint32_t li,
+                                                    const Bsdf& bl, const Bsdf& bc) {
+  return X.s + X.t == 2u ? (X.s == 0u ? 1.f : 0.f) : bd_weight(P, X, P.S.lights[li], bl, bc);
+}
+
 // sample_path (bdpt_renderer.cpp:43-116) for path sample i of the chunk: the sum of the
 // strategies with t >= 2 goes to P.rad[i] (w: 1, or 0 where dort's film drops the
 // sample); the t = 1 strategies are splatted.
@@ -526,137 +663,406 @@ __global__ __launch_bounds__(BD_BLOCK) void k_bdpt(WaveParams P, BdptArgs A) {
   uint32_t light = 0, ext_rays = 0;
   const uint32_t nl = bd_light_walk<WIDTH>(P, A, i, pixel, sample, r0[2], st, light, ext_rays);
   const uint32_t nc = bd_camera_walk<WIDTH>(P, A, i, pixel, sample, fx, fy, st, ext_rays);
-  const V3 cam = v3(S.c2w[3], S.c2w[7], S.c2w[11]);
-  const V3 zero = v3(0.f, 0.f, 0.f);
-  V3 sum = zero;
+  V3 sum = v3(0.f, 0.f, 0.f);
   uint32_t shadow_rays = 0;
-  auto visible = [&](const TraceRay& r) {
-    ++shadow_rays;
-    TraceHit h;
-    return !bd_trace<true, WIDTH, true>(S, r, st, h);
-  };
   BdPath X;
   X.A = &A;
   X.i = i;
   for (uint32_t t = 1; t <= nc; ++t) {
     const BdV zc = bd_load(A.cv, A.n, i, t - 1u);
-    const bool z_bg = bv_kind(zc) == BV_BACKGROUND;
-    const bool z_surf = bv_kind(zc) == BV_SURFACE;
     // the last camera vertex's Bsdf (bd_weight reads it only where it is a surface)
-    Bsdf bc = z_surf ? bd_bsdf(S, zc) : bd_no_bsdf();
+    const Bsdf bc = bv_kind(zc) == BV_SURFACE ? bd_bsdf(S, zc) : bd_no_bsdf();
     for (uint32_t s = 0; s <= nl; ++s) {
-      if (t == 1u && !A.use_t1) continue;
-      if (s + t < P.min_depth + 2u || s + t > P.max_depth + 2u) continue;
-      if (s > 0u && z_bg) continue;  // the virtual vertex connects to nothing
-      // s = 1, t = 1 joins the camera to a fresh light point; path_weight gives that
-      // strategy 0 for area and background lights (bdpt_renderer.cpp:494-503)
-      if (s == 1u && t == 1u) continue;
       X.s = s;
       X.t = t;
-      V3 c = zero;
-      uint32_t li = light;
-      float2 fp = make_float2(0.f, 0.f);
-      Bsdf bl = bd_no_bsdf();  // the last light vertex's, for s >= 2
-      if (s == 0u) {
-        // the camera subpath ends on an emitter (bdpt_renderer.cpp:319-339)
-        if (t < 2u) continue;
-        BdV zp = bd_load(A.cv, A.n, i, t - 2u);
-        if (z_bg) {
-          li = zc.info >> 8;
-          V3 d = zc.p - zp.p;
-          c = (li == (uint32_t)S.env_light ? env_background(S, normalize(d)) : S.inf_rad) * zc.alpha;
-        } else {
-          uint32_t e = zc.info >> 8;
-          if (e == 0u) continue;
-          li = e - 1u;
-          if (dot(zc.nn, zp.p - zc.p) >= 0.f) c = xyz(S.lights[li].radiance) * zc.alpha;
-        }
-        if (is_black(c)) continue;
-      } else if (s == 1u) {
-        // a fresh light vertex for the camera subpath (bdpt_renderer.cpp:342-388)
-        if (!z_surf) continue;
-        float rc[4];
-        bd_rng(P, pixel, sample, t, 3, rc);
-        float pick;
-        li = lt_pick_light(S, rc[0], pick);
-        if (pick == 0.f) continue;
-        const DevLight L = S.lights[li];
-        BdPivot lp = bd_sample_pivot(S, L, zc.p, zc.eps, rc[1], rc[2]);
-        if (is_black(lp.rad) || lp.pdf == 0.f) continue;
-        BdV zp = bd_load(A.cv, A.n, i, t - 2u);
-        V3 bwo = normalize(zp.p - zc.p);
-        V3 f = bsdf_eval(bc, lp.wi, bwo, BSDF_ALL);
-        if (is_black(f)) continue;
-        if (!visible(lp.shadow)) continue;
-        BdV fl;
-        if (L.kind != LK_AREA) {
-          fl.p = zc.p + lp.wi;  // the incident direction, stored as a point
-          fl.nn = zero;
-          fl.eps = 0.f;
-          fl.fwd = lp.pdf;
-        } else {
-          fl.p = lp.p;
-          fl.nn = lp.nn;
-          fl.eps = lp.eps;
-          fl.fwd = lp.pdf * fabsf(dot(lp.nn, lp.wi)) / length_sq(lp.p - zc.p);
-        }
-        fl.alpha = zero;
-        fl.bwd = bsdf_pdf(bc, lp.wi, bwo, BSDF_ALL);  // light_f_pdf(wi, bsdf_wo)
-        fl.leaf = 0xFFFFFFFFu;
-        fl.b1 = fl.b2 = 0.f;
-        fl.info = BV_LIGHT << 1;
-        X.first_light = fl;
-        c = zc.alpha * f * lp.rad * (fabsf(dot(zc.nn, lp.wi)) / (pick * lp.pdf));
-      } else {
-        const BdV yl = bd_load(A.lv, A.n, i, s - 1u);
-        const BdV yp = bd_load(A.lv, A.n, i, s - 2u);
-        bl = bd_bsdf(S, yl);
-        if (t == 1u) {
-          // the light subpath to the camera (bdpt_renderer.cpp:390-422)
-          float imp;
-          if (!lt_importance(P, lt_cam_point(S, yl.p), fp, imp)) continue;
-          V3 wi = normalize(yp.p - yl.p), wo = normalize(cam - yl.p);
-          V3 f = bsdf_eval(bl, wi, wo, BSDF_ALL);
-          if (is_black(f)) continue;
-          TraceRay sr;
-          float dist = length(cam - yl.p);
-          sr.o = yl.p; sr.d = (cam - yl.p) / dist; sr.tmin = yl.eps; sr.tmax = dist; sr.ignore = 0xFFFFFFFFu;
-          if (!visible(sr)) continue;
-          BdV fc;
-          fc.p = cam; fc.eps = 0.f; fc.nn = zero; fc.fwd = 1.f; fc.bwd = 0.f;
-          fc.alpha = v3(imp, imp, imp);
-          fc.leaf = 0xFFFFFFFFu; fc.b1 = fc.b2 = 0.f;
-          fc.info = BV_CAMERA << 1;
-          X.first_camera = fc;
-          c = yl.alpha * f * imp * (fabsf(dot(yl.nn, wo)) / length_sq(cam - yl.p));
-        } else {
-          // the two subpaths joined (bdpt_renderer.cpp:423-443)
-          if (!z_surf) continue;
-          BdV zp = bd_load(A.cv, A.n, i, t - 2u);
-          V3 lwi = normalize(yp.p - yl.p), cwo = normalize(zp.p - zc.p), wo = normalize(zc.p - yl.p);
-          V3 lf = bsdf_eval(bl, lwi, wo, BSDF_ALL);
-          if (is_black(lf)) continue;
-          V3 cf = bsdf_eval(bc, -wo, cwo, BSDF_ALL);
-          if (is_black(cf)) continue;
-          TraceRay sr;
-          float dist = length(zc.p - yl.p);
-          sr.o = yl.p; sr.d = (zc.p - yl.p) / dist; sr.tmin = yl.eps; sr.tmax = dist - zc.eps;
-          sr.ignore = 0xFFFFFFFFu;
-          if (!visible(sr)) continue;
-          float geom = fabsf(dot(wo, yl.nn)) * fabsf(dot(wo, zc.nn)) / length_sq(yl.p - zc.p);
-          c = yl.alpha * zc.alpha * lf * cf * geom;
-        }
+      Bsdf bl = bd_no_bsdf();
+      BdStrategy o;
+      if (!bd_strategy(P, A, X, pixel, sample, light, zc, bc, bl, o)) continue;
+      if (o.ray) {
+        ++shadow_rays;
+        TraceHit h;
+        if (bd_trace<true, WIDTH, true>(S, o.shadow, st, h)) continue;
       }
-      if (is_black(c)) continue;
-      float w = s + t == 2u ? (s == 0u ? 1.f : 0.f) : bd_weight(P, X, S.lights[li], bl, bc);
-      V3 wc = c * w;
-      if (t == 1u) bd_splat(P, A, fp, wc);
+      V3 wc = o.c * bd_strategy_weight(P, X, o.li, bl, bc);
+      if (t == 1u) bd_splat(P, A, o.fp, wc);
       else sum = sum + wc;
     }
   }
   P.rad[i] = make_float4(sum.x, sum.y, sum.z, on_edge ? 0.f : 1.f);
   atomicAdd((unsigned long long*)&P.stats[1], (unsigned long long)ext_rays);
   atomicAdd((unsigned long long*)&P.stats[2], (unsigned long long)shadow_rays);
+}
+
+// ---------------------------------------------------------------- wavefront form
+//
+// The chunk of k_bdpt in stages that reuse the path tracer's trace kernels:
+//   k_bd_lstart, then per bounce k_extend + k_bd_lshade: the light subpaths;
+//   k_bd_cstart, then per bounce k_extend + k_bd_cshade: the camera subpaths;
+//   k_bd_strat: every strategy's contribution and MIS weight (bd_strategy), recorded per
+//     path in strategy order, its shadow ray staged; k_bd_gather: the staged rays into one
+//     queue; k_connect: their visibility (an occluded ray clears its record);
+//   k_bd_resolve: the visible records, summed in strategy order (t >= 2) or splatted.
+// Each stage draws the random numbers k_bdpt draws at the same point of the walk, so both
+// forms trace the same paths.  Vertices are stored as soon as they exist, and a vertex's
+// backward pdf, known once the next vertex is, is patched into its record then.
+
+__device__ __forceinline__ void bd_patch_bwd(float4* base, uint32_t n, uint32_t i, uint32_t k, float bwd) {
+  base[(size_t)4 * k * n + 2 * (size_t)n + i].w = bwd;
+}
+
+// Light subpath, vertex 0 and the first ray (random_light_walk, bdpt_renderer.cpp:118-146).
+__global__ __launch_bounds__(SHADE_BLOCK) void k_bd_lstart(WaveParams P, BdptArgsHost A) {
+  __shared__ uint32_t sh[SHADE_BLOCK / 64 + 1];
+  const uint32_t i = blockIdx.x * SHADE_BLOCK + threadIdx.x;
+  bool go = false;
+  float4 qo = make_float4(0.f, 0.f, 0.f, 0.f), qd = qo;  // the first ray, if any
+  if (i < P.n_chunk) {
+    const DevScene& S = P.S;
+    uint32_t pixel, sample;
+    order_sample_pixel(P.ord, P.s0, P.q0 + i, sample, pixel);
+    float r0[4];
+    bd_rng(P, pixel, sample, 0, 0, r0);
+    float pick;
+    const uint32_t light = lt_pick_light(S, r0[2], pick);
+    uint32_t nvert = 0;
+    if (pick != 0.f) {
+      const DevLight L = S.lights[light];
+      float r[4];
+      bd_rng(P, pixel, sample, 0, 1, r);
+      const LtRay lr = lt_light_ray(S, L, r);
+      if (lr.pos_pdf != 0.f && lr.dir_pdf != 0.f) {
+        BdV prev;
+        prev.p = lr.o; prev.eps = lr.tmin; prev.nn = lr.nn;
+        prev.fwd = lr.pos_pdf; prev.bwd = 0.f;
+        prev.alpha = lr.rad / (lr.pos_pdf * pick);
+        prev.leaf = 0xFFFFFFFFu; prev.b1 = prev.b2 = 0.f;
+        prev.info = BV_LIGHT << 1;
+        bd_store(A.lv, A.n, i, 0, prev);
+        nvert = 1;
+        A.w0[i] = make_float4(1.f, 1.f, 1.f, lr.dir_pdf);
+        A.w1[i] = make_uint4(1u, light, L.kind != LK_AREA ? 1u : 0u, 0u);
+        go = nvert < A.nv;
+        qo = make_float4(lr.o.x, lr.o.y, lr.o.z, lr.tmin);
+        qd = make_float4(lr.d.x, lr.d.y, lr.d.z, INFINITY);
+      }
+    }
+    A.nl[i] = nvert | (light << 8);
+  }
+  const uint32_t q = block_append(&P.counters[CNT_QUEUE], go, sh);
+  if (go) {
+    P.qo[0][q] = qo;
+    P.qd[0][q] = qd;
+    P.qpath[0][q] = i;
+  }
+}
+
+// One bounce of random_light_walk (bdpt_renderer.cpp:146-181) for the light rays of bounce b.
+__global__ __launch_bounds__(SHADE_BLOCK) void k_bd_lshade(WaveParams P, BdptArgsHost A, uint32_t b) {
+  __shared__ uint32_t sh[SHADE_BLOCK / 64 + 1];
+  const DevScene& S = P.S;
+  const uint32_t cur = b & 1u, nxt = cur ^ 1u;
+  const uint32_t n = P.counters[CNT_QUEUE + b];
+  for (uint32_t base = blockIdx.x * SHADE_BLOCK; base < n; base += gridDim.x * SHADE_BLOCK) {
+    const uint32_t i = base + threadIdx.x;
+    bool go = false;
+    uint32_t p = 0;
+    V3 go_o = v3(0.f, 0.f, 0.f), go_d = go_o;
+    float go_eps = 0.f;
+    if (i < n) {
+      p = P.qpath[cur][i];
+      const float4 h4 = P.hit[i];
+      const uint32_t leaf = __float_as_uint(h4.w);
+      if (leaf != 0xFFFFFFFFu) {
+        const float4 w0 = A.w0[p];
+        const uint4 w1 = A.w1[p];
+        uint32_t nvert = w1.x;
+        const bool distant = w1.z != 0u;
+        const V3 prev_f = xyz(w0);
+        const float fwd_dir_pdf = w0.w;
+        uint32_t pixel, sample;
+        order_sample_pixel(P.ord, P.s0, P.q0 + p, sample, pixel);
+        Surface g = surface_at(S, leaf, h4.x, h4.y, h4.z);
+        Bsdf bsdf = make_bsdf(S, g);
+        const V3 wi = normalize(-xyz(P.qd[cur][i]));
+        float rb[4];
+        bd_rng(P, pixel, sample, 1u + b, 0, rb);
+        V3 wo;
+        float wo_pdf;
+        uint32_t fl;
+        V3 f = bsdf_sample(bsdf, wi, BSDF_ALL, rb[0], rb[1], rb[2], wo, wo_pdf, fl);
+        if (wo_pdf != 0.f) {
+          const float bwd_dir_pdf = bsdf_pdf(bsdf, wi, wo, BSDF_ALL);
+          const BdV prev = bd_load(A.lv, A.n, p, nvert - 1u);
+          const float d2 = length_sq(g.p - prev.p);
+          if (d2 >= 1e-9f) {
+            V3 scale = prev_f * (fabsf(dot(prev.nn, wi)) / fwd_dir_pdf);
+            bool live = true;
+            if (b >= 2u) {
+              float rr = fminf(average(scale), 0.9f);
+              if (rb[3] > rr) live = false;
+              else scale = scale / rr;
+            }
+            if (live) {
+              BdV y;
+              y.p = g.p; y.eps = g.eps; y.nn = g.nn;
+              y.fwd = (b == 0u && distant) ? fwd_dir_pdf : fwd_dir_pdf * fabsf(dot(g.nn, wi)) / d2;
+              y.bwd = 0.f;
+              y.alpha = prev.alpha * scale;
+              y.leaf = leaf; y.b1 = h4.y; y.b2 = h4.z;
+              y.info = (BV_SURFACE << 1) | ((fl & BSDF_DELTA) ? 1u : 0u);
+              if (!is_black(y.alpha)) {
+                bd_patch_bwd(A.lv, A.n, p, nvert - 1u, bwd_dir_pdf * fabsf(dot(prev.nn, wi)) / d2);
+                bd_store(A.lv, A.n, p, nvert, y);
+                ++nvert;
+                A.nl[p] = nvert | (w1.y << 8);
+                A.w0[p] = make_float4(f.x, f.y, f.z, wo_pdf);
+                A.w1[p] = make_uint4(nvert, w1.y, w1.z, 0u);
+                go = b + 1u <= P.max_depth && nvert < A.nv;
+                go_o = g.p; go_d = wo; go_eps = g.eps;
+              }
+            }
+          }
+        }
+      }
+    }
+    const uint32_t q = block_append(&P.counters[CNT_QUEUE + b + 1u], go, sh);
+    if (go) {
+      P.qo[nxt][q] = make_float4(go_o.x, go_o.y, go_o.z, go_eps);
+      P.qd[nxt][q] = make_float4(go_d.x, go_d.y, go_d.z, INFINITY);
+      P.qpath[nxt][q] = p;
+    }
+  }
+}
+
+// Camera subpath, vertex 0 and the first ray (random_camera_walk, bdpt_renderer.cpp:183-204).
+__global__ __launch_bounds__(SHADE_BLOCK) void k_bd_cstart(WaveParams P, BdptArgsHost A) {
+  __shared__ uint32_t sh[SHADE_BLOCK / 64 + 1];
+  const DevScene& S = P.S;
+  const uint32_t i = blockIdx.x * SHADE_BLOCK + threadIdx.x;
+  bool go = false;
+  V3 o = v3(0.f, 0.f, 0.f), d = o;
+  if (i < P.n_chunk) {
+    uint32_t pixel, sample;
+    order_sample_pixel(P.ord, P.s0, P.q0 + i, sample, pixel);
+    float r0[4];
+    bd_rng(P, pixel, sample, 0, 0, r0);
+    const float fx = float(pixel % P.xr) + r0[0], fy = float(pixel / P.xr) + r0[1];
+    camera_ray(S, fx, fy, P.xr, P.yr, o, d);
+    const float dir_pdf = bd_camera_pdf_at(P, fx, fy);
+    BdV prev;
+    prev.p = o; prev.eps = 0.f; prev.nn = d;
+    prev.fwd = 1.f; prev.bwd = 0.f;
+    prev.alpha = v3(dir_pdf, dir_pdf, dir_pdf);  // importance / pos_pdf (pos_pdf = 1)
+    prev.leaf = 0xFFFFFFFFu; prev.b1 = prev.b2 = 0.f;
+    prev.info = BV_CAMERA << 1;
+    bd_store(A.cv, A.n, i, 0, prev);
+    A.nc[i] = 1u;
+    A.w1[i] = make_uint4(1u, 0u, 0u, 0u);
+    if (!is_black(prev.alpha) && A.nv > 1u) {
+      const V3 scale = v3(1.f, 1.f, 1.f) * (fabsf(dot(prev.nn, d)) / dir_pdf);
+      A.w0[i] = make_float4(scale.x, scale.y, scale.z, dir_pdf);
+      go = true;
+    }
+  }
+  const uint32_t q = block_append(&P.counters[CNT_QUEUE], go, sh);
+  if (go) {
+    P.qo[0][q] = make_float4(o.x, o.y, o.z, 0.f);
+    P.qd[0][q] = make_float4(d.x, d.y, d.z, INFINITY);
+    P.qpath[0][q] = i;
+  }
+}
+
+// One bounce of random_camera_walk (bdpt_renderer.cpp:204-273) for the camera rays of
+// bounce b; the Russian roulette of the next bounce decides whether its ray is queued.
+__global__ __launch_bounds__(SHADE_BLOCK) void k_bd_cshade(WaveParams P, BdptArgsHost A, uint32_t b) {
+  __shared__ uint32_t sh[SHADE_BLOCK / 64 + 1];
+  const DevScene& S = P.S;
+  const uint32_t cur = b & 1u, nxt = cur ^ 1u;
+  const uint32_t n = P.counters[CNT_QUEUE + b];
+  for (uint32_t base = blockIdx.x * SHADE_BLOCK; base < n; base += gridDim.x * SHADE_BLOCK) {
+    const uint32_t i = base + threadIdx.x;
+    bool go = false;
+    uint32_t p = 0;
+    V3 go_o = v3(0.f, 0.f, 0.f), go_d = go_o;
+    float go_eps = 0.f;
+    if (i < n) {
+      p = P.qpath[cur][i];
+      const float4 h4 = P.hit[i];
+      const uint32_t leaf = __float_as_uint(h4.w);
+      const float4 w0 = A.w0[p];
+      uint32_t nvert = A.w1[p].x;
+      const V3 scale = xyz(w0);
+      const float fwd_dir_pdf = w0.w;
+      const V3 rd = xyz(P.qd[cur][i]);
+      const BdV prev = bd_load(A.cv, A.n, p, nvert - 1u);
+      uint32_t pixel, sample;
+      order_sample_pixel(P.ord, P.s0, P.q0 + p, sample, pixel);
+      if (leaf == 0xFFFFFFFFu) {
+        // the "virtual vertex" of the escaped ray (bdpt_renderer.cpp:223-239)
+        float r2[4];
+        bd_rng(P, pixel, sample, 1u + b, 2, r2);
+        uint32_t bg = 0;
+        const float bgp = bd_pick_background(S, r2[0], bg);
+        if (bgp != 0.f) {
+          BdV z;
+          z.p = prev.p + rd; z.eps = 0.f; z.nn = v3(0.f, 0.f, 0.f);
+          z.fwd = fwd_dir_pdf; z.bwd = 0.f;
+          z.alpha = prev.alpha * scale / bgp;
+          z.leaf = 0xFFFFFFFFu; z.b1 = z.b2 = 0.f;
+          z.info = (BV_BACKGROUND << 1) | (bg << 8);
+          bd_store(A.cv, A.n, p, nvert, z);
+          ++nvert;
+          A.nc[p] = nvert;
+        }
+      } else {
+        Surface g = surface_at(S, leaf, h4.x, h4.y, h4.z);
+        Bsdf bsdf = make_bsdf(S, g);
+        const V3 wo = -rd;
+        float rb[4];
+        bd_rng(P, pixel, sample, 1u + b, 1, rb);
+        V3 wi;
+        float wi_pdf;
+        uint32_t fl;
+        V3 f = bsdf_sample(bsdf, wo, BSDF_ALL, rb[1], rb[2], rb[3], wi, wi_pdf, fl);  // sample_light_f
+        const float d2 = length_sq(g.p - prev.p);
+        if (d2 >= 1e-9f) {
+          const uint32_t emitter = g.light >= 0 ? (uint32_t)g.light + 1u : 0u;
+          BdV z;
+          z.p = g.p; z.eps = g.eps; z.nn = g.nn;
+          z.fwd = fwd_dir_pdf * fabsf(dot(g.nn, wo)) / d2;
+          z.bwd = 0.f;
+          z.alpha = prev.alpha * scale;
+          z.leaf = leaf; z.b1 = h4.y; z.b2 = h4.z;
+          z.info = (BV_SURFACE << 1) | ((fl & BSDF_DELTA) ? 1u : 0u) | (emitter << 8);
+          if (!(is_black(z.alpha) && emitter == 0u)) {
+            if (wi_pdf != 0.f) {
+              const float bwd_dir_pdf = bsdf_pdf(bsdf, wo, wi, BSDF_ALL);  // camera_f_pdf(wo, wi)
+              bd_patch_bwd(A.cv, A.n, p, nvert - 1u, bwd_dir_pdf * fabsf(dot(prev.nn, wo)) / d2);
+            }
+            bd_store(A.cv, A.n, p, nvert, z);
+            ++nvert;
+            A.nc[p] = nvert;
+            A.w1[p] = make_uint4(nvert, 0u, 0u, 0u);
+            // the top of the next bounce: its Russian roulette, before its ray is traced
+            if (wi_pdf != 0.f && b + 1u <= P.max_depth && nvert < A.nv && !is_black(z.alpha)) {
+              float rn[4];
+              bd_rng(P, pixel, sample, 2u + b, 1, rn);
+              V3 sc = f * (fabsf(dot(z.nn, wi)) / wi_pdf);
+              bool live = true;
+              if (b + 1u >= 2u) {
+                float rr = fminf(average(sc), 0.9f);
+                if (rn[0] > rr) live = false;
+                else sc = sc / rr;
+              }
+              if (live) {
+                A.w0[p] = make_float4(sc.x, sc.y, sc.z, wi_pdf);
+                go = true;
+                go_o = g.p; go_d = wi; go_eps = g.eps;
+              }
+            }
+          }
+        }
+      }
+    }
+    const uint32_t q = block_append(&P.counters[CNT_QUEUE + b + 1u], go, sh);
+    if (go) {
+      P.qo[nxt][q] = make_float4(go_o.x, go_o.y, go_o.z, go_eps);
+      P.qd[nxt][q] = make_float4(go_d.x, go_d.y, go_d.z, INFINITY);
+      P.qpath[nxt][q] = p;
+    }
+  }
+}
+
+// Every strategy of path i (sample_path's loops, t outer, s inner): its weighted
+// contribution as a record (w = 1: summed, 2: splatted) in strategy order, and the shadow
+// ray of a strategy that needs one, staged beside the path with its record's index.
+__global__ __launch_bounds__(SHADE_BLOCK) void k_bd_strat(WaveParams P, BdptArgsHost H) {
+  const uint32_t i = blockIdx.x * SHADE_BLOCK + threadIdx.x;
+  if (i >= P.n_chunk) return;
+  const DevScene& S = P.S;
+  BdptArgs A;
+  A.lv = H.lv; A.cv = H.cv; A.n = H.n; A.nv = H.nv; A.use_t1 = H.use_t1; A.splat = H.splat;
+  uint32_t pixel, sample;
+  order_sample_pixel(P.ord, P.s0, P.q0 + i, sample, pixel);
+  const uint32_t lw = H.nl[i], nl = lw & 0xFFu, light = lw >> 8, nc = H.nc[i];
+  const size_t base = (size_t)i * H.ns;
+  uint32_t nrec = 0, nray = 0;
+  BdPath X;
+  X.A = &A;
+  X.i = i;
+  for (uint32_t t = 1; t <= nc; ++t) {
+    const BdV zc = bd_load(A.cv, A.n, i, t - 1u);
+    const Bsdf bc = bv_kind(zc) == BV_SURFACE ? bd_bsdf(S, zc) : bd_no_bsdf();
+    for (uint32_t s = 0; s <= nl; ++s) {
+      X.s = s;
+      X.t = t;
+      Bsdf bl = bd_no_bsdf();
+      BdStrategy o;
+      if (!bd_strategy(P, A, X, pixel, sample, light, zc, bc, bl, o)) continue;
+      const V3 wc = o.c * bd_strategy_weight(P, X, o.li, bl, bc);
+      const uint32_t k = nrec++;
+      H.rec[base + k] = make_float4(wc.x, wc.y, wc.z, t == 1u ? 2.f : 1.f);
+      if (t == 1u) H.rec_fp[base + k] = o.fp;
+      if (o.ray) {
+        H.stage_o[base + nray] = make_float4(o.shadow.o.x, o.shadow.o.y, o.shadow.o.z, o.shadow.tmin);
+        H.stage_d[base + nray] = make_float4(o.shadow.d.x, o.shadow.d.y, o.shadow.d.z, o.shadow.tmax);
+        H.nray[base + nray] = k;
+        ++nray;
+      }
+    }
+  }
+  H.nrec[i] = nrec | (nray << 16);
+}
+
+// The staged shadow rays of every path into one queue (one atomic per workgroup); a ray's
+// slot word is its record's index, which k_connect clears on an occluded ray.
+__global__ __launch_bounds__(SHADE_BLOCK) void k_bd_gather(WaveParams P, BdptArgsHost A) {
+  __shared__ uint32_t sh[SHADE_BLOCK / 64 + 1];
+  const uint32_t i = blockIdx.x * SHADE_BLOCK + threadIdx.x;
+  const uint32_t cnt = i < P.n_chunk ? A.nrec[i] >> 16 : 0u;
+  const uint32_t q = block_append_n(&P.counters[CNT_SHADOW], cnt, sh);
+  const size_t base = (size_t)i * A.ns;
+  for (uint32_t k = 0; k < cnt; ++k) {
+    P.so[q + k] = A.stage_o[base + k];
+    P.sd[q + k] = A.stage_d[base + k];
+    P.smeta[q + k] = make_uint2((uint32_t)base + A.nray[base + k], 0xFFFFFFFFu);
+  }
+}
+
+// The visible records of path i: t >= 2 summed in strategy order into P.rad[i] (w: 1, or
+// 0 where dort's film drops the sample), t = 1 splatted.
+__global__ __launch_bounds__(SHADE_BLOCK) void k_bd_resolve(WaveParams P, BdptArgsHost H) {
+  const uint32_t i = blockIdx.x * SHADE_BLOCK + threadIdx.x;
+  if (i >= P.n_chunk) return;
+  BdptArgs A;
+  A.lv = H.lv; A.cv = H.cv; A.n = H.n; A.nv = H.nv; A.use_t1 = H.use_t1; A.splat = H.splat;
+  uint32_t pixel, sample;
+  order_sample_pixel(P.ord, P.s0, P.q0 + i, sample, pixel);
+  float r0[4];
+  bd_rng(P, pixel, sample, 0, 0, r0);
+  const bool on_edge = r0[0] == 0.f || r0[1] == 0.f;
+  const uint32_t nrec = H.nrec[i] & 0xFFFFu;
+  const size_t base = (size_t)i * H.ns;
+  V3 sum = v3(0.f, 0.f, 0.f);
+  for (uint32_t k = 0; k < nrec; ++k) {
+    const float4 r = H.rec[base + k];
+    if (r.w == 1.f) sum = sum + xyz(r);
+    else if (r.w == 2.f) bd_splat(P, A, H.rec_fp[base + k], xyz(r));
+  }
+  P.rad[i] = make_float4(sum.x, sum.y, sum.z, on_edge ? 0.f : 1.f);
+}
+
+// Adds a stage's ray counts to the statistics: the extension queues (walks) or the
+// shadow queue (strategies).
+__global__ void k_bd_count(WaveParams P, uint32_t shadow) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  if (shadow) {
+    P.stats[2] += P.counters[CNT_SHADOW];
+  } else {
+    unsigned long long e = 0;
+    for (uint32_t b = 0; b < MAX_BOUNCES; ++b) e += P.counters[CNT_QUEUE + b];
+    P.stats[1] += e;
+  }
 }
 
 // The chunk's per-path sums added to the film pixel by pixel in sample order, as
@@ -685,6 +1091,34 @@ __global__ __launch_bounds__(256) void k_bdpt_film(WaveParams P) {
   P.film_rgb[3 * (size_t)px + 2] = fb;
   P.film_w[px] = fw;
   if (dropped) atomicAdd((unsigned long long*)&P.stats[3], (unsigned long long)dropped);
+}
+
+hipError_t launch_bdpt_wave(const WaveParams& P, const BdptArgsHost& A, const KernelCfg& ce,
+                            const KernelCfg& cc, uint32_t shade_grid, hipStream_t st) {
+  const uint32_t blocks = (P.n_chunk + SHADE_BLOCK - 1u) / SHADE_BLOCK;
+  const uint32_t grid = std::min(blocks, std::max(1u, shade_grid));
+  hipError_t e;
+  // light subpaths, then camera subpaths: the same queues and counters, zeroed between
+  for (int walk = 0; walk < 2; ++walk) {
+    if ((e = launch_zero(P.counters, NUM_COUNTERS, st)) != hipSuccess) return e;
+    if (walk == 0) hipLaunchKernelGGL(k_bd_lstart, dim3(blocks), dim3(SHADE_BLOCK), 0, st, P, A);
+    else hipLaunchKernelGGL(k_bd_cstart, dim3(blocks), dim3(SHADE_BLOCK), 0, st, P, A);
+    for (uint32_t b = 0; b <= P.max_depth; ++b) {
+      if ((e = launch_extend(P, ce, b, false, st)) != hipSuccess) return e;
+      if (walk == 0) hipLaunchKernelGGL(k_bd_lshade, dim3(grid), dim3(SHADE_BLOCK), 0, st, P, A, b);
+      else hipLaunchKernelGGL(k_bd_cshade, dim3(grid), dim3(SHADE_BLOCK), 0, st, P, A, b);
+    }
+    hipLaunchKernelGGL(k_bd_count, dim3(1), dim3(64), 0, st, P, 0u);
+  }
+  // strategies, their shadow rays, the visible contributions
+  if ((e = launch_zero(P.counters, NUM_COUNTERS, st)) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_bd_strat, dim3(blocks), dim3(SHADE_BLOCK), 0, st, P, A);
+  hipLaunchKernelGGL(k_bd_gather, dim3(blocks), dim3(SHADE_BLOCK), 0, st, P, A);
+  if ((e = launch_connect(P, cc, 0, false, st)) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_bd_resolve, dim3(blocks), dim3(SHADE_BLOCK), 0, st, P, A);
+  hipLaunchKernelGGL(k_bd_count, dim3(1), dim3(64), 0, st, P, 1u);
+  hipLaunchKernelGGL(k_bdpt_film, dim3((P.npix + 255u) / 256u), dim3(256), 0, st, P);
+  return hipGetLastError();
 }
 
 hipError_t launch_bdpt(const WaveParams& P, const BdptArgsHost& H, uint32_t width, hipStream_t st) {

@@ -99,7 +99,7 @@ __device__ __forceinline__ LtRay lt_light_ray(const DevScene& S, const DevLight&
     sincos_small(TWO_PI_F * r[3], sp, cp);
     dir = v3(cp * rr, sp * rr, z);
     out.dir_pdf = INV_FOUR_PI_F;
-    out.rad = S.inf_rad;
+    out.rad = xyz(L.radiance);  // the light table holds the infinite light's radiance
   }
   float dr = sqrtf(r[0]), ds, dc;  // uniform_disk_sample
   sincos_small(TWO_PI_F * r[1], ds, dc);

@@ -165,8 +165,11 @@ enum {
   DGPU_TRACE_BOUNDED = 8u,  /* dgpu_trace_*: every ray origin lies within the scene's range (no
                                coordinate beyond max |vertex or camera coordinate|), which
                                allows the path tracer's faster box test */
-  DGPU_ADJOINT_MEGAKERNEL = 16u /* dgpu_adjoint: one thread per path traces and replays it in
-                                   one kernel, instead of the two wavefront passes */
+  DGPU_ADJOINT_MEGAKERNEL = 16u, /* dgpu_adjoint: one thread per path traces and replays it in
+                                    one kernel, instead of the two wavefront passes */
+  DGPU_BDPT_WAVEFRONT = 32u      /* dgpu_render_bdpt: the wavefront stages (subpath walks over
+                                    the path tracer's trace kernels, then the strategies and
+                                    their shadow rays) instead of one thread per path sample */
 };
 
 /* ---- rendering ----------------------------------------------------------- */
