@@ -158,12 +158,12 @@ def bdpt_bench(ctx, gsc, film, wfilm, args):
     for name, sc, res, its, depth in (("terrain", gsc, args.res, 4, 8), ("test_box", box, 512, 16, 5)):
         opts = abi.render_opts(res, res, spp=its, seed=123, max_depth=depth)
         row = {"workload": f"{name}, {res}x{res} x {its} iterations, max depth {depth}"}
-        for form, wave in (("wavefront", True), ("megakernel", False)):
-            sc.render_bdpt(opts, films=(film, wfilm, splat), device=True, wavefront=wave)
+        for form, mega in (("wavefront", False), ("megakernel", True)):
+            sc.render_bdpt(opts, films=(film, wfilm, splat), device=True, megakernel=mega)
             ts, st = [], None
             for _ in range(3):
                 t0 = time.perf_counter()
-                *_, st = sc.render_bdpt(opts, films=(film, wfilm, splat), device=True, wavefront=wave)
+                *_, st = sc.render_bdpt(opts, films=(film, wfilm, splat), device=True, megakernel=mega)
                 ts.append(time.perf_counter() - t0)
             t = float(np.median(ts))
             row[form] = {"Mpaths_s": res * res * its / t / 1e6, "ms": t * 1e3,

@@ -17,7 +17,7 @@ BUF_DEVICE = 1
 RENDER_PROFILE = 2
 BATCH_NO_GRAPH = 4
 ADJOINT_MEGAKERNEL = 16
-BDPT_WAVEFRONT = 32
+BDPT_MEGAKERNEL = 32
 TRACE_BOUNDED = 8
 
 f3 = C.c_float * 3

@@ -1097,23 +1097,30 @@ dgpu_status dgpu_render_bdpt(dgpu_scene* s, const dgpu_render_opts* o, uint32_t 
   // BdptRenderer::render returns at once without lights (bdpt_renderer.cpp:11)
   const uint64_t total = s->S.num_lights ? (uint64_t)npix * (o->sample_end - o->sample_begin) : 0;
   const uint32_t nv = o->max_depth + 2u;
-  const bool mega = !(o->flags & DGPU_BDPT_WAVEFRONT);
+  const bool mega = o->flags & DGPU_BDPT_MEGAKERNEL;
   // strategies (s, t) with t >= 1 and s + t <= nv: the records per path of the wavefront form
   const uint32_t ns = nv * (nv + 1u) / 2u;
   // per path: two subpaths of nv vertices and the staged sample; the megakernel adds its
-  // traversal spill slice, the wavefront form its walk state, strategy records, staged and
-  // queued shadow rays and the extension queues
+  // traversal spill slice.  The wavefront form adds its walk state and two extension rays
+  // per bounce (both queues and the hits); its strategy stage, which runs over parts of
+  // the chunk, needs per path ns records (24 B) and shadow-queue slots (40 B).
   const uint64_t per_path = mega ? 2ull * nv * BDPT_VERTEX_BYTES + 16ull + 4ull * s->stack_cap
-                                 : 2ull * nv * BDPT_VERTEX_BYTES + 16ull + 44ull + (uint64_t)ns * 100ull + 88ull;
+                                 : 2ull * nv * BDPT_VERTEX_BYTES + 16ull + 64ull + 8ull + 176ull;
+  const uint64_t per_strat = (uint64_t)ns * 64ull;
   size_t free_b = 0, total_b = 0;
   if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
-  const uint64_t budget = std::min<uint64_t>(mega ? (8ull << 30) : (16ull << 30), free_b / 4);
-  uint64_t cap = std::max<uint64_t>(1ull << 16, budget / per_path);
+  const uint64_t budget = std::min<uint64_t>(mega ? (8ull << 30) : (32ull << 30), free_b / 4);
+  // the wavefront form gives half of its budget to the walks, half to the strategy stage
+  uint64_t cap = std::max<uint64_t>(1ull << 16, (mega ? budget : budget / 2) / per_path);
   if (s->queue_tile) cap = s->queue_tile;
-  // chunk positions q0 + i and record indices i * ns stay below 2^32
+  // chunk positions q0 + i below 2^32; both walks' queue words (path index | camera bit)
+  // below 2^31 entries
   const uint32_t C = (uint32_t)std::min<uint64_t>(
-      std::min<uint64_t>(std::min<uint64_t>(total ? total : 1, cap), 0x7FFFFFC0ull),
-      std::min<uint64_t>(std::max<uint64_t>(1, 0xFFFFFFFFull - npix), 0xFFFFFFFFull / ns));
+      std::min<uint64_t>(std::min<uint64_t>(total ? total : 1, cap), 0x3FFFFFC0ull),
+      std::max<uint64_t>(1, 0xFFFFFFFFull - npix));
+  // paths per strategy part: record indices j * ns below 2^32
+  const uint32_t M = (uint32_t)std::min<uint64_t>(
+      std::min<uint64_t>(C, std::max<uint64_t>(1ull << 16, budget / 2 / per_strat)), 0xFFFFFFFFull / ns);
   WaveParams P;
   memset(&P, 0, sizeof P);
   P.S = s->S;
@@ -1143,27 +1150,24 @@ dgpu_status dgpu_render_bdpt(dgpu_scene* s, const dgpu_render_opts* o, uint32_t 
     cc.grid = ctx->grid_for(3, cc);
     shade_grid = ctx->shade_blocks(false, false, false);
     P.spill = (uint32_t*)ctx->buf("spill", (size_t)std::max(ce.grid, cc.grid) * s->cfg.block * s->stack_cap * 4, &st);
-    P.qo[0] = (float4*)ctx->buf("qo0", (size_t)C * 16, &st);
-    P.qo[1] = (float4*)ctx->buf("qo1", (size_t)C * 16, &st);
-    P.qd[0] = (float4*)ctx->buf("qd0", (size_t)C * 16, &st);
-    P.qd[1] = (float4*)ctx->buf("qd1", (size_t)C * 16, &st);
-    P.qpath[0] = (uint32_t*)ctx->buf("qp0", (size_t)C * 4, &st);
-    P.qpath[1] = (uint32_t*)ctx->buf("qp1", (size_t)C * 4, &st);
-    P.hit = (float4*)ctx->buf("hit", (size_t)C * 16, &st);
-    P.so = (float4*)ctx->buf("so", (size_t)C * ns * 16, &st);
-    P.sd = (float4*)ctx->buf("sd", (size_t)C * ns * 16, &st);
-    P.smeta = (uint2*)ctx->buf("smeta", (size_t)C * ns * 8, &st);
+    const size_t q = (size_t)2 * C;  // a light and a camera ray per path and bounce
+    P.qo[0] = (float4*)ctx->buf("qo0", q * 16, &st);
+    P.qo[1] = (float4*)ctx->buf("qo1", q * 16, &st);
+    P.qd[0] = (float4*)ctx->buf("qd0", q * 16, &st);
+    P.qd[1] = (float4*)ctx->buf("qd1", q * 16, &st);
+    P.qpath[0] = (uint32_t*)ctx->buf("qp0", q * 4, &st);
+    P.qpath[1] = (uint32_t*)ctx->buf("qp1", q * 4, &st);
+    P.hit = (float4*)ctx->buf("hit", q * 16, &st);
+    P.so = (float4*)ctx->buf("so", (size_t)M * ns * 16, &st);
+    P.sd = (float4*)ctx->buf("sd", (size_t)M * ns * 16, &st);
+    P.smeta = (uint2*)ctx->buf("smeta", (size_t)M * ns * 8, &st);
     P.counters = (uint32_t*)ctx->buf("counters", NUM_COUNTERS * 4, &st);
-    A.w0 = (float4*)ctx->buf("bd_w0", (size_t)C * 16, &st);
-    A.w1 = (uint4*)ctx->buf("bd_w1", (size_t)C * 16, &st);
+    A.w0 = (float4*)ctx->buf("bd_w0", q * 16, &st);  // light walks, then camera walks
+    A.w1 = (uint4*)ctx->buf("bd_w1", q * 16, &st);
     A.nl = (uint32_t*)ctx->buf("bd_nl", (size_t)C * 4, &st);
     A.nc = (uint32_t*)ctx->buf("bd_nc", (size_t)C * 4, &st);
-    A.nrec = (uint32_t*)ctx->buf("bd_nrec", (size_t)C * 4, &st);
-    A.rec = (float4*)ctx->buf("bd_rec", (size_t)C * ns * 16, &st);
-    A.rec_fp = (float2*)ctx->buf("bd_rec_fp", (size_t)C * ns * 8, &st);
-    A.stage_o = (float4*)ctx->buf("bd_stage_o", (size_t)C * ns * 16, &st);
-    A.stage_d = (float4*)ctx->buf("bd_stage_d", (size_t)C * ns * 16, &st);
-    A.nray = (uint32_t*)ctx->buf("bd_nray", (size_t)C * ns * 4, &st);
+    A.rec = (float4*)ctx->buf("bd_rec", (size_t)M * ns * 16, &st);
+    A.rec_fp = (float2*)ctx->buf("bd_rec_fp", (size_t)M * ns * 8, &st);
     P.pend = A.rec;  // k_connect clears an occluded ray's record (its .w)
   }
   const bool dev = o->flags & DGPU_BUF_DEVICE;
@@ -1186,7 +1190,7 @@ dgpu_status dgpu_render_bdpt(dgpu_scene* s, const dgpu_render_opts* o, uint32_t 
     P.q0 = (uint32_t)(g0 % npix);
     P.s0 = o->sample_begin + (uint32_t)(g0 / npix);
     if (mega) HIP_TRY(launch_bdpt(P, A, s->S.width, stream));
-    else HIP_TRY(launch_bdpt_wave(P, A, ce, cc, shade_grid, stream));
+    else HIP_TRY(launch_bdpt_wave(P, A, ce, cc, shade_grid, M, stream));
   }
   HIP_TRY(hipEventRecord(ctx->ev1, stream));
   unsigned long long hstats[STATS_WORDS] = {};

@@ -391,6 +391,20 @@ struct BdPath {
     uint32_t j = s + t - 1u - k;
     return (j == 0u && t == 1u) ? first_camera : bd_load(A->cv, A->n, i, j);
   }
+  // one word of vertex k's record c (1: normal + forward pdf, 2: throughput + backward
+  // pdf, 3: triangle + barycentrics + info), without loading the rest of the vertex
+  __device__ float word(uint32_t k, uint32_t c) const {
+    if (k < s) {
+      if (k == 0u && s == 1u) return c == 1u ? first_light.fwd : c == 2u ? first_light.bwd : __uint_as_float(first_light.info);
+      return A->lv[(4 * (size_t)k + c) * A->n + i].w;
+    }
+    uint32_t j = s + t - 1u - k;
+    if (j == 0u && t == 1u) return c == 1u ? first_camera.fwd : c == 2u ? first_camera.bwd : __uint_as_float(first_camera.info);
+    return A->cv[(4 * (size_t)j + c) * A->n + i].w;
+  }
+  __device__ float fwd(uint32_t k) const { return word(k, 1u); }
+  __device__ float bwd(uint32_t k) const { return word(k, 2u); }
+  __device__ bool delta(uint32_t k) const { return (__float_as_uint(word(k, 3u)) & 1u) != 0u; }
 };
 
 // path_weight (bdpt_renderer.cpp:468-667) for a strategy with s + t > 2.  L: the path's
@@ -403,7 +417,7 @@ __device__ float bd_weight(const WaveParams& P, const BdPath& X, const DevLight&
   const bool distant = L.kind != LK_AREA;
   float x0_pivot;
   if (s == 1u) {
-    x0_pivot = X.at(0).fwd;
+    x0_pivot = X.fwd(0);
   } else {
     BdV x0 = X.at(0), x1 = X.at(1);
     V3 wi = normalize(x0.p - x1.p);
@@ -426,7 +440,7 @@ __device__ float bd_weight(const WaveParams& P, const BdPath& X, const DevLight&
       }
       return ray_pdf / x0_pivot;
     }
-    if (k < s) return X.at(k).fwd;
+    if (k < s) return X.fwd(k);
     if (k == s && t != 1u) {
       BdV xo = X.at(k), x = X.at(k - 1u), xi = X.at(k - 2u);
       V3 bwi = normalize(xi.p - x.p), bwo = normalize(xo.p - x.p);
@@ -435,7 +449,7 @@ __device__ float bd_weight(const WaveParams& P, const BdPath& X, const DevLight&
       if (d2 == 0.f) return 0.f;
       return wo_pdf * fabsf(dot(bwo, xo.nn)) / d2;
     }
-    return X.at(k).bwd;  // s < k < s + t - 1
+    return X.bwd(k);  // s < k < s + t - 1
   };
   // pdf of sampling x_k from x_{k+1} (from the camera side)
   auto pdf_camera = [&](uint32_t k) -> float {
@@ -449,7 +463,7 @@ __device__ float bd_weight(const WaveParams& P, const BdPath& X, const DevLight&
       if (pivot == 0.f) return 0.f;
       return ray_area / pivot;
     }
-    if (k >= s) return X.at(k).fwd;  // s <= k < s + t - 2
+    if (k >= s) return X.fwd(k);  // s <= k < s + t - 2
     if (k + 1u == s && t >= 2u) {
       BdV xi = X.at(k), x = X.at(k + 1u), xo = X.at(k + 2u);
       V3 bwi = normalize(xi.p - x.p), bwo = normalize(xo.p - x.p);
@@ -457,7 +471,7 @@ __device__ float bd_weight(const WaveParams& P, const BdPath& X, const DevLight&
       if (k == 0u && distant) return wi_pdf;
       return wi_pdf * fabsf(dot(xi.nn, bwi)) / length_sq(xi.p - x.p);
     }
-    return X.at(k).bwd;  // k < s - 1
+    return X.bwd(k);  // k < s - 1
   };
   float inv = 1.f, r = 1.f;
   for (uint32_t j = 1; j + 1u <= t; ++j) {
@@ -465,7 +479,7 @@ __device__ float bd_weight(const WaveParams& P, const BdPath& X, const DevLight&
     if (bw == 0.f) break;
     r *= fw / bw;
     if (r == 0.f) break;
-    if (bv_delta(X.at(s + j - 1u)) || bv_delta(X.at(s + j))) continue;
+    if (X.delta(s + j - 1u) || X.delta(s + j)) continue;
     inv += r;
   }
   r = 1.f;
@@ -474,7 +488,7 @@ __device__ float bd_weight(const WaveParams& P, const BdPath& X, const DevLight&
     if (bw == 0.f) break;
     r *= fw / bw;
     if (r == 0.f) break;
-    if (bv_delta(X.at(s - j)) || (j < s && bv_delta(X.at(s - j - 1u)))) continue;
+    if (X.delta(s - j) || (j < s && X.delta(s - j - 1u))) continue;
     inv += r;
   }
   return 1.f / inv;
@@ -696,12 +710,13 @@ __global__ __launch_bounds__(BD_BLOCK) void k_bdpt(WaveParams P, BdptArgs A) {
 // ---------------------------------------------------------------- wavefront form
 //
 // The chunk of k_bdpt in stages that reuse the path tracer's trace kernels:
-//   k_bd_lstart, then per bounce k_extend + k_bd_lshade: the light subpaths;
-//   k_bd_cstart, then per bounce k_extend + k_bd_cshade: the camera subpaths;
-//   k_bd_strat: every strategy's contribution and MIS weight (bd_strategy), recorded per
-//     path in strategy order, its shadow ray staged; k_bd_gather: the staged rays into one
-//     queue; k_connect: their visibility (an occluded ray clears its record);
-//   k_bd_resolve: the visible records, summed in strategy order (t >= 2) or splatted.
+//   k_bd_start, then per bounce k_extend + k_bd_shade: both subpaths of every path, the
+//     light and the camera rays of one bounce traced by one launch;
+//   then per part of the chunk (the strategy records need far more memory per path than
+//   the subpaths): k_bd_strat: one thread per strategy slot and path, its contribution
+//     and MIS weight (bd_strategy), recorded in the slot, its shadow ray queued;
+//     k_connect: their visibility (an occluded ray clears its record); k_bd_resolve: the
+//     visible records, summed in strategy order (t >= 2) or splatted.
 // Each stage draws the random numbers k_bdpt draws at the same point of the walk, so both
 // forms trace the same paths.  Vertices are stored as soon as they exist, and a vertex's
 // backward pdf, known once the next vertex is, is patched into its record then.
@@ -710,330 +725,359 @@ __device__ __forceinline__ void bd_patch_bwd(float4* base, uint32_t n, uint32_t 
   base[(size_t)4 * k * n + 2 * (size_t)n + i].w = bwd;
 }
 
-// Light subpath, vertex 0 and the first ray (random_light_walk, bdpt_renderer.cpp:118-146).
-__global__ __launch_bounds__(SHADE_BLOCK) void k_bd_lstart(WaveParams P, BdptArgsHost A) {
-  __shared__ uint32_t sh[SHADE_BLOCK / 64 + 1];
-  const uint32_t i = blockIdx.x * SHADE_BLOCK + threadIdx.x;
-  bool go = false;
-  float4 qo = make_float4(0.f, 0.f, 0.f, 0.f), qd = qo;  // the first ray, if any
-  if (i < P.n_chunk) {
-    const DevScene& S = P.S;
-    uint32_t pixel, sample;
-    order_sample_pixel(P.ord, P.s0, P.q0 + i, sample, pixel);
-    float r0[4];
-    bd_rng(P, pixel, sample, 0, 0, r0);
-    float pick;
-    const uint32_t light = lt_pick_light(S, r0[2], pick);
-    uint32_t nvert = 0;
-    if (pick != 0.f) {
-      const DevLight L = S.lights[light];
-      float r[4];
-      bd_rng(P, pixel, sample, 0, 1, r);
-      const LtRay lr = lt_light_ray(S, L, r);
-      if (lr.pos_pdf != 0.f && lr.dir_pdf != 0.f) {
-        BdV prev;
-        prev.p = lr.o; prev.eps = lr.tmin; prev.nn = lr.nn;
-        prev.fwd = lr.pos_pdf; prev.bwd = 0.f;
-        prev.alpha = lr.rad / (lr.pos_pdf * pick);
-        prev.leaf = 0xFFFFFFFFu; prev.b1 = prev.b2 = 0.f;
-        prev.info = BV_LIGHT << 1;
-        bd_store(A.lv, A.n, i, 0, prev);
-        nvert = 1;
-        A.w0[i] = make_float4(1.f, 1.f, 1.f, lr.dir_pdf);
-        A.w1[i] = make_uint4(1u, light, L.kind != LK_AREA ? 1u : 0u, 0u);
-        go = nvert < A.nv;
-        qo = make_float4(lr.o.x, lr.o.y, lr.o.z, lr.tmin);
-        qd = make_float4(lr.d.x, lr.d.y, lr.d.z, INFINITY);
-      }
-    }
-    A.nl[i] = nvert | (light << 8);
+// Queue words of the walks: the path index, with the top bit set for a camera ray.
+constexpr uint32_t BD_CAMERA_RAY = 0x80000000u;
+
+// Appends two classes of entries for a whole workgroup of SHADE_BLOCK threads with one
+// atomic: the workgroup's `a` entries first, then its `b` entries, so that the waves
+// which later read the queue mostly see runs of one class.  sh holds 2 * waves + 1 words.
+__device__ __forceinline__ void block_append2(uint32_t* counter, bool a, bool b, uint32_t* sh,
+                                              uint32_t& qa, uint32_t& qb) {
+  constexpr uint32_t WAVES = SHADE_BLOCK / 64;
+  const unsigned long long ma = __ballot(a), mb = __ballot(b);
+  const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
+  if (lane == 0) {
+    sh[wave] = (uint32_t)__popcll(ma);
+    sh[WAVES + 1 + wave] = (uint32_t)__popcll(mb);
   }
-  const uint32_t q = block_append(&P.counters[CNT_QUEUE], go, sh);
-  if (go) {
-    P.qo[0][q] = qo;
-    P.qd[0][q] = qd;
-    P.qpath[0][q] = i;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t ta = 0, tb = 0;
+    for (uint32_t w = 0; w < WAVES; ++w) { uint32_t c = sh[w]; sh[w] = ta; ta += c; }
+    for (uint32_t w = 0; w < WAVES; ++w) { uint32_t c = sh[WAVES + 1 + w]; sh[WAVES + 1 + w] = ta + tb; tb += c; }
+    sh[WAVES] = ta + tb ? atomicAdd(counter, ta + tb) : 0u;
   }
+  __syncthreads();
+  const unsigned long long below = lane == 0 ? 0ull : ((1ull << lane) - 1ull);
+  qa = sh[WAVES] + sh[wave] + (uint32_t)__popcll(ma & below);
+  qb = sh[WAVES] + sh[WAVES + 1 + wave] + (uint32_t)__popcll(mb & below);
 }
 
-// One bounce of random_light_walk (bdpt_renderer.cpp:146-181) for the light rays of bounce b.
-__global__ __launch_bounds__(SHADE_BLOCK) void k_bd_lshade(WaveParams P, BdptArgsHost A, uint32_t b) {
-  __shared__ uint32_t sh[SHADE_BLOCK / 64 + 1];
+// Light subpath of path i, vertex 0 and the first ray (random_light_walk,
+// bdpt_renderer.cpp:118-146).  Walk state at A.w0 / A.w1[i].
+__device__ __forceinline__ bool bd_light_start(const WaveParams& P, const BdptArgsHost& A, uint32_t i,
+                                               uint32_t pixel, uint32_t sample, float4& qo, float4& qd) {
   const DevScene& S = P.S;
-  const uint32_t cur = b & 1u, nxt = cur ^ 1u;
-  const uint32_t n = P.counters[CNT_QUEUE + b];
-  for (uint32_t base = blockIdx.x * SHADE_BLOCK; base < n; base += gridDim.x * SHADE_BLOCK) {
-    const uint32_t i = base + threadIdx.x;
-    bool go = false;
-    uint32_t p = 0;
-    V3 go_o = v3(0.f, 0.f, 0.f), go_d = go_o;
-    float go_eps = 0.f;
-    if (i < n) {
-      p = P.qpath[cur][i];
-      const float4 h4 = P.hit[i];
-      const uint32_t leaf = __float_as_uint(h4.w);
-      if (leaf != 0xFFFFFFFFu) {
-        const float4 w0 = A.w0[p];
-        const uint4 w1 = A.w1[p];
-        uint32_t nvert = w1.x;
-        const bool distant = w1.z != 0u;
-        const V3 prev_f = xyz(w0);
-        const float fwd_dir_pdf = w0.w;
-        uint32_t pixel, sample;
-        order_sample_pixel(P.ord, P.s0, P.q0 + p, sample, pixel);
-        Surface g = surface_at(S, leaf, h4.x, h4.y, h4.z);
-        Bsdf bsdf = make_bsdf(S, g);
-        const V3 wi = normalize(-xyz(P.qd[cur][i]));
-        float rb[4];
-        bd_rng(P, pixel, sample, 1u + b, 0, rb);
-        V3 wo;
-        float wo_pdf;
-        uint32_t fl;
-        V3 f = bsdf_sample(bsdf, wi, BSDF_ALL, rb[0], rb[1], rb[2], wo, wo_pdf, fl);
-        if (wo_pdf != 0.f) {
-          const float bwd_dir_pdf = bsdf_pdf(bsdf, wi, wo, BSDF_ALL);
-          const BdV prev = bd_load(A.lv, A.n, p, nvert - 1u);
-          const float d2 = length_sq(g.p - prev.p);
-          if (d2 >= 1e-9f) {
-            V3 scale = prev_f * (fabsf(dot(prev.nn, wi)) / fwd_dir_pdf);
-            bool live = true;
-            if (b >= 2u) {
-              float rr = fminf(average(scale), 0.9f);
-              if (rb[3] > rr) live = false;
-              else scale = scale / rr;
-            }
-            if (live) {
-              BdV y;
-              y.p = g.p; y.eps = g.eps; y.nn = g.nn;
-              y.fwd = (b == 0u && distant) ? fwd_dir_pdf : fwd_dir_pdf * fabsf(dot(g.nn, wi)) / d2;
-              y.bwd = 0.f;
-              y.alpha = prev.alpha * scale;
-              y.leaf = leaf; y.b1 = h4.y; y.b2 = h4.z;
-              y.info = (BV_SURFACE << 1) | ((fl & BSDF_DELTA) ? 1u : 0u);
-              if (!is_black(y.alpha)) {
-                bd_patch_bwd(A.lv, A.n, p, nvert - 1u, bwd_dir_pdf * fabsf(dot(prev.nn, wi)) / d2);
-                bd_store(A.lv, A.n, p, nvert, y);
-                ++nvert;
-                A.nl[p] = nvert | (w1.y << 8);
-                A.w0[p] = make_float4(f.x, f.y, f.z, wo_pdf);
-                A.w1[p] = make_uint4(nvert, w1.y, w1.z, 0u);
-                go = b + 1u <= P.max_depth && nvert < A.nv;
-                go_o = g.p; go_d = wo; go_eps = g.eps;
-              }
-            }
-          }
-        }
-      }
-    }
-    const uint32_t q = block_append(&P.counters[CNT_QUEUE + b + 1u], go, sh);
-    if (go) {
-      P.qo[nxt][q] = make_float4(go_o.x, go_o.y, go_o.z, go_eps);
-      P.qd[nxt][q] = make_float4(go_d.x, go_d.y, go_d.z, INFINITY);
-      P.qpath[nxt][q] = p;
+  float r0[4];
+  bd_rng(P, pixel, sample, 0, 0, r0);
+  float pick;
+  const uint32_t light = lt_pick_light(S, r0[2], pick);
+  uint32_t nvert = 0;
+  bool go = false;
+  if (pick != 0.f) {
+    const DevLight L = S.lights[light];
+    float r[4];
+    bd_rng(P, pixel, sample, 0, 1, r);
+    const LtRay lr = lt_light_ray(S, L, r);
+    if (lr.pos_pdf != 0.f && lr.dir_pdf != 0.f) {
+      BdV prev;
+      prev.p = lr.o; prev.eps = lr.tmin; prev.nn = lr.nn;
+      prev.fwd = lr.pos_pdf; prev.bwd = 0.f;
+      prev.alpha = lr.rad / (lr.pos_pdf * pick);
+      prev.leaf = 0xFFFFFFFFu; prev.b1 = prev.b2 = 0.f;
+      prev.info = BV_LIGHT << 1;
+      bd_store(A.lv, A.n, i, 0, prev);
+      nvert = 1;
+      A.w0[i] = make_float4(1.f, 1.f, 1.f, lr.dir_pdf);
+      A.w1[i] = make_uint4(1u, light, L.kind != LK_AREA ? 1u : 0u, 0u);
+      go = nvert < A.nv;
+      qo = make_float4(lr.o.x, lr.o.y, lr.o.z, lr.tmin);
+      qd = make_float4(lr.d.x, lr.d.y, lr.d.z, INFINITY);
     }
   }
+  A.nl[i] = nvert | (light << 8);
+  return go;
 }
 
-// Camera subpath, vertex 0 and the first ray (random_camera_walk, bdpt_renderer.cpp:183-204).
-__global__ __launch_bounds__(SHADE_BLOCK) void k_bd_cstart(WaveParams P, BdptArgsHost A) {
-  __shared__ uint32_t sh[SHADE_BLOCK / 64 + 1];
+// Camera subpath of path i, vertex 0 and the first ray (random_camera_walk,
+// bdpt_renderer.cpp:183-204).  Walk state at A.w0 / A.w1[A.n + i].
+__device__ __forceinline__ bool bd_camera_start(const WaveParams& P, const BdptArgsHost& A, uint32_t i,
+                                                uint32_t pixel, uint32_t sample, float4& qo, float4& qd) {
   const DevScene& S = P.S;
+  float r0[4];
+  bd_rng(P, pixel, sample, 0, 0, r0);
+  const float fx = float(pixel % P.xr) + r0[0], fy = float(pixel / P.xr) + r0[1];
+  V3 o, d;
+  camera_ray(S, fx, fy, P.xr, P.yr, o, d);
+  const float dir_pdf = bd_camera_pdf_at(P, fx, fy);
+  BdV prev;
+  prev.p = o; prev.eps = 0.f; prev.nn = d;
+  prev.fwd = 1.f; prev.bwd = 0.f;
+  prev.alpha = v3(dir_pdf, dir_pdf, dir_pdf);  // importance / pos_pdf (pos_pdf = 1)
+  prev.leaf = 0xFFFFFFFFu; prev.b1 = prev.b2 = 0.f;
+  prev.info = BV_CAMERA << 1;
+  bd_store(A.cv, A.n, i, 0, prev);
+  A.nc[i] = 1u;
+  A.w1[A.n + i] = make_uint4(1u, 0u, 0u, 0u);
+  if (is_black(prev.alpha) || A.nv <= 1u) return false;
+  const V3 scale = v3(1.f, 1.f, 1.f) * (fabsf(dot(prev.nn, d)) / dir_pdf);
+  A.w0[A.n + i] = make_float4(scale.x, scale.y, scale.z, dir_pdf);
+  qo = make_float4(o.x, o.y, o.z, 0.f);
+  qd = make_float4(d.x, d.y, d.z, INFINITY);
+  return true;
+}
+
+// Both subpaths' first vertex and first ray; the workgroup's light rays are queued ahead
+// of its camera rays.
+__global__ __launch_bounds__(SHADE_BLOCK) void k_bd_start(WaveParams P, BdptArgsHost A) {
+  __shared__ uint32_t sh[2 * (SHADE_BLOCK / 64) + 1];
   const uint32_t i = blockIdx.x * SHADE_BLOCK + threadIdx.x;
-  bool go = false;
-  V3 o = v3(0.f, 0.f, 0.f), d = o;
+  bool gl = false, gc = false;
+  float4 lo = make_float4(0.f, 0.f, 0.f, 0.f), ld = lo, co = lo, cd = lo;
   if (i < P.n_chunk) {
     uint32_t pixel, sample;
     order_sample_pixel(P.ord, P.s0, P.q0 + i, sample, pixel);
-    float r0[4];
-    bd_rng(P, pixel, sample, 0, 0, r0);
-    const float fx = float(pixel % P.xr) + r0[0], fy = float(pixel / P.xr) + r0[1];
-    camera_ray(S, fx, fy, P.xr, P.yr, o, d);
-    const float dir_pdf = bd_camera_pdf_at(P, fx, fy);
-    BdV prev;
-    prev.p = o; prev.eps = 0.f; prev.nn = d;
-    prev.fwd = 1.f; prev.bwd = 0.f;
-    prev.alpha = v3(dir_pdf, dir_pdf, dir_pdf);  // importance / pos_pdf (pos_pdf = 1)
-    prev.leaf = 0xFFFFFFFFu; prev.b1 = prev.b2 = 0.f;
-    prev.info = BV_CAMERA << 1;
-    bd_store(A.cv, A.n, i, 0, prev);
-    A.nc[i] = 1u;
-    A.w1[i] = make_uint4(1u, 0u, 0u, 0u);
-    if (!is_black(prev.alpha) && A.nv > 1u) {
-      const V3 scale = v3(1.f, 1.f, 1.f) * (fabsf(dot(prev.nn, d)) / dir_pdf);
-      A.w0[i] = make_float4(scale.x, scale.y, scale.z, dir_pdf);
-      go = true;
-    }
+    gl = bd_light_start(P, A, i, pixel, sample, lo, ld);
+    gc = bd_camera_start(P, A, i, pixel, sample, co, cd);
   }
-  const uint32_t q = block_append(&P.counters[CNT_QUEUE], go, sh);
-  if (go) {
-    P.qo[0][q] = make_float4(o.x, o.y, o.z, 0.f);
-    P.qd[0][q] = make_float4(d.x, d.y, d.z, INFINITY);
-    P.qpath[0][q] = i;
+  uint32_t ql, qc;
+  block_append2(&P.counters[CNT_QUEUE], gl, gc, sh, ql, qc);
+  if (gl) {
+    P.qo[0][ql] = lo;
+    P.qd[0][ql] = ld;
+    P.qpath[0][ql] = i;
+  }
+  if (gc) {
+    P.qo[0][qc] = co;
+    P.qd[0][qc] = cd;
+    P.qpath[0][qc] = i | BD_CAMERA_RAY;
   }
 }
 
-// One bounce of random_camera_walk (bdpt_renderer.cpp:204-273) for the camera rays of
-// bounce b; the Russian roulette of the next bounce decides whether its ray is queued.
-__global__ __launch_bounds__(SHADE_BLOCK) void k_bd_cshade(WaveParams P, BdptArgsHost A, uint32_t b) {
-  __shared__ uint32_t sh[SHADE_BLOCK / 64 + 1];
+// One bounce of random_light_walk (bdpt_renderer.cpp:146-181) for the light ray at queue
+// slot i of bounce b (path p); returns whether the next ray is queued.
+__device__ __forceinline__ bool bd_light_step(const WaveParams& P, const BdptArgsHost& A, uint32_t b, uint32_t i,
+                                              uint32_t p, V3& go_o, V3& go_d, float& go_eps) {
   const DevScene& S = P.S;
-  const uint32_t cur = b & 1u, nxt = cur ^ 1u;
-  const uint32_t n = P.counters[CNT_QUEUE + b];
-  for (uint32_t base = blockIdx.x * SHADE_BLOCK; base < n; base += gridDim.x * SHADE_BLOCK) {
-    const uint32_t i = base + threadIdx.x;
-    bool go = false;
-    uint32_t p = 0;
-    V3 go_o = v3(0.f, 0.f, 0.f), go_d = go_o;
-    float go_eps = 0.f;
-    if (i < n) {
-      p = P.qpath[cur][i];
-      const float4 h4 = P.hit[i];
-      const uint32_t leaf = __float_as_uint(h4.w);
-      const float4 w0 = A.w0[p];
-      uint32_t nvert = A.w1[p].x;
-      const V3 scale = xyz(w0);
-      const float fwd_dir_pdf = w0.w;
-      const V3 rd = xyz(P.qd[cur][i]);
-      const BdV prev = bd_load(A.cv, A.n, p, nvert - 1u);
-      uint32_t pixel, sample;
-      order_sample_pixel(P.ord, P.s0, P.q0 + p, sample, pixel);
-      if (leaf == 0xFFFFFFFFu) {
-        // the "virtual vertex" of the escaped ray (bdpt_renderer.cpp:223-239)
-        float r2[4];
-        bd_rng(P, pixel, sample, 1u + b, 2, r2);
-        uint32_t bg = 0;
-        const float bgp = bd_pick_background(S, r2[0], bg);
-        if (bgp != 0.f) {
-          BdV z;
-          z.p = prev.p + rd; z.eps = 0.f; z.nn = v3(0.f, 0.f, 0.f);
-          z.fwd = fwd_dir_pdf; z.bwd = 0.f;
-          z.alpha = prev.alpha * scale / bgp;
-          z.leaf = 0xFFFFFFFFu; z.b1 = z.b2 = 0.f;
-          z.info = (BV_BACKGROUND << 1) | (bg << 8);
-          bd_store(A.cv, A.n, p, nvert, z);
-          ++nvert;
-          A.nc[p] = nvert;
-        }
-      } else {
-        Surface g = surface_at(S, leaf, h4.x, h4.y, h4.z);
-        Bsdf bsdf = make_bsdf(S, g);
-        const V3 wo = -rd;
-        float rb[4];
-        bd_rng(P, pixel, sample, 1u + b, 1, rb);
-        V3 wi;
-        float wi_pdf;
-        uint32_t fl;
-        V3 f = bsdf_sample(bsdf, wo, BSDF_ALL, rb[1], rb[2], rb[3], wi, wi_pdf, fl);  // sample_light_f
-        const float d2 = length_sq(g.p - prev.p);
-        if (d2 >= 1e-9f) {
-          const uint32_t emitter = g.light >= 0 ? (uint32_t)g.light + 1u : 0u;
-          BdV z;
-          z.p = g.p; z.eps = g.eps; z.nn = g.nn;
-          z.fwd = fwd_dir_pdf * fabsf(dot(g.nn, wo)) / d2;
-          z.bwd = 0.f;
-          z.alpha = prev.alpha * scale;
-          z.leaf = leaf; z.b1 = h4.y; z.b2 = h4.z;
-          z.info = (BV_SURFACE << 1) | ((fl & BSDF_DELTA) ? 1u : 0u) | (emitter << 8);
-          if (!(is_black(z.alpha) && emitter == 0u)) {
-            if (wi_pdf != 0.f) {
-              const float bwd_dir_pdf = bsdf_pdf(bsdf, wo, wi, BSDF_ALL);  // camera_f_pdf(wo, wi)
-              bd_patch_bwd(A.cv, A.n, p, nvert - 1u, bwd_dir_pdf * fabsf(dot(prev.nn, wo)) / d2);
-            }
-            bd_store(A.cv, A.n, p, nvert, z);
-            ++nvert;
-            A.nc[p] = nvert;
-            A.w1[p] = make_uint4(nvert, 0u, 0u, 0u);
-            // the top of the next bounce: its Russian roulette, before its ray is traced
-            if (wi_pdf != 0.f && b + 1u <= P.max_depth && nvert < A.nv && !is_black(z.alpha)) {
-              float rn[4];
-              bd_rng(P, pixel, sample, 2u + b, 1, rn);
-              V3 sc = f * (fabsf(dot(z.nn, wi)) / wi_pdf);
-              bool live = true;
-              if (b + 1u >= 2u) {
-                float rr = fminf(average(sc), 0.9f);
-                if (rn[0] > rr) live = false;
-                else sc = sc / rr;
-              }
-              if (live) {
-                A.w0[p] = make_float4(sc.x, sc.y, sc.z, wi_pdf);
-                go = true;
-                go_o = g.p; go_d = wi; go_eps = g.eps;
-              }
-            }
-          }
-        }
-      }
-    }
-    const uint32_t q = block_append(&P.counters[CNT_QUEUE + b + 1u], go, sh);
-    if (go) {
-      P.qo[nxt][q] = make_float4(go_o.x, go_o.y, go_o.z, go_eps);
-      P.qd[nxt][q] = make_float4(go_d.x, go_d.y, go_d.z, INFINITY);
-      P.qpath[nxt][q] = p;
-    }
-  }
-}
-
-// Every strategy of path i (sample_path's loops, t outer, s inner): its weighted
-// contribution as a record (w = 1: summed, 2: splatted) in strategy order, and the shadow
-// ray of a strategy that needs one, staged beside the path with its record's index.
-__global__ __launch_bounds__(SHADE_BLOCK) void k_bd_strat(WaveParams P, BdptArgsHost H) {
-  const uint32_t i = blockIdx.x * SHADE_BLOCK + threadIdx.x;
-  if (i >= P.n_chunk) return;
-  const DevScene& S = P.S;
-  BdptArgs A;
-  A.lv = H.lv; A.cv = H.cv; A.n = H.n; A.nv = H.nv; A.use_t1 = H.use_t1; A.splat = H.splat;
+  const uint32_t cur = b & 1u;
+  const float4 h4 = P.hit[i];
+  const uint32_t leaf = __float_as_uint(h4.w);
+  if (leaf == 0xFFFFFFFFu) return false;
+  const float4 w0 = A.w0[p];
+  const uint4 w1 = A.w1[p];
+  uint32_t nvert = w1.x;
+  const bool distant = w1.z != 0u;
+  const V3 prev_f = xyz(w0);
+  const float fwd_dir_pdf = w0.w;
   uint32_t pixel, sample;
-  order_sample_pixel(P.ord, P.s0, P.q0 + i, sample, pixel);
-  const uint32_t lw = H.nl[i], nl = lw & 0xFFu, light = lw >> 8, nc = H.nc[i];
-  const size_t base = (size_t)i * H.ns;
-  uint32_t nrec = 0, nray = 0;
-  BdPath X;
-  X.A = &A;
-  X.i = i;
-  for (uint32_t t = 1; t <= nc; ++t) {
-    const BdV zc = bd_load(A.cv, A.n, i, t - 1u);
-    const Bsdf bc = bv_kind(zc) == BV_SURFACE ? bd_bsdf(S, zc) : bd_no_bsdf();
-    for (uint32_t s = 0; s <= nl; ++s) {
+  order_sample_pixel(P.ord, P.s0, P.q0 + p, sample, pixel);
+  Surface g = surface_at(S, leaf, h4.x, h4.y, h4.z);
+  Bsdf bsdf = make_bsdf(S, g);
+  const V3 wi = normalize(-xyz(P.qd[cur][i]));
+  float rb[4];
+  bd_rng(P, pixel, sample, 1u + b, 0, rb);
+  V3 wo;
+  float wo_pdf;
+  uint32_t fl;
+  V3 f = bsdf_sample(bsdf, wi, BSDF_ALL, rb[0], rb[1], rb[2], wo, wo_pdf, fl);
+  if (wo_pdf == 0.f) return false;
+  const float bwd_dir_pdf = bsdf_pdf(bsdf, wi, wo, BSDF_ALL);
+  const BdV prev = bd_load(A.lv, A.n, p, nvert - 1u);
+  const float d2 = length_sq(g.p - prev.p);
+  if (d2 < 1e-9f) return false;
+  V3 scale = prev_f * (fabsf(dot(prev.nn, wi)) / fwd_dir_pdf);
+  if (b >= 2u) {
+    float rr = fminf(average(scale), 0.9f);
+    if (rb[3] > rr) return false;
+    scale = scale / rr;
+  }
+  BdV y;
+  y.p = g.p; y.eps = g.eps; y.nn = g.nn;
+  y.fwd = (b == 0u && distant) ? fwd_dir_pdf : fwd_dir_pdf * fabsf(dot(g.nn, wi)) / d2;
+  y.bwd = 0.f;
+  y.alpha = prev.alpha * scale;
+  y.leaf = leaf; y.b1 = h4.y; y.b2 = h4.z;
+  y.info = (BV_SURFACE << 1) | ((fl & BSDF_DELTA) ? 1u : 0u);
+  if (is_black(y.alpha)) return false;
+  bd_patch_bwd(A.lv, A.n, p, nvert - 1u, bwd_dir_pdf * fabsf(dot(prev.nn, wi)) / d2);
+  bd_store(A.lv, A.n, p, nvert, y);
+  ++nvert;
+  A.nl[p] = nvert | (w1.y << 8);
+  A.w0[p] = make_float4(f.x, f.y, f.z, wo_pdf);
+  A.w1[p] = make_uint4(nvert, w1.y, w1.z, 0u);
+  go_o = g.p; go_d = wo; go_eps = g.eps;
+  return b + 1u <= P.max_depth && nvert < A.nv;
+}
+
+// One bounce of random_camera_walk (bdpt_renderer.cpp:204-273) for the camera ray at
+// queue slot i of bounce b (path p); the Russian roulette of the next bounce decides
+// whether its ray is queued.
+__device__ __forceinline__ bool bd_camera_step(const WaveParams& P, const BdptArgsHost& A, uint32_t b, uint32_t i,
+                                               uint32_t p, V3& go_o, V3& go_d, float& go_eps) {
+  const DevScene& S = P.S;
+  const uint32_t cur = b & 1u;
+  const float4 h4 = P.hit[i];
+  const uint32_t leaf = __float_as_uint(h4.w);
+  const float4 w0 = A.w0[A.n + p];
+  uint32_t nvert = A.w1[A.n + p].x;
+  const V3 scale = xyz(w0);
+  const float fwd_dir_pdf = w0.w;
+  const V3 rd = xyz(P.qd[cur][i]);
+  const BdV prev = bd_load(A.cv, A.n, p, nvert - 1u);
+  uint32_t pixel, sample;
+  order_sample_pixel(P.ord, P.s0, P.q0 + p, sample, pixel);
+  if (leaf == 0xFFFFFFFFu) {
+    // the "virtual vertex" of the escaped ray (bdpt_renderer.cpp:223-239)
+    float r2[4];
+    bd_rng(P, pixel, sample, 1u + b, 2, r2);
+    uint32_t bg = 0;
+    const float bgp = bd_pick_background(S, r2[0], bg);
+    if (bgp != 0.f) {
+      BdV z;
+      z.p = prev.p + rd; z.eps = 0.f; z.nn = v3(0.f, 0.f, 0.f);
+      z.fwd = fwd_dir_pdf; z.bwd = 0.f;
+      z.alpha = prev.alpha * scale / bgp;
+      z.leaf = 0xFFFFFFFFu; z.b1 = z.b2 = 0.f;
+      z.info = (BV_BACKGROUND << 1) | (bg << 8);
+      bd_store(A.cv, A.n, p, nvert, z);
+      A.nc[p] = nvert + 1u;
+    }
+    return false;
+  }
+  Surface g = surface_at(S, leaf, h4.x, h4.y, h4.z);
+  Bsdf bsdf = make_bsdf(S, g);
+  const V3 wo = -rd;
+  float rb[4];
+  bd_rng(P, pixel, sample, 1u + b, 1, rb);
+  V3 wi;
+  float wi_pdf;
+  uint32_t fl;
+  V3 f = bsdf_sample(bsdf, wo, BSDF_ALL, rb[1], rb[2], rb[3], wi, wi_pdf, fl);  // sample_light_f
+  const float d2 = length_sq(g.p - prev.p);
+  if (d2 < 1e-9f) return false;
+  const uint32_t emitter = g.light >= 0 ? (uint32_t)g.light + 1u : 0u;
+  BdV z;
+  z.p = g.p; z.eps = g.eps; z.nn = g.nn;
+  z.fwd = fwd_dir_pdf * fabsf(dot(g.nn, wo)) / d2;
+  z.bwd = 0.f;
+  z.alpha = prev.alpha * scale;
+  z.leaf = leaf; z.b1 = h4.y; z.b2 = h4.z;
+  z.info = (BV_SURFACE << 1) | ((fl & BSDF_DELTA) ? 1u : 0u) | (emitter << 8);
+  if (is_black(z.alpha) && emitter == 0u) return false;
+  if (wi_pdf != 0.f) {
+    const float bwd_dir_pdf = bsdf_pdf(bsdf, wo, wi, BSDF_ALL);  // camera_f_pdf(wo, wi)
+    bd_patch_bwd(A.cv, A.n, p, nvert - 1u, bwd_dir_pdf * fabsf(dot(prev.nn, wo)) / d2);
+  }
+  bd_store(A.cv, A.n, p, nvert, z);
+  ++nvert;
+  A.nc[p] = nvert;
+  A.w1[A.n + p] = make_uint4(nvert, 0u, 0u, 0u);
+  // the top of the next bounce: its Russian roulette, before its ray is traced
+  if (wi_pdf == 0.f || b + 1u > P.max_depth || nvert >= A.nv || is_black(z.alpha)) return false;
+  float rn[4];
+  bd_rng(P, pixel, sample, 2u + b, 1, rn);
+  V3 sc = f * (fabsf(dot(z.nn, wi)) / wi_pdf);
+  if (b + 1u >= 2u) {
+    float rr = fminf(average(sc), 0.9f);
+    if (rn[0] > rr) return false;
+    sc = sc / rr;
+  }
+  A.w0[A.n + p] = make_float4(sc.x, sc.y, sc.z, wi_pdf);
+  go_o = g.p; go_d = wi; go_eps = g.eps;
+  return true;
+}
+
+// One bounce of both walks: each traced ray of bounce b extends its own subpath.
+__global__ __launch_bounds__(SHADE_BLOCK) void k_bd_shade(WaveParams P, BdptArgsHost A, uint32_t b) {
+  __shared__ uint32_t sh[SHADE_BLOCK / 64 + 1];
+  const uint32_t cur = b & 1u, nxt = cur ^ 1u;
+  const uint32_t n = P.counters[CNT_QUEUE + b];
+  for (uint32_t base = blockIdx.x * SHADE_BLOCK; base < n; base += gridDim.x * SHADE_BLOCK) {
+    const uint32_t i = base + threadIdx.x;
+    bool go = false;
+    uint32_t qp = 0;
+    V3 go_o = v3(0.f, 0.f, 0.f), go_d = go_o;
+    float go_eps = 0.f;
+    if (i < n) {
+      qp = P.qpath[cur][i];
+      const uint32_t p = qp & ~BD_CAMERA_RAY;
+      go = (qp & BD_CAMERA_RAY) ? bd_camera_step(P, A, b, i, p, go_o, go_d, go_eps)
+                                : bd_light_step(P, A, b, i, p, go_o, go_d, go_eps);
+    }
+    const uint32_t q = block_append(&P.counters[CNT_QUEUE + b + 1u], go, sh);
+    if (go) {
+      P.qo[nxt][q] = make_float4(go_o.x, go_o.y, go_o.z, go_eps);
+      P.qd[nxt][q] = make_float4(go_d.x, go_d.y, go_d.z, INFINITY);
+      P.qpath[nxt][q] = qp;
+    }
+  }
+}
+
+// Strategy slots: slot k of the part's record arrays holds the strategy (s, t) that is
+// k-th in sample_path's order (t outer from 1, s inner from 0, s + t <= nv).
+__device__ __forceinline__ void bd_slot_st(uint32_t k, uint32_t nv, uint32_t& s, uint32_t& t) {
+  t = 1u;
+  while (k > nv - t) {  // nv - t + 1 slots for camera vertex count t
+    k -= nv - t + 1u;
+    ++t;
+  }
+  s = k;
+}
+
+__device__ __forceinline__ uint32_t bd_slot_of(uint32_t s, uint32_t t, uint32_t nv) {
+  // slots before t: sum over t' < t of (nv - t' + 1)
+  return (t - 1u) * nv - (t - 1u) * (t - 2u) / 2u + s;
+}
+
+// One strategy (s, t) of one path per thread, a wave's 64 lanes on one strategy of 64
+// consecutive paths, so the wave's loads of a vertex are coalesced and the MIS weight's
+// loop runs the same length in every lane.  Paths j0 .. j0 + m - 1 of the chunk: the
+// record of path j0 + j's slot k at [k * m + j] (w = 1: summed, 2: splatted, 0: none),
+// written for every slot whose s and t lie within the subpaths' vertex counts; a
+// strategy that needs a shadow ray appends it to the shadow queue with its record's
+// index.
+__global__ __launch_bounds__(SHADE_BLOCK, 4) void k_bd_strat(WaveParams P, BdptArgsHost H, uint32_t j0, uint32_t m) {
+  __shared__ uint32_t sh[SHADE_BLOCK / 64 + 1];
+  const DevScene& S = P.S;
+  const uint32_t wps = (m + 63u) / 64u;  // waves per slot
+  const uint32_t w = blockIdx.x * (SHADE_BLOCK / 64) + (threadIdx.x >> 6);
+  const uint32_t k = w / wps, j = (w % wps) * 64u + lane_id();
+  bool go = false;
+  BdStrategy o;
+  size_t rk = 0;
+  if (k < H.ns && j < m) {
+    uint32_t s, t;
+    bd_slot_st(k, H.nv, s, t);
+    const uint32_t i = j0 + j;
+    const uint32_t lw = H.nl[i], nl = lw & 0xFFu, light = lw >> 8, nc = H.nc[i];
+    if (s <= nl && t <= nc) {
+      BdptArgs A;
+      A.lv = H.lv; A.cv = H.cv; A.n = H.n; A.nv = H.nv; A.use_t1 = H.use_t1; A.splat = H.splat;
+      uint32_t pixel, sample;
+      order_sample_pixel(P.ord, P.s0, P.q0 + i, sample, pixel);
+      BdPath X;
+      X.A = &A;
+      X.i = i;
       X.s = s;
       X.t = t;
+      const BdV zc = bd_load(A.cv, A.n, i, t - 1u);
+      const Bsdf bc = bv_kind(zc) == BV_SURFACE ? bd_bsdf(S, zc) : bd_no_bsdf();
       Bsdf bl = bd_no_bsdf();
-      BdStrategy o;
-      if (!bd_strategy(P, A, X, pixel, sample, light, zc, bc, bl, o)) continue;
-      const V3 wc = o.c * bd_strategy_weight(P, X, o.li, bl, bc);
-      const uint32_t k = nrec++;
-      H.rec[base + k] = make_float4(wc.x, wc.y, wc.z, t == 1u ? 2.f : 1.f);
-      if (t == 1u) H.rec_fp[base + k] = o.fp;
-      if (o.ray) {
-        H.stage_o[base + nray] = make_float4(o.shadow.o.x, o.shadow.o.y, o.shadow.o.z, o.shadow.tmin);
-        H.stage_d[base + nray] = make_float4(o.shadow.d.x, o.shadow.d.y, o.shadow.d.z, o.shadow.tmax);
-        H.nray[base + nray] = k;
-        ++nray;
+      float4 r = make_float4(0.f, 0.f, 0.f, 0.f);
+      rk = (size_t)k * m + j;
+      if (bd_strategy(P, A, X, pixel, sample, light, zc, bc, bl, o)) {
+        const V3 wc = o.c * bd_strategy_weight(P, X, o.li, bl, bc);
+        r = make_float4(wc.x, wc.y, wc.z, t == 1u ? 2.f : 1.f);
+        if (t == 1u) H.rec_fp[rk] = o.fp;
+        go = o.ray;
       }
+      H.rec[rk] = r;
     }
   }
-  H.nrec[i] = nrec | (nray << 16);
-}
-
-// The staged shadow rays of every path into one queue (one atomic per workgroup); a ray's
-// slot word is its record's index, which k_connect clears on an occluded ray.
-__global__ __launch_bounds__(SHADE_BLOCK) void k_bd_gather(WaveParams P, BdptArgsHost A) {
-  __shared__ uint32_t sh[SHADE_BLOCK / 64 + 1];
-  const uint32_t i = blockIdx.x * SHADE_BLOCK + threadIdx.x;
-  const uint32_t cnt = i < P.n_chunk ? A.nrec[i] >> 16 : 0u;
-  const uint32_t q = block_append_n(&P.counters[CNT_SHADOW], cnt, sh);
-  const size_t base = (size_t)i * A.ns;
-  for (uint32_t k = 0; k < cnt; ++k) {
-    P.so[q + k] = A.stage_o[base + k];
-    P.sd[q + k] = A.stage_d[base + k];
-    P.smeta[q + k] = make_uint2((uint32_t)base + A.nray[base + k], 0xFFFFFFFFu);
+  const uint32_t q = block_append(&P.counters[CNT_SHADOW], go, sh);
+  if (go) {
+    P.so[q] = make_float4(o.shadow.o.x, o.shadow.o.y, o.shadow.o.z, o.shadow.tmin);
+    P.sd[q] = make_float4(o.shadow.d.x, o.shadow.d.y, o.shadow.d.z, o.shadow.tmax);
+    P.smeta[q] = make_uint2((uint32_t)rk, 0xFFFFFFFFu);
   }
 }
 
-// The visible records of path i: t >= 2 summed in strategy order into P.rad[i] (w: 1, or
-// 0 where dort's film drops the sample), t = 1 splatted.
-__global__ __launch_bounds__(SHADE_BLOCK) void k_bd_resolve(WaveParams P, BdptArgsHost H) {
-  const uint32_t i = blockIdx.x * SHADE_BLOCK + threadIdx.x;
-  if (i >= P.n_chunk) return;
+// The visible records of path j0 + j: t >= 2 summed in strategy order into P.rad (w: 1,
+// or 0 where dort's film drops the sample), t = 1 splatted.
+__global__ __launch_bounds__(SHADE_BLOCK) void k_bd_resolve(WaveParams P, BdptArgsHost H, uint32_t j0, uint32_t m) {
+  const uint32_t j = blockIdx.x * SHADE_BLOCK + threadIdx.x;
+  if (j >= m) return;
+  const uint32_t i = j0 + j;
   BdptArgs A;
   A.lv = H.lv; A.cv = H.cv; A.n = H.n; A.nv = H.nv; A.use_t1 = H.use_t1; A.splat = H.splat;
   uint32_t pixel, sample;
@@ -1041,13 +1085,16 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_bd_resolve(WaveParams P, BdptAr
   float r0[4];
   bd_rng(P, pixel, sample, 0, 0, r0);
   const bool on_edge = r0[0] == 0.f || r0[1] == 0.f;
-  const uint32_t nrec = H.nrec[i] & 0xFFFFu;
-  const size_t base = (size_t)i * H.ns;
+  const uint32_t nl = H.nl[i] & 0xFFu, nc = H.nc[i];
   V3 sum = v3(0.f, 0.f, 0.f);
-  for (uint32_t k = 0; k < nrec; ++k) {
-    const float4 r = H.rec[base + k];
-    if (r.w == 1.f) sum = sum + xyz(r);
-    else if (r.w == 2.f) bd_splat(P, A, H.rec_fp[base + k], xyz(r));
+  for (uint32_t t = 1; t <= nc; ++t) {
+    const uint32_t s_end = std::min(nl, H.nv - t);
+    for (uint32_t s = 0; s <= s_end; ++s) {
+      const size_t rk = (size_t)bd_slot_of(s, t, H.nv) * m + j;
+      const float4 r = H.rec[rk];
+      if (r.w == 1.f) sum = sum + xyz(r);
+      else if (r.w == 2.f) bd_splat(P, A, H.rec_fp[rk], xyz(r));
+    }
   }
   P.rad[i] = make_float4(sum.x, sum.y, sum.z, on_edge ? 0.f : 1.f);
 }
@@ -1094,29 +1141,30 @@ __global__ __launch_bounds__(256) void k_bdpt_film(WaveParams P) {
 }
 
 hipError_t launch_bdpt_wave(const WaveParams& P, const BdptArgsHost& A, const KernelCfg& ce,
-                            const KernelCfg& cc, uint32_t shade_grid, hipStream_t st) {
+                            const KernelCfg& cc, uint32_t shade_grid, uint32_t part, hipStream_t st) {
   const uint32_t blocks = (P.n_chunk + SHADE_BLOCK - 1u) / SHADE_BLOCK;
   const uint32_t grid = std::min(blocks, std::max(1u, shade_grid));
   hipError_t e;
-  // light subpaths, then camera subpaths: the same queues and counters, zeroed between
-  for (int walk = 0; walk < 2; ++walk) {
-    if ((e = launch_zero(P.counters, NUM_COUNTERS, st)) != hipSuccess) return e;
-    if (walk == 0) hipLaunchKernelGGL(k_bd_lstart, dim3(blocks), dim3(SHADE_BLOCK), 0, st, P, A);
-    else hipLaunchKernelGGL(k_bd_cstart, dim3(blocks), dim3(SHADE_BLOCK), 0, st, P, A);
-    for (uint32_t b = 0; b <= P.max_depth; ++b) {
-      if ((e = launch_extend(P, ce, b, false, st)) != hipSuccess) return e;
-      if (walk == 0) hipLaunchKernelGGL(k_bd_lshade, dim3(grid), dim3(SHADE_BLOCK), 0, st, P, A, b);
-      else hipLaunchKernelGGL(k_bd_cshade, dim3(grid), dim3(SHADE_BLOCK), 0, st, P, A, b);
-    }
-    hipLaunchKernelGGL(k_bd_count, dim3(1), dim3(64), 0, st, P, 0u);
-  }
-  // strategies, their shadow rays, the visible contributions
+  // both subpaths: bounce b's light and camera rays in one queue
   if ((e = launch_zero(P.counters, NUM_COUNTERS, st)) != hipSuccess) return e;
-  hipLaunchKernelGGL(k_bd_strat, dim3(blocks), dim3(SHADE_BLOCK), 0, st, P, A);
-  hipLaunchKernelGGL(k_bd_gather, dim3(blocks), dim3(SHADE_BLOCK), 0, st, P, A);
-  if ((e = launch_connect(P, cc, 0, false, st)) != hipSuccess) return e;
-  hipLaunchKernelGGL(k_bd_resolve, dim3(blocks), dim3(SHADE_BLOCK), 0, st, P, A);
-  hipLaunchKernelGGL(k_bd_count, dim3(1), dim3(64), 0, st, P, 1u);
+  hipLaunchKernelGGL(k_bd_start, dim3(blocks), dim3(SHADE_BLOCK), 0, st, P, A);
+  for (uint32_t b = 0; b <= P.max_depth; ++b) {
+    if ((e = launch_extend(P, ce, b, false, st)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_bd_shade, dim3(grid), dim3(SHADE_BLOCK), 0, st, P, A, b);
+  }
+  hipLaunchKernelGGL(k_bd_count, dim3(1), dim3(64), 0, st, P, 0u);
+  // strategies, their shadow rays, the visible contributions: `part` paths at a time
+  for (uint32_t j0 = 0; j0 < P.n_chunk; j0 += part) {
+    const uint32_t m = std::min(part, P.n_chunk - j0);
+    const uint32_t mb = (m + SHADE_BLOCK - 1u) / SHADE_BLOCK;
+    const uint64_t waves = (uint64_t)A.ns * ((m + 63u) / 64u);
+    if ((e = launch_zero(P.counters, NUM_COUNTERS, st)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_bd_strat, dim3((uint32_t)((waves + SHADE_BLOCK / 64 - 1) / (SHADE_BLOCK / 64))),
+                       dim3(SHADE_BLOCK), 0, st, P, A, j0, m);
+    if ((e = launch_connect(P, cc, 0, false, st)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_bd_resolve, dim3(mb), dim3(SHADE_BLOCK), 0, st, P, A, j0, m);
+    hipLaunchKernelGGL(k_bd_count, dim3(1), dim3(64), 0, st, P, 1u);
+  }
   hipLaunchKernelGGL(k_bdpt_film, dim3((P.npix + 255u) / 256u), dim3(256), 0, st, P);
   return hipGetLastError();
 }

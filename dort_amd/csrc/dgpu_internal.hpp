@@ -148,18 +148,16 @@ struct BdptArgsHost {
   uint32_t use_t1;
   float* splat;      // splat sums, npix * 3
   // wavefront form (launch_bdpt_wave): per path
-  float4* w0;        // walk state: throughput factor (light: last f; camera: the pending
-                     // ray's scale) and the forward direction pdf
-  uint4* w1;         // walk state: vertices so far, light index, distant light
+  float4* w0;        // walk state, light walks at [i], camera walks at [n + i]: throughput
+                     // factor (light: last f; camera: the pending ray's scale) and the
+                     // forward direction pdf
+  uint4* w1;         // walk state, likewise: vertices so far, light index, distant light
   uint32_t* nl;      // light subpath vertices | the path's light << 8
   uint32_t* nc;      // camera subpath vertices
-  uint32_t* nrec;    // strategy records | staged shadow rays << 16
-  float4* rec;       // ns records per path: weighted contribution, w = 1 (sum) / 2 (splat) / 0
+  // strategy stage, over a part of m paths of the chunk: slot k of path j at [k * m + j]
+  float4* rec;       // weighted contribution, w = 1 (sum) / 2 (splat) / 0
   float2* rec_fp;    // film position of a splat record
-  float4* stage_o;   // ns staged shadow rays per path: origin + tmin
-  float4* stage_d;   //                                  direction + tmax
-  uint32_t* nray;    // per staged shadow ray: the index of its record
-  uint32_t ns;       // records per path: nv (nv + 1)
+  uint32_t ns;       // strategy slots per path: nv (nv + 1) / 2
 };
 constexpr uint32_t BDPT_VERTEX_BYTES = 64;
 
@@ -179,7 +177,7 @@ hipError_t launch_bdpt(const WaveParams& P, const BdptArgsHost& A, uint32_t widt
 // The wavefront form of the same chunk: the light and camera subpaths walked by extend and
 // a subpath shade per bounce, the strategies' connections traced by connect.
 hipError_t launch_bdpt_wave(const WaveParams& P, const BdptArgsHost& A, const KernelCfg& ce,
-                            const KernelCfg& cc, uint32_t shade_grid, hipStream_t st);
+                            const KernelCfg& cc, uint32_t shade_grid, uint32_t part, hipStream_t st);
 hipError_t launch_extend(const WaveParams& P, const KernelCfg& cfg, uint32_t bounce, bool batch,
                          hipStream_t st);
 // max_blocks: the shade grid (workgroups of 256 that stride over the queue)

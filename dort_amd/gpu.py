@@ -356,9 +356,9 @@ class Scene:
         rgb, _ = self.render_light(opts)
         return rgb / float(opts.sample_end - opts.sample_begin)
 
-    def render_bdpt(self, opts, use_t1_paths=True, films=None, device=False, cancel=None, wavefront=False):
-        """dort's bidirectional path tracer (one thread per path sample, or the wavefront stages
-        with wavefront=True): adds to films (rgb (H,W,3), weight (H,W), splat (H,W,3)); returns
+    def render_bdpt(self, opts, use_t1_paths=True, films=None, device=False, cancel=None, megakernel=False):
+        """dort's bidirectional path tracer (the wavefront stages, or one thread per path sample
+        with megakernel=True): adds to films (rgb (H,W,3), weight (H,W), splat (H,W,3)); returns
         (rgb, weight, splat, stats)."""
         if films is None:
             films = (np.zeros((opts.y_res, opts.x_res, 3), np.float32),
@@ -369,17 +369,17 @@ class Scene:
         o = abi.RenderOpts.from_buffer_copy(opts)
         if device:
             o.flags |= abi.BUF_DEVICE
-        if wavefront:
-            o.flags |= abi.BDPT_WAVEFRONT
+        if megakernel:
+            o.flags |= abi.BDPT_MEGAKERNEL
         st = abi.RenderStats()
         cptr = cancel.ctypes.data if cancel is not None else None
         check(lib().dgpu_render_bdpt(self.h, C.byref(o), 1 if use_t1_paths else 0, ptr(rgb), ptr(w), ptr(splat),
                                      C.byref(st), cptr))
         return rgb, w, splat, abi.stats_dict(st)
 
-    def bdpt_image(self, opts, use_t1_paths=True, wavefront=False):
+    def bdpt_image(self, opts, use_t1_paths=True, megakernel=False):
         """The bidirectional image as dort's film gives it: rgb / weight + splat / iterations."""
-        rgb, w, splat, _ = self.render_bdpt(opts, use_t1_paths, wavefront=wavefront)
+        rgb, w, splat, _ = self.render_bdpt(opts, use_t1_paths, megakernel=megakernel)
         img = np.where(w[..., None] > 0, rgb / np.maximum(w, 1e-30)[..., None], 0.0)
         return (img + splat / float(opts.sample_end - opts.sample_begin)).astype(np.float32)
 

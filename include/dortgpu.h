@@ -167,9 +167,9 @@ enum {
                                allows the path tracer's faster box test */
   DGPU_ADJOINT_MEGAKERNEL = 16u, /* dgpu_adjoint: one thread per path traces and replays it in
                                     one kernel, instead of the two wavefront passes */
-  DGPU_BDPT_WAVEFRONT = 32u      /* dgpu_render_bdpt: the wavefront stages (subpath walks over
-                                    the path tracer's trace kernels, then the strategies and
-                                    their shadow rays) instead of one thread per path sample */
+  DGPU_BDPT_MEGAKERNEL = 32u     /* dgpu_render_bdpt: one thread per path sample walks both
+                                    subpaths and forms every strategy, instead of the
+                                    wavefront stages */
 };
 
 /* ---- rendering ----------------------------------------------------------- */

@@ -145,8 +145,8 @@ def test_bdpt_wavefront_equals_megakernel(gpu_ctx, scene_name, max_depth, t1):
           "env": lambda: scenes.terrain_scene(quads=40, seed=3, sky_res=(128, 64))}[scene_name]()
     s = gpu_ctx.scene(sc)
     o = abi.render_opts(96, 80, spp=6, seed=17, max_depth=max_depth)
-    rw, ww, sw, stw = s.render_bdpt(o, use_t1_paths=t1, wavefront=True)
-    rm, wm, sm, stm = s.render_bdpt(o, use_t1_paths=t1)
+    rw, ww, sw, stw = s.render_bdpt(o, use_t1_paths=t1)
+    rm, wm, sm, stm = s.render_bdpt(o, use_t1_paths=t1, megakernel=True)
     print(scene_name, "extension rays", stw["extension_rays"], stm["extension_rays"],
           "shadow", stw["shadow_rays"], stm["shadow_rays"])
     assert np.array_equal(ww, wm)
@@ -169,8 +169,25 @@ def test_bdpt_wavefront_chunks(gpu_ctx):
     the image."""
     sc = scenes.dort_test_box("area")
     o = abi.render_opts(40, 30, spp=3, seed=8, max_depth=5)
-    ra, wa, sa, _ = gpu_ctx.scene(sc).render_bdpt(o, wavefront=True)
-    rb, wb, sb, _ = gpu_ctx.scene(sc, queue_tile=1000).render_bdpt(o, wavefront=True)
+    ra, wa, sa, _ = gpu_ctx.scene(sc).render_bdpt(o)
+    rb, wb, sb, _ = gpu_ctx.scene(sc, queue_tile=1000).render_bdpt(o)
     assert np.array_equal(wa, wb)
     np.testing.assert_allclose(rb, ra, rtol=1e-6, atol=1e-7)
     np.testing.assert_allclose(sb, sa, rtol=1e-4, atol=1e-6)
+
+
+def test_bdpt_wavefront_strategy_parts(gpu_ctx):
+    """At depth 29 (the largest) a path has 496 strategy slots (49.6 kB of records, staged
+    rays and shadow-queue slots), so the strategy stage of a 524,288-path chunk runs over
+    parts of it (at most 16 GiB of records at a time: 346k paths) while both walks cover
+    the whole chunk; the result is still the one-thread-per-path kernel's."""
+    s = gpu_ctx.scene(scenes.dort_test_box("area"))
+    o = abi.render_opts(256, 256, spp=8, seed=3, max_depth=29)
+    rw, ww, sw, stw = s.render_bdpt(o)
+    rm, wm, sm, stm = s.render_bdpt(o, megakernel=True)
+    assert np.array_equal(ww, wm)
+    assert stw["extension_rays"] == stm["extension_rays"]
+    assert stw["shadow_rays"] == stm["shadow_rays"]
+    close = (np.abs(rw - rm) <= 1e-4 * np.abs(rm) + 1e-6).all(axis=-1).mean()
+    assert close >= 0.99999
+    np.testing.assert_allclose(sw.sum(), sm.sum(), rtol=1e-4)
