@@ -445,7 +445,9 @@ def main():
                 "avg_launch_ms": round(avg_ms, 4), "launches": nl,
                 "algorithmic_bytes_per_launch": int(bytes_per_launch),
                 "kernel_ms": {k: round(v[0], 2) for k, v in kernels.items()},
-                "other_ms": round(ps["other_ms"], 2)}
+                "other_ms": round(ps["other_ms"], 2),
+                "overlap": "each bounce's connect runs beside the next bounce's extend on a second "
+                           "stream; launch times include the time the two kernels share the GPU"}
     log(f"profile: {roofline['kernel_ms']} dominant {dom} avg {avg_ms:.3f} ms, {achieved:.1f} GB/s")
 
     if args.calib:

@@ -56,6 +56,10 @@ struct dgpu_ctx {
   int device = 0;
   int num_cus = 0;
   hipStream_t stream = nullptr;
+  // the render's shadow-ray tests run on a second stream of the same (default) priority,
+  // each bounce's beside the next bounce's extension rays; created on first use
+  hipStream_t stream2 = nullptr;
+  std::vector<hipEvent_t> ev_sync;  // ordering events (no timing), reused across renders
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::mutex mu;
   std::map<uint64_t, uint32_t> grid_cache;
@@ -73,12 +77,28 @@ struct dgpu_ctx {
     return ev_pool[i];
   }
 
+  hipEvent_t sync_event(size_t i) {
+    while (ev_sync.size() <= i) {
+      hipEvent_t e = nullptr;
+      if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return nullptr;
+      ev_sync.push_back(e);
+    }
+    return ev_sync[i];
+  }
+
+  hipStream_t second_stream() {
+    if (!stream2 && hipStreamCreateWithFlags(&stream2, hipStreamNonBlocking) != hipSuccess) stream2 = nullptr;
+    return stream2;
+  }
+
   ~dgpu_ctx() {
     (void)hipSetDevice(device);
     for (auto& kv : bufs) if (kv.second.p) (void)hipFree(kv.second.p);
     for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
+    for (hipEvent_t e : ev_sync) (void)hipEventDestroy(e);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
+    if (stream2) (void)hipStreamDestroy(stream2);
     if (stream) (void)hipStreamDestroy(stream);
   }
 
@@ -827,6 +847,10 @@ dgpu_status dgpu_render(dgpu_scene* s, const dgpu_render_opts* o, float* film_rg
   P.counters = (uint32_t*)ctx->buf("counters", NUM_COUNTERS * 4, &st);
   P.stats = (unsigned long long*)ctx->buf("stats", STATS_WORDS * 8, &st);
   P.spill = (uint32_t*)ctx->buf("spill", (size_t)maxgrid * s->cfg.block * s->stack_cap * 4, &st);
+  // the shadow-ray tests run beside the extension rays: a traversal spill area of their own
+  uint32_t* spill_c = (uint32_t*)ctx->buf("spill_c", (size_t)cc.grid * s->cfg.block * s->stack_cap * 4, &st);
+  hipStream_t stream_c = ctx->second_stream();
+  if (!stream_c) return fail(DGPU_EDEVICE, "stream creation failed");
   P.stack_cap = s->stack_cap;
   bool dev = o->flags & DGPU_BUF_DEVICE;
   if (dev) {
@@ -845,17 +869,25 @@ dgpu_status dgpu_render(dgpu_scene* s, const dgpu_render_opts* o, float* film_rg
   const bool profile = (o->flags & DGPU_RENDER_PROFILE) != 0;
   std::vector<std::pair<int, size_t>> marks;  // (kernel kind, first event of the pair)
   size_t next_ev = 0;
-  auto timed = [&](int kind, auto&& launch) -> hipError_t {
+  auto timed = [&](int kind, hipStream_t on, auto&& launch) -> hipError_t {
     if (!profile) return launch();
     hipEvent_t a = ctx->pool_event(next_ev), b = ctx->pool_event(next_ev + 1);
     if (!a || !b) return hipErrorOutOfMemory;
-    hipError_t e = hipEventRecord(a, stream);
+    hipError_t e = hipEventRecord(a, on);
     if (e == hipSuccess) e = launch();
-    if (e == hipSuccess) e = hipEventRecord(b, stream);
+    if (e == hipSuccess) e = hipEventRecord(b, on);
     marks.push_back({kind, next_ev});
     next_ev += 2;
     return e;
   };
+  // Bounce b's shadow rays (connect, on stream_c) need only bounce b's shade, and the
+  // next bounce's extension rays (extend, on stream) need the same: the two run side by
+  // side, which overlaps the latency-bound tails of the small late bounces.  Shade b + 1
+  // adds the visible NEE of bounce b, so it waits for connect b.
+  hipEvent_t ev_shade = ctx->sync_event(0), ev_conn = ctx->sync_event(1);
+  if (!ev_shade || !ev_conn) return fail(DGPU_EDEVICE, "event creation failed");
+  WaveParams Pc = P;
+  Pc.spill = spill_c;
   HIP_TRY(hipEventRecord(ctx->ev0, stream));
   bool cancelled = false;
   for (uint64_t g0 = 0; g0 < total && !cancelled; g0 += C) {
@@ -863,14 +895,22 @@ dgpu_status dgpu_render(dgpu_scene* s, const dgpu_render_opts* o, float* film_rg
     P.n_chunk = (uint32_t)std::min<uint64_t>(C, total - g0);
     P.q0 = (uint32_t)(g0 % npix);
     P.s0 = o->sample_begin + (uint32_t)(g0 / npix);
-    HIP_TRY(timed(3, [&] { return launch_raygen(P, P.n_chunk, false, stream); }));
+    Pc = P;
+    Pc.spill = spill_c;
+    HIP_TRY(timed(3, stream, [&] { return launch_raygen(P, P.n_chunk, false, stream); }));
     for (uint32_t b = 0; b <= o->max_depth; ++b) {
       if (cancel && *cancel) { cancelled = true; break; }
-      HIP_TRY(timed(0, [&] { return launch_extend(P, ce, b, false, stream); }));
-      HIP_TRY(timed(1, [&] { return launch_shade(P, P.n_chunk, b, false, stream, shade_grid); }));
-      HIP_TRY(timed(2, [&] { return launch_connect(P, cc, b, false, stream); }));
+      HIP_TRY(timed(0, stream, [&] { return launch_extend(P, ce, b, false, stream); }));
+      if (b > 0) HIP_TRY(hipStreamWaitEvent(stream, ev_conn, 0));
+      HIP_TRY(timed(1, stream, [&] { return launch_shade(P, P.n_chunk, b, false, stream, shade_grid); }));
+      HIP_TRY(hipEventRecord(ev_shade, stream));
+      HIP_TRY(hipStreamWaitEvent(stream_c, ev_shade, 0));
+      HIP_TRY(timed(2, stream_c, [&] { return launch_connect(Pc, cc, b, false, stream_c); }));
+      HIP_TRY(hipEventRecord(ev_conn, stream_c));
     }
-    if (!cancelled) HIP_TRY(timed(3, [&] { return launch_finalize(P, P.n_chunk, false, stream); }));
+    // the last connect (or, after a cancel, the last one launched) before what follows
+    HIP_TRY(hipStreamWaitEvent(stream, ev_conn, 0));
+    if (!cancelled) HIP_TRY(timed(3, stream, [&] { return launch_finalize(P, P.n_chunk, false, stream); }));
   }
   HIP_TRY(hipEventRecord(ctx->ev1, stream));
   unsigned long long hstats[STATS_WORDS] = {};
@@ -1585,6 +1625,11 @@ dgpu_status dgpu_render_batch(dgpu_ctx* ctx, const dgpu_batch_item* items, uint3
   P.stats = (unsigned long long*)ctx->buf("stats", STATS_WORDS * 8, &st);
   P.spill = (uint32_t*)ctx->buf("spill", spill_threads * stack_cap * 4, &st);
   P.stack_cap = stack_cap;
+  // shadow-ray tests beside the next bounce's extension rays, as in dgpu_render
+  WaveParams Pc;
+  uint32_t* spill_c = (uint32_t*)ctx->buf("spill_c", spill_threads * stack_cap * 4, &st);
+  hipStream_t stream_c = ctx->second_stream();
+  if (!stream_c) return fail(DGPU_EDEVICE, "stream creation failed");
   BatchEntry* dtab = (BatchEntry*)ctx->buf("batch_tab", n * sizeof(BatchEntry), &st);
   ChunkDesc* dchunks = (ChunkDesc*)ctx->buf("batch_chunks", chunks.size() * sizeof(ChunkDesc), &st);
   uint32_t* dcur = (uint32_t*)ctx->buf("batch_cur", 64, &st);
@@ -1593,6 +1638,8 @@ dgpu_status dgpu_render_batch(dgpu_ctx* ctx, const dgpu_batch_item* items, uint3
   P.num_scenes = n;
   P.chunks = dchunks;
   P.chunk_cur = dcur;
+  Pc = P;
+  Pc.spill = spill_c;
   HIP_TRY(hipMemcpyAsync(dtab, tab.data(), n * sizeof(BatchEntry), hipMemcpyHostToDevice, stream));
   HIP_TRY(hipMemcpyAsync(dchunks, chunks.data(), chunks.size() * sizeof(ChunkDesc), hipMemcpyHostToDevice, stream));
   HIP_TRY(hipMemsetAsync(dcur, 0, 4, stream));
@@ -1601,16 +1648,29 @@ dgpu_status dgpu_render_batch(dgpu_ctx* ctx, const dgpu_batch_item* items, uint3
   HIP_TRY(hipStreamSynchronize(stream));
 
   // chunk c: raygen -> (extend -> shade -> connect) x (max_depth + 1) -> finalize -> advance,
-  // with its group's trace kernels; the kernels read the chunk index from the device
+  // with its group's trace kernels; the kernels read the chunk index from the device.
+  // Each bounce's connect runs on stream_c beside the next bounce's extend (the next
+  // shade waits for it); every ordering point has an event of its own, so that a graph
+  // captured from the two streams holds each dependency as recorded.
+  size_t next_sync = 0;
   auto chunk_sequence = [&](size_t c) -> hipError_t {
     const KernelCfg& ce = group_ce[chunk_group[c]];
     const KernelCfg& cc = group_cc[chunk_group[c]];
+    hipEvent_t ev_conn = nullptr;
     hipError_t e = launch_raygen(P, cap, true, stream);
     for (uint32_t b = 0; e == hipSuccess && b <= o0.max_depth; ++b) {
       e = launch_extend(P, ce, b, true, stream);
+      if (e == hipSuccess && ev_conn) e = hipStreamWaitEvent(stream, ev_conn, 0);
       if (e == hipSuccess) e = launch_shade(P, cap, b, true, stream, shade_grid);
-      if (e == hipSuccess) e = launch_connect(P, cc, b, true, stream);
+      hipEvent_t ev_shade = ctx->sync_event(next_sync++);
+      ev_conn = ctx->sync_event(next_sync++);
+      if (!ev_shade || !ev_conn) return hipErrorOutOfMemory;
+      if (e == hipSuccess) e = hipEventRecord(ev_shade, stream);
+      if (e == hipSuccess) e = hipStreamWaitEvent(stream_c, ev_shade, 0);
+      if (e == hipSuccess) e = launch_connect(Pc, cc, b, true, stream_c);
+      if (e == hipSuccess) e = hipEventRecord(ev_conn, stream_c);
     }
+    if (e == hipSuccess && ev_conn) e = hipStreamWaitEvent(stream, ev_conn, 0);
     if (e == hipSuccess) e = launch_finalize(P, cap, true, stream);
     if (e == hipSuccess) e = launch_advance(dcur, stream);
     return e;
