@@ -1007,6 +1007,9 @@ dgpu_status dgpu_render_light(dgpu_scene* s, const dgpu_render_opts* o, float* f
   P.counters = (uint32_t*)ctx->buf("counters", NUM_COUNTERS * 4, &st);
   P.stats = (unsigned long long*)ctx->buf("stats", STATS_WORDS * 8, &st);
   P.spill = (uint32_t*)ctx->buf("spill", (size_t)maxgrid * s->cfg.block * s->stack_cap * 4, &st);
+  uint32_t* spill_c = (uint32_t*)ctx->buf("spill_c", (size_t)cc.grid * s->cfg.block * s->stack_cap * 4, &st);
+  hipStream_t stream_c = ctx->second_stream();
+  if (!stream_c) return fail(DGPU_EDEVICE, "stream creation failed");
   P.stack_cap = s->stack_cap;
   bool dev = o->flags & DGPU_BUF_DEVICE;
   P.film_rgb = dev ? film_splat_rgb : (float*)ctx->buf("film_rgb", (size_t)npix * 12, &st);
@@ -1016,33 +1019,50 @@ dgpu_status dgpu_render_light(dgpu_scene* s, const dgpu_render_opts* o, float* f
   const bool profile = (o->flags & DGPU_RENDER_PROFILE) != 0;
   std::vector<std::pair<int, size_t>> marks;
   size_t next_ev = 0;
-  auto timed = [&](int kind, auto&& launch) -> hipError_t {
+  auto timed = [&](int kind, hipStream_t on, auto&& launch) -> hipError_t {
     if (!profile) return launch();
     hipEvent_t a = ctx->pool_event(next_ev), b = ctx->pool_event(next_ev + 1);
     if (!a || !b) return hipErrorOutOfMemory;
-    hipError_t e = hipEventRecord(a, stream);
+    hipError_t e = hipEventRecord(a, on);
     if (e == hipSuccess) e = launch();
-    if (e == hipSuccess) e = hipEventRecord(b, stream);
+    if (e == hipSuccess) e = hipEventRecord(b, on);
     marks.push_back({kind, next_ev});
     next_ev += 2;
     return e;
   };
+  // A stage's camera connections (connect, then splat, on stream_c) need only the stage's
+  // shade, as does the next stage's extension (extend, on stream): the two run side by
+  // side.  The next shade reuses the shadow queue and its pending records, so it waits
+  // for the splat.
+  hipEvent_t ev_shade = ctx->sync_event(0), ev_splat = ctx->sync_event(1);
+  if (!ev_shade || !ev_splat) return fail(DGPU_EDEVICE, "event creation failed");
+  WaveParams Pc = P;
   HIP_TRY(hipEventRecord(ctx->ev0, stream));
   bool cancelled = false;
+  auto connect_stage = [&](uint32_t stage) -> hipError_t {
+    hipError_t e = hipEventRecord(ev_shade, stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(stream_c, ev_shade, 0);
+    if (e == hipSuccess) e = timed(2, stream_c, [&] { return launch_connect(Pc, cc, stage, false, stream_c); });
+    if (e == hipSuccess) e = timed(3, stream_c, [&] { return launch_lt_splat(Pc, stage, stream_c, grid); });
+    if (e == hipSuccess) e = hipEventRecord(ev_splat, stream_c);
+    return e;
+  };
   for (uint64_t g0 = 0; g0 < total && !cancelled; g0 += C) {
     P.g0 = g0;
     P.n_chunk = (uint32_t)std::min<uint64_t>(C, total - g0);
+    Pc = P;
+    Pc.spill = spill_c;
     HIP_TRY(launch_zero(P.counters, NUM_COUNTERS, stream));
-    HIP_TRY(timed(1, [&] { return launch_lt_start(P, stream, grid); }));
-    HIP_TRY(timed(2, [&] { return launch_connect(P, cc, 0, false, stream); }));
-    HIP_TRY(timed(3, [&] { return launch_lt_splat(P, 0, stream, grid); }));
+    HIP_TRY(timed(1, stream, [&] { return launch_lt_start(P, stream, grid); }));
+    HIP_TRY(connect_stage(0));
     for (uint32_t b = 0; b + 3u <= P.max_len; ++b) {
       if (cancel && *cancel) { cancelled = true; break; }
-      HIP_TRY(timed(0, [&] { return launch_extend(P, ce, b, false, stream); }));
-      HIP_TRY(timed(1, [&] { return launch_lt_shade(P, b, stream, grid); }));
-      HIP_TRY(timed(2, [&] { return launch_connect(P, cc, b + 1, false, stream); }));
-      HIP_TRY(timed(3, [&] { return launch_lt_splat(P, b + 1, stream, grid); }));
+      HIP_TRY(timed(0, stream, [&] { return launch_extend(P, ce, b, false, stream); }));
+      HIP_TRY(hipStreamWaitEvent(stream, ev_splat, 0));
+      HIP_TRY(timed(1, stream, [&] { return launch_lt_shade(P, b, stream, grid); }));
+      HIP_TRY(connect_stage(b + 1));
     }
+    HIP_TRY(hipStreamWaitEvent(stream, ev_splat, 0));
     HIP_TRY(launch_lt_stats(P, stream));
   }
   HIP_TRY(hipEventRecord(ctx->ev1, stream));
@@ -1379,6 +1399,10 @@ dgpu_status dgpu_adjoint(dgpu_scene* s, const dgpu_render_opts* o, const float* 
     P.adj_L = (float4*)ctx->buf("adj_L", (size_t)C * 16, &st);
     P.adj_pend = (float4*)ctx->buf("adj_pend", (size_t)C * slots * 16, &st);
     P.adj_cont = (float4*)ctx->buf("adj_cont", (size_t)C * 16, &st);
+    uint32_t* spill_c = (uint32_t*)ctx->buf("spill_c", (size_t)cc.grid * s->cfg.block * s->stack_cap * 4, &st);
+    hipStream_t stream_c = ctx->second_stream();
+    hipEvent_t ev_shade = ctx->sync_event(0), ev_conn = ctx->sync_event(1);
+    if (!stream_c || !ev_shade || !ev_conn) return fail(DGPU_EDEVICE, "stream/event creation failed");
     if (st != DGPU_OK) return st;
     P.adj_dldi = a_dldi;
     P.adj_g_tex = a_tex;
@@ -1392,14 +1416,23 @@ dgpu_status dgpu_adjoint(dgpu_scene* s, const dgpu_render_opts* o, const float* 
       P.n_chunk = (uint32_t)std::min<uint64_t>(C, total - g0);
       P.q0 = (uint32_t)(g0 % npix);
       P.s0 = o->sample_begin + (uint32_t)(g0 / npix);
+      WaveParams Pc = P;
+      Pc.spill = spill_c;
       for (int pass = 0; pass < 2; ++pass) {
+        // as in dgpu_render: bounce b's connect (stream_c) beside bounce b + 1's extend;
+        // shade b + 1 takes bounce b's visible NEE, so it waits for that connect
         HIP_TRY(launch_raygen(P, P.n_chunk, false, stream));
         for (uint32_t b = 0; b <= o->max_depth; ++b) {
           HIP_TRY(launch_extend(P, ce, b, false, stream));
+          if (b > 0) HIP_TRY(hipStreamWaitEvent(stream, ev_conn, 0));
           if (pass == 0) HIP_TRY(launch_shade(P, P.n_chunk, b, false, stream, shade_grid));
           else HIP_TRY(launch_shade_adjoint(P, P.n_chunk, b, stream, shade_grid_adj));
-          HIP_TRY(launch_connect(P, cc, b, false, stream));
+          HIP_TRY(hipEventRecord(ev_shade, stream));
+          HIP_TRY(hipStreamWaitEvent(stream_c, ev_shade, 0));
+          HIP_TRY(launch_connect(Pc, cc, b, false, stream_c));
+          HIP_TRY(hipEventRecord(ev_conn, stream_c));
         }
+        HIP_TRY(hipStreamWaitEvent(stream, ev_conn, 0));
         if (pass == 0) HIP_TRY(launch_adj_total(P, stream));
         else HIP_TRY(launch_adj_final(P, stream));
       }
