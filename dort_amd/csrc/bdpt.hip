@@ -713,8 +713,9 @@ __global__ __launch_bounds__(BD_BLOCK) void k_bdpt(WaveParams P, BdptArgs A) {
 //   k_bd_start, then per bounce k_extend + k_bd_shade: both subpaths of every path, the
 //     light and the camera rays of one bounce traced by one launch;
 //   then per part of the chunk (the strategy records need far more memory per path than
-//   the subpaths): k_bd_strat: one thread per strategy slot and path, its contribution
-//     and MIS weight (bd_strategy), recorded in the slot, its shadow ray queued;
+//   the subpaths): k_bd_strat: one thread per path, the workgroup in step over the
+//     strategy slots, each contribution and MIS weight (bd_strategy) recorded in its
+//     slot, its shadow ray queued;
 //     k_connect: their visibility (an occluded ray clears its record); k_bd_resolve: the
 //     visible records, summed in strategy order (t >= 2) or splatted.
 // Each stage draws the random numbers k_bdpt draws at the same point of the walk, so both
@@ -1004,71 +1005,75 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_bd_shade(WaveParams P, BdptArgs
 }
 
 // Strategy slots: slot k of the part's record arrays holds the strategy (s, t) that is
-// k-th in sample_path's order (t outer from 1, s inner from 0, s + t <= nv).
-__device__ __forceinline__ void bd_slot_st(uint32_t k, uint32_t nv, uint32_t& s, uint32_t& t) {
-  t = 1u;
-  while (k > nv - t) {  // nv - t + 1 slots for camera vertex count t
-    k -= nv - t + 1u;
-    ++t;
-  }
-  s = k;
-}
-
+// k-th in sample_path's order (t outer from 1, s inner from 0, s + t <= nv); the slots
+// before t number sum over t' < t of (nv - t' + 1).
 __device__ __forceinline__ uint32_t bd_slot_of(uint32_t s, uint32_t t, uint32_t nv) {
-  // slots before t: sum over t' < t of (nv - t' + 1)
   return (t - 1u) * nv - (t - 1u) * (t - 2u) / 2u + s;
 }
 
-// One strategy (s, t) of one path per thread, a wave's 64 lanes on one strategy of 64
-// consecutive paths, so the wave's loads of a vertex are coalesced and the MIS weight's
-// loop runs the same length in every lane.  Paths j0 .. j0 + m - 1 of the chunk: the
-// record of path j0 + j's slot k at [k * m + j] (w = 1: summed, 2: splatted, 0: none),
-// written for every slot whose s and t lie within the subpaths' vertex counts; a
-// strategy that needs a shadow ray appends it to the shadow queue with its record's
-// index.
-__global__ __launch_bounds__(SHADE_BLOCK, 4) void k_bd_strat(WaveParams P, BdptArgsHost H, uint32_t j0, uint32_t m) {
+// Every strategy of the workgroup's 256 paths, one path per thread: the workgroup walks
+// the strategy slots in order (t outer, s inner) up to its paths' largest vertex counts,
+// a thread taking part where its own subpaths reach (s, t), so the lanes of a wave work
+// on one (s, t) at a time (coalesced vertex loads, one loop length in the MIS weight).
+// Paths j0 .. j0 + m - 1 of the chunk: the record of path j0 + j's slot k at [k * m + j]
+// (w = 1: summed, 2: splatted, 0: none), written for every slot whose s and t lie within
+// the subpaths' vertex counts; a strategy that needs a shadow ray appends it to the
+// shadow queue with its record's index (one atomic per workgroup and slot).
+__global__ __launch_bounds__(SHADE_BLOCK, 3) void k_bd_strat(WaveParams P, BdptArgsHost H, uint32_t j0, uint32_t m) {
   __shared__ uint32_t sh[SHADE_BLOCK / 64 + 1];
+  __shared__ uint32_t smax[2];
   const DevScene& S = P.S;
-  const uint32_t wps = (m + 63u) / 64u;  // waves per slot
-  const uint32_t w = blockIdx.x * (SHADE_BLOCK / 64) + (threadIdx.x >> 6);
-  const uint32_t k = w / wps, j = (w % wps) * 64u + lane_id();
-  bool go = false;
-  BdStrategy o;
-  size_t rk = 0;
-  if (k < H.ns && j < m) {
-    uint32_t s, t;
-    bd_slot_st(k, H.nv, s, t);
-    const uint32_t i = j0 + j;
-    const uint32_t lw = H.nl[i], nl = lw & 0xFFu, light = lw >> 8, nc = H.nc[i];
-    if (s <= nl && t <= nc) {
-      BdptArgs A;
-      A.lv = H.lv; A.cv = H.cv; A.n = H.n; A.nv = H.nv; A.use_t1 = H.use_t1; A.splat = H.splat;
-      uint32_t pixel, sample;
-      order_sample_pixel(P.ord, P.s0, P.q0 + i, sample, pixel);
-      BdPath X;
-      X.A = &A;
-      X.i = i;
-      X.s = s;
-      X.t = t;
-      const BdV zc = bd_load(A.cv, A.n, i, t - 1u);
-      const Bsdf bc = bv_kind(zc) == BV_SURFACE ? bd_bsdf(S, zc) : bd_no_bsdf();
-      Bsdf bl = bd_no_bsdf();
-      float4 r = make_float4(0.f, 0.f, 0.f, 0.f);
-      rk = (size_t)k * m + j;
-      if (bd_strategy(P, A, X, pixel, sample, light, zc, bc, bl, o)) {
-        const V3 wc = o.c * bd_strategy_weight(P, X, o.li, bl, bc);
-        r = make_float4(wc.x, wc.y, wc.z, t == 1u ? 2.f : 1.f);
-        if (t == 1u) H.rec_fp[rk] = o.fp;
-        go = o.ray;
-      }
-      H.rec[rk] = r;
-    }
+  const uint32_t j = blockIdx.x * SHADE_BLOCK + threadIdx.x;
+  const bool in = j < m;
+  const uint32_t i = j0 + j;
+  uint32_t nl = 0, nc = 0, light = 0, pixel = 0, sample = 0;
+  if (threadIdx.x == 0) { smax[0] = 0u; smax[1] = 0u; }
+  __syncthreads();
+  if (in) {
+    const uint32_t lw = H.nl[i];
+    nl = lw & 0xFFu;
+    light = lw >> 8;
+    nc = H.nc[i];
+    order_sample_pixel(P.ord, P.s0, P.q0 + i, sample, pixel);
+    atomicMax(&smax[0], nl);
+    atomicMax(&smax[1], nc);
   }
-  const uint32_t q = block_append(&P.counters[CNT_SHADOW], go, sh);
-  if (go) {
-    P.so[q] = make_float4(o.shadow.o.x, o.shadow.o.y, o.shadow.o.z, o.shadow.tmin);
-    P.sd[q] = make_float4(o.shadow.d.x, o.shadow.d.y, o.shadow.d.z, o.shadow.tmax);
-    P.smeta[q] = make_uint2((uint32_t)rk, 0xFFFFFFFFu);
+  __syncthreads();
+  const uint32_t mnl = smax[0], mnc = smax[1];
+  BdptArgs A;
+  A.lv = H.lv; A.cv = H.cv; A.n = H.n; A.nv = H.nv; A.use_t1 = H.use_t1; A.splat = H.splat;
+  for (uint32_t t = 1; t <= mnc; ++t) {
+    const uint32_t s_end = std::min(mnl, H.nv - t);
+    for (uint32_t s = 0; s <= s_end; ++s) {
+      bool go = false;
+      BdStrategy o;
+      size_t rk = 0;
+      if (in && s <= nl && t <= nc) {
+        BdPath X;
+        X.A = &A;
+        X.i = i;
+        X.s = s;
+        X.t = t;
+        const BdV zc = bd_load(A.cv, A.n, i, t - 1u);
+        const Bsdf bc = bv_kind(zc) == BV_SURFACE ? bd_bsdf(S, zc) : bd_no_bsdf();
+        Bsdf bl = bd_no_bsdf();
+        float4 r = make_float4(0.f, 0.f, 0.f, 0.f);
+        rk = (size_t)bd_slot_of(s, t, H.nv) * m + j;
+        if (bd_strategy(P, A, X, pixel, sample, light, zc, bc, bl, o)) {
+          const V3 wc = o.c * bd_strategy_weight(P, X, o.li, bl, bc);
+          r = make_float4(wc.x, wc.y, wc.z, t == 1u ? 2.f : 1.f);
+          if (t == 1u) H.rec_fp[rk] = o.fp;
+          go = o.ray;
+        }
+        H.rec[rk] = r;
+      }
+      const uint32_t q = block_append(&P.counters[CNT_SHADOW], go, sh);
+      if (go) {
+        P.so[q] = make_float4(o.shadow.o.x, o.shadow.o.y, o.shadow.o.z, o.shadow.tmin);
+        P.sd[q] = make_float4(o.shadow.d.x, o.shadow.d.y, o.shadow.d.z, o.shadow.tmax);
+        P.smeta[q] = make_uint2((uint32_t)rk, 0xFFFFFFFFu);
+      }
+    }
   }
 }
 
@@ -1157,10 +1162,8 @@ hipError_t launch_bdpt_wave(const WaveParams& P, const BdptArgsHost& A, const Ke
   for (uint32_t j0 = 0; j0 < P.n_chunk; j0 += part) {
     const uint32_t m = std::min(part, P.n_chunk - j0);
     const uint32_t mb = (m + SHADE_BLOCK - 1u) / SHADE_BLOCK;
-    const uint64_t waves = (uint64_t)A.ns * ((m + 63u) / 64u);
     if ((e = launch_zero(P.counters, NUM_COUNTERS, st)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_bd_strat, dim3((uint32_t)((waves + SHADE_BLOCK / 64 - 1) / (SHADE_BLOCK / 64))),
-                       dim3(SHADE_BLOCK), 0, st, P, A, j0, m);
+    hipLaunchKernelGGL(k_bd_strat, dim3(mb), dim3(SHADE_BLOCK), 0, st, P, A, j0, m);
     if ((e = launch_connect(P, cc, 0, false, st)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_bd_resolve, dim3(mb), dim3(SHADE_BLOCK), 0, st, P, A, j0, m);
     hipLaunchKernelGGL(k_bd_count, dim3(1), dim3(64), 0, st, P, 1u);
