@@ -142,9 +142,21 @@ __device__ __forceinline__ void trace_loop(IO& io, uint32_t n, uint32_t* fetch, 
   st.spill = spill + (size_t)blockIdx.x * BLOCK + threadIdx.x;
   st.spill_stride = gthreads;
   st.cap = stack_cap;
+  // Atomics on the fetch counter (one address for the chip) serialise in L2 at ~90M/s:
+  // the last one of every wave, which only finds the queue drained, cost a launch ~90 us
+  // at its end.  The first wave of a workgroup to find the queue drained says so in LDS,
+  // and the workgroup's other waves leave without their atomic; an empty queue costs
+  // none.
+  __shared__ uint32_t drained;
+  if (threadIdx.x == 0) drained = 0u;
+  __syncthreads();
+  if (n == 0u) return;
   for (;;) {
-    uint32_t base = 0;
-    if (lane_id() == 0) base = atomicAdd(fetch, 64u * NBATCH);
+    uint32_t base = n;
+    if (lane_id() == 0 && *(volatile uint32_t*)&drained == 0u) {
+      base = atomicAdd(fetch, 64u * NBATCH);
+      if (base >= n) *(volatile uint32_t*)&drained = 1u;
+    }
     base = __shfl(base, 0);
     if (base >= n) break;
     if (refill != 0u) trace_batch_refill<ANY, WIDTH, FAST, NBATCH>(io, n, base, st, refill);
