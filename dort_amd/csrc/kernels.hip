@@ -626,16 +626,17 @@ __device__ __forceinline__ void shade_body(const WaveParams& P, const DevScene& 
     }
     bounces = flags & F_BOUNCES;
     bool last_delta = (flags & F_DELTA) != 0u;
-    float4 o4 = P.qo[cur][i], d4 = P.qd[cur][i], h4 = P.hit[i];
-    V3 ro = xyz(o4), rd = xyz(d4);
+    float4 d4 = P.qd[cur][i], h4 = P.hit[i];
+    V3 rd = xyz(d4);
     uint32_t leaf = __float_as_uint(h4.w);
     bool is = leaf != 0xFFFFFFFFu;
     if (is) g = surface_at(S, leaf, h4.x, h4.y, h4.z);
     if ((bounces == 0u || last_delta) && bounces >= P.min_depth && bounces <= P.max_depth) {
       if (is) {
         int32_t li = g.light;
-        // Intersection::eval_radiance -> DiffuseLight::eval_radiance (diffuse_light.cpp:83-93)
-        if (li >= 0 && dot(g.nn, ro - g.p) >= 0.f) rad = rad + thr * xyz(S.lights[li].radiance);
+        // Intersection::eval_radiance -> DiffuseLight::eval_radiance (diffuse_light.cpp:83-93);
+        // the ray's origin is read only for a hit on an emitter
+        if (li >= 0 && dot(g.nn, xyz(P.qo[cur][i]) - g.p) >= 0.f) rad = rad + thr * xyz(S.lights[li].radiance);
       } else {
         rad = rad + thr * background_radiance(S, rd);
       }
