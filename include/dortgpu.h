@@ -307,9 +307,15 @@ dgpu_status dgpu_render_light(dgpu_scene* scene, const dgpu_render_opts* opts, f
  * dort's image is film_rgb_sum / film_weight + film_splat_rgb / (total iterations).  The
  * strategy s = t = 1 has weight 0 for area and background lights and is not formed.
  * Uses x_res, y_res, sample_begin/sample_end, seed, scene_id, flags (DGPU_BUF_DEVICE:
- * all three films are device memory), min_depth and max_depth (< 30); the other options
- * are the path tracer's.  A scene without lights adds nothing (bdpt_renderer.cpp:11).
- * Splats are float atomics, so their sums vary in the last bits from run to run. */
+ * all three films are device memory; DGPU_BDPT_MEGAKERNEL: one thread per path sample
+ * instead of the wavefront stages, the same subpaths and strategies), min_depth and
+ * max_depth (< 30); the other options are the path tracer's.  A scene without lights
+ * adds nothing (bdpt_renderer.cpp:11).  The wavefront stages keep per path of a chunk
+ * 2 (max_depth + 2) 64 B of subpath vertices and 264 B of walk state, and per path of the
+ * part of the chunk whose strategies are formed at once 64 B per strategy slot
+ * ((max_depth + 2) (max_depth + 3) / 2 slots); both are sized from free device memory,
+ * at most 32 GiB together.  Splats are float atomics, so their sums vary in the last
+ * bits from run to run. */
 dgpu_status dgpu_render_bdpt(dgpu_scene* scene, const dgpu_render_opts* opts, uint32_t use_t1_paths,
                              float* film_rgb_sum, float* film_weight, float* film_splat_rgb,
                              dgpu_stats* out_stats, const volatile int* cancel);
