@@ -418,7 +418,7 @@ def main():
     ms_per_step = total / args.steps * 1e3
     value = paths * args.steps / total / 1e6
 
-    # live per-kernel timing (HIP events around every launch, on the library's stream)
+    # live per-kernel timing (HIP events around every launch, on the stream it runs on)
     popts = abi.RenderOpts.from_buffer_copy(opts)
     popts.flags |= abi.RENDER_PROFILE
     _, _, ps = gsc.render(popts, film, wfilm, device=True)
