@@ -125,7 +125,8 @@ def test_bdpt_agrees_with_pt(gpu_ctx, name):
 
 
 @pytest.mark.parametrize("scene_name,max_depth,t1", [("area", 5, True), ("infi", 4, True), ("glass", 6, True),
-                                                     ("env", 6, True), ("area", 3, False)])
+                                                     ("env", 6, True), ("area", 3, False), ("textured", 5, True),
+                                                     ("rough_glass", 5, True)])
 def test_bdpt_wavefront_equals_megakernel(gpu_ctx, scene_name, max_depth, t1):
     """The wavefront stages walk the same subpaths and form the same strategies as the
     one-thread-per-path kernel (the same random numbers at the same points of the walks),
@@ -135,14 +136,16 @@ def test_bdpt_wavefront_equals_megakernel(gpu_ctx, scene_name, max_depth, t1):
     The two forms trace with different kernels, and the compiler fuses multiplies and adds
     (fp-contract) differently in each.  In the area-lit, open and terrain scenes those
     last-bit differences change no decision, so the ray counts agree exactly.  On the
-    glass Cornell box (a dielectric block, whose refractions carry such differences on)
+    glass Cornell boxes (dielectric blocks, whose refractions carry such differences on)
     they change a few: a handful of paths walk differently (3 extension rays in 254,165
     at depth 3), so there the counts agree to 1e-4 and a few pixels differ.  A build with
     -ffp-contract=off makes the glass case agree exactly too (ray counts, pixels and
     splats)."""
     sc = {"area": lambda: scenes.dort_test_box("area"), "infi": lambda: scenes.dort_test_box("infi", open_box=True),
           "glass": lambda: scenes.glass_cornell(),
-          "env": lambda: scenes.terrain_scene(quads=40, seed=3, sky_res=(128, 64))}[scene_name]()
+          "env": lambda: scenes.terrain_scene(quads=40, seed=3, sky_res=(128, 64)),
+          "textured": lambda: scenes.textured_cornell(texels="f32"),
+          "rough_glass": lambda: scenes.rough_glass_cornell(0.2)}[scene_name]()
     s = gpu_ctx.scene(sc)
     o = abi.render_opts(96, 80, spp=6, seed=17, max_depth=max_depth)
     rw, ww, sw, stw = s.render_bdpt(o, use_t1_paths=t1)
@@ -150,7 +153,7 @@ def test_bdpt_wavefront_equals_megakernel(gpu_ctx, scene_name, max_depth, t1):
     print(scene_name, "extension rays", stw["extension_rays"], stm["extension_rays"],
           "shadow", stw["shadow_rays"], stm["shadow_rays"])
     assert np.array_equal(ww, wm)
-    exact = scene_name != "glass"
+    exact = scene_name not in ("glass", "rough_glass")
     for key in ("extension_rays", "shadow_rays"):
         if exact:
             assert stw[key] == stm[key], key
@@ -190,4 +193,19 @@ def test_bdpt_wavefront_strategy_parts(gpu_ctx):
     assert stw["shadow_rays"] == stm["shadow_rays"]
     close = (np.abs(rw - rm) <= 1e-4 * np.abs(rm) + 1e-6).all(axis=-1).mean()
     assert close >= 0.99999
+    np.testing.assert_allclose(sw.sum(), sm.sum(), rtol=1e-4)
+
+
+def test_bdpt_wavefront_min_depth_and_sample_range(gpu_ctx):
+    """min_depth and a sample range that starts past 0 (one chunk of the film's samples):
+    the wavefront stages still give the one-thread-per-path kernel's result."""
+    s = gpu_ctx.scene(scenes.dort_test_box("area"))
+    o = abi.render_opts(64, 48, spp=4, seed=9, min_depth=2, max_depth=6)
+    o.sample_begin, o.sample_end = 3, 7
+    rw, ww, sw, stw = s.render_bdpt(o)
+    rm, wm, sm, stm = s.render_bdpt(o, megakernel=True)
+    assert stw["paths"] == 64 * 48 * 4
+    assert np.array_equal(ww, wm)
+    assert stw["extension_rays"] == stm["extension_rays"] and stw["shadow_rays"] == stm["shadow_rays"]
+    assert (np.abs(rw - rm) <= 1e-4 * np.abs(rm) + 1e-6).all(axis=-1).mean() >= 0.99999
     np.testing.assert_allclose(sw.sum(), sm.sum(), rtol=1e-4)
