@@ -713,9 +713,9 @@ __global__ __launch_bounds__(BD_BLOCK) void k_bdpt(WaveParams P, BdptArgs A) {
 //   k_bd_start, then per bounce k_extend + k_bd_shade: both subpaths of every path, the
 //     light and the camera rays of one bounce traced by one launch;
 //   then per part of the chunk (the strategy records need far more memory per path than
-//   the subpaths): k_bd_strat: one thread per path, the workgroup in step over the
-//     strategy slots, each contribution and MIS weight (bd_strategy) recorded in its
-//     slot, its shadow ray queued;
+//   the subpaths): k_bd_strat<kind> for the four kinds of strategy: one thread per
+//     path, the workgroup in step over the kind's slots, each contribution and MIS
+//     weight (bd_strategy) recorded in its slot, its shadow ray queued;
 //     k_connect: their visibility (an occluded ray clears its record); k_bd_resolve: the
 //     visible records, summed in strategy order (t >= 2) or splatted.
 // Each stage draws the random numbers k_bdpt draws at the same point of the walk, so both
@@ -1011,15 +1011,21 @@ __device__ __forceinline__ uint32_t bd_slot_of(uint32_t s, uint32_t t, uint32_t 
   return (t - 1u) * nv - (t - 1u) * (t - 2u) / 2u + s;
 }
 
-// Every strategy of the workgroup's 256 paths, one path per thread: the workgroup walks
-// the strategy slots in order (t outer, s inner) up to its paths' largest vertex counts,
-// a thread taking part where its own subpaths reach (s, t), so the lanes of a wave work
-// on one (s, t) at a time (coalesced vertex loads, one loop length in the MIS weight).
-// Paths j0 .. j0 + m - 1 of the chunk: the record of path j0 + j's slot k at [k * m + j]
-// (w = 1: summed, 2: splatted, 0: none), written for every slot whose s and t lie within
-// the subpaths' vertex counts; a strategy that needs a shadow ray appends it to the
-// shadow queue with its record's index (one atomic per workgroup and slot).
-__global__ __launch_bounds__(SHADE_BLOCK, 3) void k_bd_strat(WaveParams P, BdptArgsHost H, uint32_t j0, uint32_t m) {
+// The strategies of one kind for the workgroup's 256 paths, one path per thread: CAT 0
+// the camera subpath alone (s = 0), 1 a fresh light point (s = 1), 2 the light subpath to
+// the camera (t = 1, s >= 2), 3 the two subpaths joined (s, t >= 2).  Each kind compiles
+// to its own kernel with only its branches of bd_strategy and bd_weight, and so its own
+// register count (80, 128, 105 and 168 VGPRs): one kernel for all kinds ran at the
+// largest.  The workgroup walks its kind's slots in order up to its paths' largest vertex
+// counts, a thread taking part where its own subpaths reach (s, t), so the lanes of a
+// wave work on one (s, t) at a time (coalesced vertex loads, one loop length in the MIS
+// weight).  Paths j0 .. j0 + m - 1 of the chunk: the record of path j0 + j's slot k at
+// [k * m + j] (w = 1: summed, 2: splatted, 0: none), written for every slot of the kind
+// whose s and t lie within the subpaths' vertex counts; a strategy that needs a shadow
+// ray appends it to the shadow queue with its record's index (one atomic per workgroup
+// and slot).
+template <int CAT>
+__global__ __launch_bounds__(SHADE_BLOCK, CAT == 3 ? 3 : 4) void k_bd_strat(WaveParams P, BdptArgsHost H, uint32_t j0, uint32_t m) {
   __shared__ uint32_t sh[SHADE_BLOCK / 64 + 1];
   __shared__ uint32_t smax[2];
   const DevScene& S = P.S;
@@ -1042,9 +1048,12 @@ __global__ __launch_bounds__(SHADE_BLOCK, 3) void k_bd_strat(WaveParams P, BdptA
   const uint32_t mnl = smax[0], mnc = smax[1];
   BdptArgs A;
   A.lv = H.lv; A.cv = H.cv; A.n = H.n; A.nv = H.nv; A.use_t1 = H.use_t1; A.splat = H.splat;
-  for (uint32_t t = 1; t <= mnc; ++t) {
-    const uint32_t s_end = std::min(mnl, H.nv - t);
-    for (uint32_t s = 0; s <= s_end; ++s) {
+  // CAT 0: s = 0; 1: s = 1; 2: t = 1, s >= 2; 3: s, t >= 2
+  const uint32_t t_lo = CAT == 2 ? 1u : 2u, t_hi = CAT == 2 ? std::min(1u, mnc) : mnc;
+  for (uint32_t t = t_lo; t <= t_hi; ++t) {
+    const uint32_t s_lo = CAT == 0 ? 0u : CAT == 1 ? 1u : 2u;
+    const uint32_t s_end = CAT <= 1 ? std::min(s_lo, std::min(mnl, H.nv - t)) : std::min(mnl, H.nv - t);
+    for (uint32_t s = s_lo; s <= s_end; ++s) {
       bool go = false;
       BdStrategy o;
       size_t rk = 0;
@@ -1094,7 +1103,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_bd_resolve(WaveParams P, BdptAr
   V3 sum = v3(0.f, 0.f, 0.f);
   for (uint32_t t = 1; t <= nc; ++t) {
     const uint32_t s_end = std::min(nl, H.nv - t);
-    for (uint32_t s = 0; s <= s_end; ++s) {
+    for (uint32_t s = t == 1u ? 2u : 0u; s <= s_end; ++s) {  // (0, 1) and (1, 1) give nothing
       const size_t rk = (size_t)bd_slot_of(s, t, H.nv) * m + j;
       const float4 r = H.rec[rk];
       if (r.w == 1.f) sum = sum + xyz(r);
@@ -1163,7 +1172,10 @@ hipError_t launch_bdpt_wave(const WaveParams& P, const BdptArgsHost& A, const Ke
     const uint32_t m = std::min(part, P.n_chunk - j0);
     const uint32_t mb = (m + SHADE_BLOCK - 1u) / SHADE_BLOCK;
     if ((e = launch_zero(P.counters, NUM_COUNTERS, st)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_bd_strat, dim3(mb), dim3(SHADE_BLOCK), 0, st, P, A, j0, m);
+    hipLaunchKernelGGL(k_bd_strat<0>, dim3(mb), dim3(SHADE_BLOCK), 0, st, P, A, j0, m);
+    hipLaunchKernelGGL(k_bd_strat<1>, dim3(mb), dim3(SHADE_BLOCK), 0, st, P, A, j0, m);
+    hipLaunchKernelGGL(k_bd_strat<2>, dim3(mb), dim3(SHADE_BLOCK), 0, st, P, A, j0, m);
+    hipLaunchKernelGGL(k_bd_strat<3>, dim3(mb), dim3(SHADE_BLOCK), 0, st, P, A, j0, m);
     if ((e = launch_connect(P, cc, 0, false, st)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_bd_resolve, dim3(mb), dim3(SHADE_BLOCK), 0, st, P, A, j0, m);
     hipLaunchKernelGGL(k_bd_count, dim3(1), dim3(64), 0, st, P, 1u);
