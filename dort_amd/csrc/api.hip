@@ -59,6 +59,7 @@ struct dgpu_ctx {
   // the render's shadow-ray tests run on a second stream of the same (default) priority,
   // each bounce's beside the next bounce's extension rays; created on first use
   hipStream_t stream2 = nullptr;
+  hipStream_t stream3 = nullptr;  // the light tracer's odd stages' camera connections
   std::vector<hipEvent_t> ev_sync;  // ordering events (no timing), reused across renders
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::mutex mu;
@@ -91,6 +92,11 @@ struct dgpu_ctx {
     return stream2;
   }
 
+  hipStream_t third_stream() {
+    if (!stream3 && hipStreamCreateWithFlags(&stream3, hipStreamNonBlocking) != hipSuccess) stream3 = nullptr;
+    return stream3;
+  }
+
   ~dgpu_ctx() {
     (void)hipSetDevice(device);
     for (auto& kv : bufs) if (kv.second.p) (void)hipFree(kv.second.p);
@@ -99,6 +105,7 @@ struct dgpu_ctx {
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     if (stream2) (void)hipStreamDestroy(stream2);
+    if (stream3) (void)hipStreamDestroy(stream3);
     if (stream) (void)hipStreamDestroy(stream);
   }
 
@@ -1007,9 +1014,19 @@ dgpu_status dgpu_render_light(dgpu_scene* s, const dgpu_render_opts* o, float* f
   P.counters = (uint32_t*)ctx->buf("counters", NUM_COUNTERS * 4, &st);
   P.stats = (unsigned long long*)ctx->buf("stats", STATS_WORDS * 8, &st);
   P.spill = (uint32_t*)ctx->buf("spill", (size_t)maxgrid * s->cfg.block * s->stack_cap * 4, &st);
-  uint32_t* spill_c = (uint32_t*)ctx->buf("spill_c", (size_t)cc.grid * s->cfg.block * s->stack_cap * 4, &st);
-  hipStream_t stream_c = ctx->second_stream();
-  if (!stream_c) return fail(DGPU_EDEVICE, "stream creation failed");
+  // Stage s's camera connections use the queue set s % 2 and run on stream_c[s % 2]: a
+  // stage's connections have until the stage after next to finish, and consecutive
+  // stages' connections (few, long rays: latency-bound launches) run side by side.
+  WaveParams Ps[2] = {P, P};
+  Ps[1].so = (float4*)ctx->buf("so_b", (size_t)C * 16, &st);
+  Ps[1].sd = (float4*)ctx->buf("sd_b", (size_t)C * 16, &st);
+  Ps[1].smeta = (uint2*)ctx->buf("smeta_b", (size_t)C * 8, &st);
+  Ps[1].pend = (float4*)ctx->buf("pend_b", (size_t)C * 16, &st);
+  Ps[1].fpos = (float2*)ctx->buf("fpos_b", (size_t)C * 8, &st);
+  uint32_t* spill_c[2] = {(uint32_t*)ctx->buf("spill_c", (size_t)cc.grid * s->cfg.block * s->stack_cap * 4, &st),
+                          (uint32_t*)ctx->buf("spill_d", (size_t)cc.grid * s->cfg.block * s->stack_cap * 4, &st)};
+  hipStream_t stream_c[2] = {ctx->second_stream(), ctx->third_stream()};
+  if (!stream_c[0] || !stream_c[1]) return fail(DGPU_EDEVICE, "stream creation failed");
   P.stack_cap = s->stack_cap;
   bool dev = o->flags & DGPU_BUF_DEVICE;
   P.film_rgb = dev ? film_splat_rgb : (float*)ctx->buf("film_rgb", (size_t)npix * 12, &st);
@@ -1030,39 +1047,56 @@ dgpu_status dgpu_render_light(dgpu_scene* s, const dgpu_render_opts* o, float* f
     next_ev += 2;
     return e;
   };
-  // A stage's camera connections (connect, then splat, on stream_c) need only the stage's
-  // shade, as does the next stage's extension (extend, on stream): the two run side by
-  // side.  The next shade reuses the shadow queue and its pending records, so it waits
-  // for the splat.
-  hipEvent_t ev_shade = ctx->sync_event(0), ev_splat = ctx->sync_event(1);
-  if (!ev_shade || !ev_splat) return fail(DGPU_EDEVICE, "event creation failed");
-  WaveParams Pc = P;
+  // The next stage's extension (extend, on stream) needs only the stage's shade, as do
+  // the stage's camera connections (connect, then splat): they run side by side.  A
+  // shade that writes stage s's connections waits for the splat of stage s - 2, the last
+  // user of the queue set s % 2.
+  hipEvent_t ev_shade[2] = {ctx->sync_event(0), ctx->sync_event(1)};
+  hipEvent_t ev_splat[2] = {ctx->sync_event(2), ctx->sync_event(3)};
+  if (!ev_shade[0] || !ev_shade[1] || !ev_splat[0] || !ev_splat[1])
+    return fail(DGPU_EDEVICE, "event creation failed");
+  WaveParams Pc[2] = {P, P};
   HIP_TRY(hipEventRecord(ctx->ev0, stream));
   bool cancelled = false;
+  bool used[2] = {false, false};  // a set's splat recorded in this render
   auto connect_stage = [&](uint32_t stage) -> hipError_t {
-    hipError_t e = hipEventRecord(ev_shade, stream);
-    if (e == hipSuccess) e = hipStreamWaitEvent(stream_c, ev_shade, 0);
-    if (e == hipSuccess) e = timed(2, stream_c, [&] { return launch_connect(Pc, cc, stage, false, stream_c); });
-    if (e == hipSuccess) e = timed(3, stream_c, [&] { return launch_lt_splat(Pc, stage, stream_c, grid); });
-    if (e == hipSuccess) e = hipEventRecord(ev_splat, stream_c);
+    const uint32_t k = stage & 1u;
+    hipError_t e = hipEventRecord(ev_shade[k], stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(stream_c[k], ev_shade[k], 0);
+    if (e == hipSuccess) e = timed(2, stream_c[k], [&] { return launch_connect(Pc[k], cc, stage, false, stream_c[k]); });
+    if (e == hipSuccess) e = timed(3, stream_c[k], [&] { return launch_lt_splat(Pc[k], stage, stream_c[k], grid); });
+    if (e == hipSuccess) e = hipEventRecord(ev_splat[k], stream_c[k]);
+    used[k] = true;
     return e;
   };
   for (uint64_t g0 = 0; g0 < total && !cancelled; g0 += C) {
-    P.g0 = g0;
-    P.n_chunk = (uint32_t)std::min<uint64_t>(C, total - g0);
-    Pc = P;
-    Pc.spill = spill_c;
+    for (int k = 0; k < 2; ++k) {
+      WaveParams q = P;
+      q.so = Ps[k].so; q.sd = Ps[k].sd; q.smeta = Ps[k].smeta; q.pend = Ps[k].pend; q.fpos = Ps[k].fpos;
+      q.g0 = g0;
+      q.n_chunk = (uint32_t)std::min<uint64_t>(C, total - g0);
+      Ps[k] = q;
+      Pc[k] = q;
+      Pc[k].spill = spill_c[k];
+    }
+    P.g0 = Ps[0].g0;
+    P.n_chunk = Ps[0].n_chunk;
+    // a new chunk zeroes the counters: every connection of the last one first
+    for (int k = 0; k < 2; ++k)
+      if (used[k]) HIP_TRY(hipStreamWaitEvent(stream, ev_splat[k], 0));
     HIP_TRY(launch_zero(P.counters, NUM_COUNTERS, stream));
-    HIP_TRY(timed(1, stream, [&] { return launch_lt_start(P, stream, grid); }));
+    HIP_TRY(timed(1, stream, [&] { return launch_lt_start(Ps[0], stream, grid); }));
     HIP_TRY(connect_stage(0));
     for (uint32_t b = 0; b + 3u <= P.max_len; ++b) {
       if (cancel && *cancel) { cancelled = true; break; }
+      const uint32_t k = (b + 1u) & 1u;  // the set of stage b + 1, which this shade writes
       HIP_TRY(timed(0, stream, [&] { return launch_extend(P, ce, b, false, stream); }));
-      HIP_TRY(hipStreamWaitEvent(stream, ev_splat, 0));
-      HIP_TRY(timed(1, stream, [&] { return launch_lt_shade(P, b, stream, grid); }));
+      if (used[k]) HIP_TRY(hipStreamWaitEvent(stream, ev_splat[k], 0));
+      HIP_TRY(timed(1, stream, [&] { return launch_lt_shade(Ps[k], b, stream, grid); }));
       HIP_TRY(connect_stage(b + 1));
     }
-    HIP_TRY(hipStreamWaitEvent(stream, ev_splat, 0));
+    for (int k = 0; k < 2; ++k)
+      if (used[k]) HIP_TRY(hipStreamWaitEvent(stream, ev_splat[k], 0));
     HIP_TRY(launch_lt_stats(P, stream));
   }
   HIP_TRY(hipEventRecord(ctx->ev1, stream));

@@ -18,9 +18,11 @@
  *  - Ray/hit/film buffers are host or device pointers, chosen per call by
  *    DGPU_BUF_DEVICE in `flags`.  Device pointers must be allocations on the
  *    context's device (e.g. torch tensors on that device).
- *  - Two HIP streams per context, both of default priority (renders test shadow
- *    rays on the second beside the first's extension rays); calls on one context
- *    are serialised by an internal mutex.  Separate contexts are independent.
+ *  - Up to three HIP streams per context, all of default priority (renders test
+ *    shadow rays on the second beside the first's extension rays; the light tracer
+ *    tests alternate stages' camera connections on the second and third); calls on
+ *    one context are serialised by an internal mutex.  Separate contexts are
+ *    independent.
  *  - Rendering is deterministic for a given (scene, opts.seed, sample range):
  *    random numbers are drawn from a counter-based generator keyed by
  *    (seed, pixel, sample, bounce, dimension), never by launch geometry.
