@@ -1275,6 +1275,10 @@ dgpu_status dgpu_render_bdpt(dgpu_scene* s, const dgpu_render_opts* o, uint32_t 
     HIP_TRY(hipMemsetAsync(P.film_w, 0, (size_t)npix * 4, stream));
     HIP_TRY(hipMemsetAsync(A.splat, 0, (size_t)npix * 12, stream));
   }
+  // the wavefront form's strategy kernels run on three streams
+  const hipStream_t side[2] = {ctx->second_stream(), ctx->third_stream()};
+  const hipEvent_t sev[3] = {ctx->sync_event(0), ctx->sync_event(1), ctx->sync_event(2)};
+  if (!side[0] || !side[1] || !sev[0] || !sev[1] || !sev[2]) return fail(DGPU_EDEVICE, "stream/event creation failed");
   HIP_TRY(hipEventRecord(ctx->ev0, stream));
   bool cancelled = false;
   for (uint64_t g0 = 0; g0 < total; g0 += C) {
@@ -1284,7 +1288,7 @@ dgpu_status dgpu_render_bdpt(dgpu_scene* s, const dgpu_render_opts* o, uint32_t 
     P.q0 = (uint32_t)(g0 % npix);
     P.s0 = o->sample_begin + (uint32_t)(g0 / npix);
     if (mega) HIP_TRY(launch_bdpt(P, A, s->S.width, stream));
-    else HIP_TRY(launch_bdpt_wave(P, A, ce, cc, shade_grid, M, stream));
+    else HIP_TRY(launch_bdpt_wave(P, A, ce, cc, shade_grid, M, stream, side, sev));
   }
   HIP_TRY(hipEventRecord(ctx->ev1, stream));
   unsigned long long hstats[STATS_WORDS] = {};

@@ -1155,7 +1155,8 @@ __global__ __launch_bounds__(256) void k_bdpt_film(WaveParams P) {
 }
 
 hipError_t launch_bdpt_wave(const WaveParams& P, const BdptArgsHost& A, const KernelCfg& ce,
-                            const KernelCfg& cc, uint32_t shade_grid, uint32_t part, hipStream_t st) {
+                            const KernelCfg& cc, uint32_t shade_grid, uint32_t part, hipStream_t st,
+                            const hipStream_t side[2], const hipEvent_t ev[3]) {
   const uint32_t blocks = (P.n_chunk + SHADE_BLOCK - 1u) / SHADE_BLOCK;
   const uint32_t grid = std::min(blocks, std::max(1u, shade_grid));
   hipError_t e;
@@ -1172,10 +1173,20 @@ hipError_t launch_bdpt_wave(const WaveParams& P, const BdptArgsHost& A, const Ke
     const uint32_t m = std::min(part, P.n_chunk - j0);
     const uint32_t mb = (m + SHADE_BLOCK - 1u) / SHADE_BLOCK;
     if ((e = launch_zero(P.counters, NUM_COUNTERS, st)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_bd_strat<0>, dim3(mb), dim3(SHADE_BLOCK), 0, st, P, A, j0, m);
-    hipLaunchKernelGGL(k_bd_strat<1>, dim3(mb), dim3(SHADE_BLOCK), 0, st, P, A, j0, m);
-    hipLaunchKernelGGL(k_bd_strat<2>, dim3(mb), dim3(SHADE_BLOCK), 0, st, P, A, j0, m);
+    // the four kinds write disjoint records and append to one queue: the joined subpaths
+    // (the largest kernel, 3 waves per SIMD) on st, the others on the side streams, so
+    // the small kernels' waves fill what the large one leaves
+    if ((e = hipEventRecord(ev[0], st)) != hipSuccess) return e;
+    for (int k = 0; k < 2; ++k)
+      if ((e = hipStreamWaitEvent(side[k], ev[0], 0)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_bd_strat<3>, dim3(mb), dim3(SHADE_BLOCK), 0, st, P, A, j0, m);
+    hipLaunchKernelGGL(k_bd_strat<1>, dim3(mb), dim3(SHADE_BLOCK), 0, side[0], P, A, j0, m);
+    hipLaunchKernelGGL(k_bd_strat<2>, dim3(mb), dim3(SHADE_BLOCK), 0, side[1], P, A, j0, m);
+    hipLaunchKernelGGL(k_bd_strat<0>, dim3(mb), dim3(SHADE_BLOCK), 0, side[1], P, A, j0, m);
+    for (int k = 0; k < 2; ++k) {
+      if ((e = hipEventRecord(ev[1 + k], side[k])) != hipSuccess) return e;
+      if ((e = hipStreamWaitEvent(st, ev[1 + k], 0)) != hipSuccess) return e;
+    }
     if ((e = launch_connect(P, cc, 0, false, st)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_bd_resolve, dim3(mb), dim3(SHADE_BLOCK), 0, st, P, A, j0, m);
     hipLaunchKernelGGL(k_bd_count, dim3(1), dim3(64), 0, st, P, 1u);

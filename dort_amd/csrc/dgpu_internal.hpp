@@ -176,8 +176,11 @@ hipError_t launch_raygen(const WaveParams& P, uint32_t capacity, bool batch, hip
 hipError_t launch_bdpt(const WaveParams& P, const BdptArgsHost& A, uint32_t width, hipStream_t st);
 // The wavefront form of the same chunk: the light and camera subpaths walked by extend and
 // a subpath shade per bounce, the strategies' connections traced by connect.
+// side[0..1]: two more streams and ev[0..2]: three events for the strategy kernels, which
+// run side by side
 hipError_t launch_bdpt_wave(const WaveParams& P, const BdptArgsHost& A, const KernelCfg& ce,
-                            const KernelCfg& cc, uint32_t shade_grid, uint32_t part, hipStream_t st);
+                            const KernelCfg& cc, uint32_t shade_grid, uint32_t part, hipStream_t st,
+                            const hipStream_t side[2], const hipEvent_t ev[3]);
 hipError_t launch_extend(const WaveParams& P, const KernelCfg& cfg, uint32_t bounce, bool batch,
                          hipStream_t st);
 // max_blocks: the shade grid (workgroups of 256 that stride over the queue)

@@ -20,8 +20,9 @@
  *    context's device (e.g. torch tensors on that device).
  *  - Up to three HIP streams per context, all of default priority (renders test
  *    shadow rays on the second beside the first's extension rays; the light tracer
- *    tests alternate stages' camera connections on the second and third); calls on
- *    one context are serialised by an internal mutex.  Separate contexts are
+ *    tests alternate stages' camera connections on the second and third, BDPT runs
+ *    its strategy kernels on all three); calls on one context are serialised by an
+ *    internal mutex.  Separate contexts are
  *    independent.
  *  - Rendering is deterministic for a given (scene, opts.seed, sample range):
  *    random numbers are drawn from a counter-based generator keyed by
