@@ -128,11 +128,13 @@ struct dgpu_ctx {
   // the free device memory, at most 16 GiB of queues and path state.  Bigger chunks
   // amortise the ramp-up and drain of every per-bounce launch (on the 2M-triangle
   // terrain the traversal kernels ran twice as fast at 64M paths as at 2M).
-  uint64_t default_tile(uint32_t slots) {
+  // sets: shadow queues and pending records per path (2 when a bounce's NEE is added two
+  // bounces later), each set with its own 16 GiB of the budget.
+  uint64_t default_tile(uint32_t slots, uint32_t sets = 1) {
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
-    const uint64_t per_path = 120ull + 56ull * slots;  // state, queues, hit, shadow queue
-    uint64_t budget = std::min<uint64_t>(16ull << 30, free_b / 4);
+    const uint64_t per_path = 120ull + 56ull * slots * sets;  // state, queues, hit, shadow queue
+    uint64_t budget = std::min<uint64_t>((16ull << 30) * sets, free_b / 4);
     return std::max<uint64_t>(1ull << 20, budget / per_path);
   }
 
@@ -817,7 +819,7 @@ dgpu_status dgpu_render(dgpu_scene* s, const dgpu_render_opts* o, float* film_rg
   const uint32_t npix = o->x_res * o->y_res;
   const uint64_t total = (uint64_t)npix * (o->sample_end - o->sample_begin);
   uint32_t slots = o->sample_all_lights ? 2 * std::max(1u, s->S.num_lights) : 2;
-  uint64_t tile = s->queue_tile ? s->queue_tile : ctx->default_tile(slots);
+  uint64_t tile = s->queue_tile ? s->queue_tile : ctx->default_tile(slots, 2);
   // q0 + n_chunk must stay below 2^32 (the kernels' 32-bit path positions)
   uint32_t C = (uint32_t)std::min<uint64_t>(std::min<uint64_t>(total ? total : 1, tile),
                                             std::min<uint64_t>(0xFFFFFFFFull / slots, std::max<uint64_t>(1, 0xFFFFFFFFull - npix)));
@@ -854,10 +856,18 @@ dgpu_status dgpu_render(dgpu_scene* s, const dgpu_render_opts* o, float* film_rg
   P.counters = (uint32_t*)ctx->buf("counters", NUM_COUNTERS * 4, &st);
   P.stats = (unsigned long long*)ctx->buf("stats", STATS_WORDS * 8, &st);
   P.spill = (uint32_t*)ctx->buf("spill", (size_t)maxgrid * s->cfg.block * s->stack_cap * 4, &st);
-  // the shadow-ray tests run beside the extension rays: a traversal spill area of their own
-  uint32_t* spill_c = (uint32_t*)ctx->buf("spill_c", (size_t)cc.grid * s->cfg.block * s->stack_cap * 4, &st);
-  hipStream_t stream_c = ctx->second_stream();
-  if (!stream_c) return fail(DGPU_EDEVICE, "stream creation failed");
+  // Bounce b's shadow rays and pending NEE use set b % 2: the odd bounces' set here.  Its
+  // shadow-ray tests run on stream_c[b % 2], beside the next bounces, each with a
+  // traversal spill area of its own; shade adds bounce b's visible NEE at bounce b + 2.
+  P.nee_lag = 2;
+  P.pend_b = (float4*)ctx->buf("pend_b", (size_t)C * slots * 16, &st);
+  float4* so_b = (float4*)ctx->buf("so_b", (size_t)C * slots * 16, &st);
+  float4* sd_b = (float4*)ctx->buf("sd_b", (size_t)C * slots * 16, &st);
+  uint2* smeta_b = (uint2*)ctx->buf("smeta_b", (size_t)C * slots * 8, &st);
+  uint32_t* spill_c[2] = {(uint32_t*)ctx->buf("spill_c", (size_t)cc.grid * s->cfg.block * s->stack_cap * 4, &st),
+                          (uint32_t*)ctx->buf("spill_d", (size_t)cc.grid * s->cfg.block * s->stack_cap * 4, &st)};
+  hipStream_t stream_c[2] = {ctx->second_stream(), ctx->third_stream()};
+  if (!stream_c[0] || !stream_c[1]) return fail(DGPU_EDEVICE, "stream creation failed");
   P.stack_cap = s->stack_cap;
   bool dev = o->flags & DGPU_BUF_DEVICE;
   if (dev) {
@@ -887,36 +897,46 @@ dgpu_status dgpu_render(dgpu_scene* s, const dgpu_render_opts* o, float* film_rg
     next_ev += 2;
     return e;
   };
-  // Bounce b's shadow rays (connect, on stream_c) need only bounce b's shade, and the
-  // next bounce's extension rays (extend, on stream) need the same: the two run side by
-  // side, which overlaps the latency-bound tails of the small late bounces.  Shade b + 1
-  // adds the visible NEE of bounce b, so it waits for connect b.
-  hipEvent_t ev_shade = ctx->sync_event(0), ev_conn = ctx->sync_event(1);
-  if (!ev_shade || !ev_conn) return fail(DGPU_EDEVICE, "event creation failed");
-  WaveParams Pc = P;
-  Pc.spill = spill_c;
+  // Bounce b's shadow rays (connect, on stream_c[b % 2]) need only bounce b's shade, as do
+  // the next bounce's extension rays (extend, on stream).  Shade b + 2, which adds bounce
+  // b's visible NEE and then reuses set b % 2, waits for that connect: a bounce's
+  // shadow-ray test runs beside the next bounce's extension and shade, and next to the
+  // neighbouring bounces' tests, which overlaps the latency-bound ends of the launches.
+  hipEvent_t ev_shade[2] = {ctx->sync_event(0), ctx->sync_event(1)};
+  hipEvent_t ev_conn[2] = {ctx->sync_event(2), ctx->sync_event(3)};
+  if (!ev_shade[0] || !ev_shade[1] || !ev_conn[0] || !ev_conn[1]) return fail(DGPU_EDEVICE, "event creation failed");
+  WaveParams Pb[2], Pc[2];
   HIP_TRY(hipEventRecord(ctx->ev0, stream));
   bool cancelled = false;
+  bool used[2] = {false, false};
   for (uint64_t g0 = 0; g0 < total && !cancelled; g0 += C) {
     P.g0 = g0;
     P.n_chunk = (uint32_t)std::min<uint64_t>(C, total - g0);
     P.q0 = (uint32_t)(g0 % npix);
     P.s0 = o->sample_begin + (uint32_t)(g0 / npix);
-    Pc = P;
-    Pc.spill = spill_c;
+    Pb[0] = P;
+    Pb[1] = P;
+    Pb[1].pend = P.pend_b; Pb[1].so = so_b; Pb[1].sd = sd_b; Pb[1].smeta = smeta_b;
+    for (int k = 0; k < 2; ++k) {
+      Pc[k] = Pb[k];
+      Pc[k].spill = spill_c[k];
+    }
     HIP_TRY(timed(3, stream, [&] { return launch_raygen(P, P.n_chunk, false, stream); }));
     for (uint32_t b = 0; b <= o->max_depth; ++b) {
       if (cancel && *cancel) { cancelled = true; break; }
+      const uint32_t k = b & 1u;
       HIP_TRY(timed(0, stream, [&] { return launch_extend(P, ce, b, false, stream); }));
-      if (b > 0) HIP_TRY(hipStreamWaitEvent(stream, ev_conn, 0));
-      HIP_TRY(timed(1, stream, [&] { return launch_shade(P, P.n_chunk, b, false, stream, shade_grid); }));
-      HIP_TRY(hipEventRecord(ev_shade, stream));
-      HIP_TRY(hipStreamWaitEvent(stream_c, ev_shade, 0));
-      HIP_TRY(timed(2, stream_c, [&] { return launch_connect(Pc, cc, b, false, stream_c); }));
-      HIP_TRY(hipEventRecord(ev_conn, stream_c));
+      if (used[k]) HIP_TRY(hipStreamWaitEvent(stream, ev_conn[k], 0));
+      HIP_TRY(timed(1, stream, [&] { return launch_shade(Pb[k], P.n_chunk, b, false, stream, shade_grid); }));
+      HIP_TRY(hipEventRecord(ev_shade[k], stream));
+      HIP_TRY(hipStreamWaitEvent(stream_c[k], ev_shade[k], 0));
+      HIP_TRY(timed(2, stream_c[k], [&] { return launch_connect(Pc[k], cc, b, false, stream_c[k]); }));
+      HIP_TRY(hipEventRecord(ev_conn[k], stream_c[k]));
+      used[k] = true;
     }
-    // the last connect (or, after a cancel, the last one launched) before what follows
-    HIP_TRY(hipStreamWaitEvent(stream, ev_conn, 0));
+    // every connect launched (after a cancel too) before what follows
+    for (int k = 0; k < 2; ++k)
+      if (used[k]) HIP_TRY(hipStreamWaitEvent(stream, ev_conn[k], 0));
     if (!cancelled) HIP_TRY(timed(3, stream, [&] { return launch_finalize(P, P.n_chunk, false, stream); }));
   }
   HIP_TRY(hipEventRecord(ctx->ev1, stream));

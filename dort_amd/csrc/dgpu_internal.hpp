@@ -96,6 +96,8 @@ struct WaveParams {
   float4* thr;             // throughput rgb + flags
   float4* rad;             // radiance accumulated so far
   float4* pend;            // pending NEE contributions [path * slots + k]; w = 1 while visible
+  float4* pend_b;          // nee_lag 2: the odd bounces' set (finalize reads both)
+  uint32_t nee_lag;        // shade adds a bounce's visible NEE nee_lag bounces later (0 = 1)
   float4* qo[2];           // extension queues: origin + tmin
   float4* qd[2];           //                   direction + tmax
   uint32_t* qpath[2];      //                   path slot

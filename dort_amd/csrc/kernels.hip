@@ -598,19 +598,26 @@ __device__ __forceinline__ void shade_body(const WaveParams& P, const DevScene& 
     dl[2] = P.adj_dldi[3 * (size_t)pixel + 2] * P.adj_inv_spp;
   }
   const bool adj_on = ADJ && adj_l.w != 0.f;
+  // With nee_lag 2 the NEE of bounce b is added at bounce b + 2 (its visibility test runs
+  // beside bounce b + 1), from the set of pending records of b's parity, which P.pend is
+  // for this launch; the first two bounces start their sets.
+  const uint32_t lag = (ADJ || P.nee_lag < 2u) ? 1u : 2u;
   if (active) {
     if (bounce == 0u) {
       // the path starts here (k_raygen writes only the queue)
       thr = v3(1.f, 1.f, 1.f);
       flags = 0u;
-      for (uint32_t k = 2u * (ALL ? P.S.num_lights : 1u); k < SL; ++k)  // slots no light writes
-        P.pend[(size_t)p * SL + k] = make_float4(0.f, 0.f, 0.f, 0.f);
     } else {
       float4 t4 = P.thr[p], r4 = P.rad[p];
       thr = xyz(t4);
       rad = xyz(r4);
       flags = __float_as_uint(t4.w);
-      // NEE of the previous vertex, now that its visibility is known
+    }
+    if (bounce < lag) {
+      for (uint32_t k = 2u * (ALL ? P.S.num_lights : 1u); k < SL; ++k)  // slots no light writes
+        P.pend[(size_t)p * SL + k] = make_float4(0.f, 0.f, 0.f, 0.f);
+    } else {
+      // NEE of the vertex lag bounces back, now that its visibility is known
       for (uint32_t k = 0; k < SL; ++k) {
         size_t s = (size_t)p * SL + k;
         float4 pe = P.pend[s];
@@ -620,10 +627,10 @@ __device__ __forceinline__ void shade_body(const WaveParams& P, const DevScene& 
           if (adj_on) adj_apply(P, P.adj_pend[s], xyz(pe), dl, s_rough);
         }
       }
-      // adjoint: the previous vertex's continuation carried everything the path gathers
-      // from here on: its first-pass radiance less what it has gathered so far
-      if (adj_on) adj_apply(P, P.adj_cont[p], xyz(adj_l) - rad, dl, s_rough);
     }
+    // adjoint: the previous vertex's continuation carried everything the path gathers
+    // from here on: its first-pass radiance less what it has gathered so far
+    if (adj_on && bounce > 0u) adj_apply(P, P.adj_cont[p], xyz(adj_l) - rad, dl, s_rough);
     bounces = flags & F_BOUNCES;
     bool last_delta = (flags & F_DELTA) != 0u;
     float4 d4 = P.qd[cur][i], h4 = P.hit[i];
@@ -689,7 +696,7 @@ __device__ __forceinline__ void shade_body(const WaveParams& P, const DevScene& 
         if (ADJ) P.adj_pend[slot_b] = adj_payload(P, bsdf, texel, g.mat, nb.d, wo);
         if (BATCH) emit_shadow_batch(P, nb, p, 0u, qb);
         else emit_shadow(P, nb, slot_b, qb);
-      } else if (active && bounce == 0u) {
+      } else if (active && bounce < lag) {
         P.pend[slot_b] = make_float4(0.f, 0.f, 0.f, 0.f);
       }
     }
@@ -706,7 +713,7 @@ __device__ __forceinline__ void shade_body(const WaveParams& P, const DevScene& 
         if (ADJ) P.adj_pend[slot_l] = adj_payload(P, bsdf, texel, g.mat, nl.d, wo);
         if (BATCH) emit_shadow_batch(P, nl, p, 1u, ql);
         else emit_shadow(P, nl, slot_l, ql);
-      } else if (active && bounce == 0u) {
+      } else if (active && bounce < lag) {
         P.pend[slot_l] = make_float4(0.f, 0.f, 0.f, 0.f);
       }
     }
@@ -746,7 +753,7 @@ __device__ __forceinline__ void shade_body(const WaveParams& P, const DevScene& 
   }
   if (active) {
     P.thr[p] = make_float4(thr.x, thr.y, thr.z, __uint_as_float(flags));
-    P.rad[p] = make_float4(rad.x, rad.y, rad.z, 0.f);
+    P.rad[p] = make_float4(rad.x, rad.y, rad.z, bounce > 0u ? 1.f : 0.f);  // w: the odd set is started
   }
   uint32_t qn = block_append(&P.counters[CNT_QUEUE + bounce + 1], cont, sh);
   if (cont) {
@@ -910,6 +917,12 @@ __device__ __forceinline__ void film_accumulate(const WaveParams& P, float* film
       float4 pe = P.pend[s];
       if (pe.w != 0.f) rad = rad + xyz(pe);
     }
+    // nee_lag 2: the odd bounces' set too, once the path has reached bounce 1
+    if (P.nee_lag >= 2u && r4.w != 0.f)
+      for (uint32_t k = 0; k < P.slots; ++k) {
+        float4 pe = P.pend_b[(size_t)slot * P.slots + k];
+        if (pe.w != 0.f) rad = rad + xyz(pe);
+      }
     bool ok = isfinite(rad.x) && isfinite(rad.y) && isfinite(rad.z) && rad.x >= 0.f &&
               rad.y >= 0.f && rad.z >= 0.f;
     if (!ok) { ++dropped; continue; }
